@@ -1,0 +1,214 @@
+/*
+ * lpma.h -- C ABI of the MI355X-native batched edge-update store.
+ *
+ * A dynamic directed graph is kept as a packed memory array (PMA) of 64-bit
+ * keys src<<32|dst in HBM: a gapped, sorted slot array with LPMA_KEY_NONE in
+ * the gaps, one row-marker key (r<<32 | 0xFFFFFFFF) per vertex, and u32 row
+ * offsets that point at the marker slots.  Every call that takes device
+ * pointers expects memory on the handle's device; `stream` is a hipStream_t
+ * (NULL = the default stream).  All calls return an lpma_status.
+ *
+ * Ownership: the caller owns every buffer it passes; the library never
+ * writes to input buffers (the reference sorts and compacts the caller's
+ * vectors in place, rpma.cuh:1557, :453).  The device view returned by
+ * lpma_get_view is owned by the handle and stays valid until the next
+ * mutating call on that handle.  One handle = one writer: calls on the same
+ * handle must not overlap.  Errors are returned, never thrown across the ABI;
+ * a failed update leaves the structure unchanged.
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in pkumod/LPMA).
+ */
+#ifndef LPMA_H
+#define LPMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LPMA_KEY_NONE 0xFFFFFFFFFFFFFFFFull /* empty slot, rpma.cuh:38 / gpma.cuh:30 */
+#define LPMA_MARKER_DST 0xFFFFFFFFu         /* dst of a row marker, gpma.cuh:34 (COL_IDX_NONE) */
+
+typedef enum lpma_status {
+    LPMA_OK = 0,
+    LPMA_ERR_INVALID_ARGUMENT = 1, /* null handle/pointer, bad configuration */
+    LPMA_ERR_INVALID_KEY = 2,      /* src or dst >= num_vertices (covers markers and KEY_NONE) */
+    LPMA_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed or the pool budget is exhausted */
+    LPMA_ERR_DEVICE = 4,           /* a HIP runtime call failed */
+    LPMA_ERR_TOO_LARGE = 5,        /* batch or capacity beyond the u32 slot/offset range */
+    LPMA_ERR_NOT_BUILT = 6         /* the device code is not available */
+} lpma_status;
+
+typedef enum lpma_op { LPMA_OP_DELETE = 0, LPMA_OP_INSERT = 1 } lpma_op;
+
+typedef struct lpma lpma_t;
+
+/* Structure configuration.  The reference hard-codes 32-slot segments
+ * (rpma.cuh:1769) and density bounds lower = 0.42*len, upper = len at every
+ * level (rpma.cuh:76-79, :164-165); those are the defaults.  Unequal leaf and
+ * root values interpolate linearly per level (the reference's commented-out
+ * variant, rpma.cuh:144-163). */
+typedef struct lpma_config {
+    uint32_t segment_slots; /* power of two in [2, 1024]; default 32 */
+    double density_lower_leaf;
+    double density_lower_root;
+    double density_upper_leaf;
+    double density_upper_root;
+    int device; /* HIP device ordinal; -1 = current device */
+} lpma_config_t;
+
+/* Device view of the gapped CSR (GPMA::keys / row_offset, gpma.cuh:41-62). */
+typedef struct lpma_view {
+    const uint64_t *keys;        /* capacity slots; LPMA_KEY_NONE in gaps */
+    const uint32_t *row_offsets; /* num_vertices+1; [r+1] = slot of row r's marker, [0] = 0 */
+    uint64_t capacity;           /* slots = segment_slots << tree_height */
+    uint64_t live_keys;          /* edges + markers */
+    uint64_t edges;
+    uint32_t num_vertices;
+    uint32_t segment_slots;
+    uint32_t tree_height;        /* the array holds 2^tree_height segments */
+    uint64_t version;            /* bumped by every successful update */
+} lpma_view_t;
+
+typedef struct lpma_stats {
+    uint64_t updates_calls;
+    uint64_t effective_updates;      /* after dedup and presence resolution */
+    uint64_t windows_per_level[32];  /* windows rebalanced per level */
+    uint64_t slots_per_level[32];    /* slots those windows cover */
+    uint64_t rebuilds;               /* root rebalances and resizes */
+    uint64_t rebuild_slots;
+    uint64_t marker_moves;           /* row-offset writes */
+    uint64_t alg_bytes_wave;         /* algorithmic bytes, 16W + 8k + 4m, per rebalance class */
+    uint64_t alg_bytes_block;
+    uint64_t alg_bytes_large;
+} lpma_stats_t;
+
+/* Per-phase device time (ms) accumulated while profiling is on. */
+typedef struct lpma_profile {
+    double sort_ms;
+    double locate_ms;     /* dedup + locate + compaction + density check */
+    double elect_ms;
+    double rebalance_wave_ms;
+    double rebalance_block_ms;
+    double large_ms;
+    double rebuild_ms;
+    double pyramid_ms;
+    uint64_t launches_wave;
+    uint64_t launches_block;
+} lpma_profile_t;
+
+/* Fills the reference defaults (32 slots, 0.42 / 1.0). */
+void lpma_config_default(lpma_config_t *cfg);
+
+/* init_csr_rpma (rpma.cuh:1807-1824) / init_rpma (rpma.cuh:1763-1795),
+ * init_csr_gpma (gpma.cuh:844-856): empty graph on num_vertices vertices
+ * (1 <= num_vertices < 2^31), one marker key per row.  cfg may be NULL. */
+int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg, void *stream);
+
+/* destroy_rpma (rpma.cuh:1720-1732). */
+int lpma_destroy(lpma_t *h);
+
+/* update_rpma (rpma.cuh:1549-1718) / update_gpma (gpma.cuh:747-816):
+ * apply one batch of n device keys (src<<32|dst) as inserts or deletes.
+ * Duplicates in the batch collapse; inserting a present key and deleting an
+ * absent key are no-ops (handle_del_mod, gpma.cuh:162-175). */
+int lpma_update(lpma_t *h, const uint64_t *d_keys, uint64_t n, int op, void *stream);
+
+/* One batch of inserts and deletes; deletes apply first, then inserts. */
+int lpma_update_mixed(lpma_t *h, const uint64_t *d_ins, uint64_t n_ins, const uint64_t *d_del, uint64_t n_del,
+                      void *stream);
+
+/* Bulk load = one insert batch (significant_insert, gpma.cuh:720-744). */
+int lpma_bulk_load(lpma_t *h, const uint64_t *d_keys, uint64_t n, void *stream);
+
+/* Public members of GPMA / RPMA (gpma.cuh:41-62, rpma.cuh:60-96). */
+int lpma_get_view(const lpma_t *h, lpma_view_t *out);
+
+/* Copies the gapped view (capacity keys, num_vertices+1 row offsets) into
+ * caller buffers; either pointer may be NULL. */
+int lpma_copy_view(lpma_t *h, uint64_t *d_keys, uint32_t *d_row_offsets, void *stream);
+
+/* Parity surface: the live edges (markers excluded) in key order, and
+ * standard CSR row offsets (num_vertices+1; [v] = number of edges with
+ * src < v).  Either pointer may be NULL.  d_edge_keys must hold
+ * view.edges keys. */
+int lpma_export_compacted(lpma_t *h, uint64_t *d_edge_keys, uint32_t *d_row_offsets, void *stream);
+
+/* get_data_rpma (rpma.cuh:1735-1761): all live keys including markers,
+ * view.live_keys entries. */
+int lpma_export_data(lpma_t *h, uint64_t *d_keys, void *stream);
+
+/* to_csr_rpma (rpma.cuh:2249-2295): col_index holds dst, or src at each
+ * row's marker position; row_offset[r+1] = position of row r's marker + 1
+ * (so row_offset[v] = edges(src < v) + v).  col_index needs view.live_keys
+ * entries, row_offset num_vertices+1. */
+int lpma_to_csr(lpma_t *h, uint32_t *d_row_offset, uint32_t *d_col_index, void *stream);
+
+/* edge_query_rpma (rpma.cuh:2167-2190): d_result[i] = 1 when d_keys[i] is a
+ * live edge, else 0. */
+int lpma_edge_query(lpma_t *h, const uint64_t *d_keys, uint64_t n, uint32_t *d_result, void *stream);
+
+/* query_rpma (rpma.cuh:1974-2036) without the time-window filter: d_len[i] =
+ * out-degree of d_srcs[i] (0 for ids >= num_vertices). */
+int lpma_neighbor_count(lpma_t *h, const uint32_t *d_srcs, uint64_t n, uint32_t *d_len, void *stream);
+
+/* Writes the neighbors (dst ids, ascending) of d_srcs[i] at
+ * d_out[d_offsets[i] ...]; d_offsets typically the exclusive scan of
+ * lpma_neighbor_count's lengths. */
+int lpma_neighbor_gather(lpma_t *h, const uint32_t *d_srcs, uint64_t n, const uint32_t *d_offsets, uint32_t *d_out,
+                         void *stream);
+
+/* Counts layout invariant violations on the device (0 = consistent). */
+int lpma_check(lpma_t *h, uint64_t *violations, void *stream);
+
+int lpma_get_stats(lpma_t *h, lpma_stats_t *out);
+int lpma_reset_stats(lpma_t *h);
+
+/* Per-phase HIP-event timing on the update stream (off by default). */
+int lpma_set_profiling(lpma_t *h, int on);
+int lpma_get_profile(lpma_t *h, lpma_profile_t *out);
+
+/* Device pool shared by all handles on a device: cap on bytes held from
+ * hipMalloc (0 = unlimited), and release of cached free blocks. */
+int lpma_set_pool_budget(uint64_t bytes);
+int lpma_pool_trim(void);
+uint64_t lpma_pool_reserved_bytes(void);
+
+const char *lpma_strerror(int code);
+
+/* ---------------------------------------------------------------------------
+ * Input formats (host memory).  Generators use MT19937-64 with rejection
+ * range reduction, so a seed fixes the output on every platform.
+ */
+
+/* `n` distinct uniform edges on V vertices, no self loops, in generation
+ * order (rejection of repeats).  Returns 0 or an lpma_status. */
+int lpma_gen_uniform(uint32_t V, uint64_t n, uint64_t seed, uint64_t *out);
+
+/* `n` distinct R-MAT edges on V = 2^scale vertices with quadrant
+ * probabilities (a, b, c, 1-a-b-c), no self loops, generation order. */
+int lpma_gen_rmat(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *out);
+
+/* Seeded Fisher-Yates shuffle of n keys in place. */
+int lpma_shuffle(uint64_t *keys, uint64_t n, uint64_t seed);
+
+/* n uniform edges on V vertices that are absent from the sorted array
+ * `present` (np keys) and distinct from each other. */
+int lpma_gen_absent(uint32_t V, const uint64_t *present, uint64_t np, uint64_t n, uint64_t seed, uint64_t *out);
+
+/* Edgelist reader (README.md:7-17, test_update_rpma.cu:174-192): first line
+ * "V E", then E lines "src dst".  With out == NULL only V and E are read.
+ * max_edges limits how many edges are read (0 = all). */
+int lpma_read_edgelist(const char *path, uint32_t *V, uint64_t *E, uint64_t max_edges, uint64_t *out);
+
+/* Raw MT19937-64 stream (for known-answer checks). */
+int lpma_mt64_fill(uint64_t seed, uint64_t n, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPMA_H */

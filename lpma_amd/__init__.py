@@ -1,0 +1,404 @@
+"""MI355X-native batched edge-update store (LPMA's update path) -- Python host mirror.
+
+Everything runs in ``liblpma.so`` (hand-written HIP kernels for gfx950 behind
+the C ABI declared in ``include/lpma.h``); this module only binds that ABI with
+ctypes and uses torch for device buffers and streams.  There is no CPU
+fallback: without the built library every call raises.
+
+The reference is a header-only CUDA C++ library with free functions over a
+struct (rpma.cuh, gpma.cuh).  Its names are mirrored here:
+
+    ==========================  =========================================
+    reference (rpma.cuh)        here
+    ==========================  =========================================
+    init_csr_rpma :1807         ``init_csr_rpma(V)`` / ``PMA(V)``
+    update_rpma :1549           ``update_rpma(pma, keys, op)`` / ``PMA.update``
+    get_data_rpma :1735         ``get_data_rpma(pma)`` / ``PMA.get_data``
+    to_csr_rpma :2249           ``to_csr_rpma(pma)`` / ``PMA.to_csr``
+    edge_query_rpma :2167       ``edge_query_rpma(pma, keys)``
+    query_rpma :1974            ``query_rpma(pma, srcs)`` (no time window)
+    destroy_rpma :1720          ``destroy_rpma(pma)`` / ``PMA.close``
+    ==========================  =========================================
+
+Keys are ``src << 32 | dst`` held in int64 tensors (the bit pattern of the
+uint64 key; KEY_NONE reads as -1).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblpma.so")
+
+KEY_NONE = 0xFFFFFFFFFFFFFFFF
+OP_DELETE = 0
+OP_INSERT = 1
+
+_STATUS = {
+    0: "ok",
+    1: "invalid argument",
+    2: "invalid key (src or dst >= num_vertices)",
+    3: "out of device memory or pool budget",
+    4: "HIP runtime error",
+    5: "batch or capacity too large",
+    6: "device code not available",
+}
+
+
+class LpmaError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: lpma status {code} ({_STATUS.get(code, 'unknown')})")
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("segment_slots", ctypes.c_uint32), ("density_lower_leaf", ctypes.c_double),
+                ("density_lower_root", ctypes.c_double), ("density_upper_leaf", ctypes.c_double),
+                ("density_upper_root", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("keys", ctypes.c_void_p), ("row_offsets", ctypes.c_void_p), ("capacity", ctypes.c_uint64),
+                ("live_keys", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("num_vertices", ctypes.c_uint32),
+                ("segment_slots", ctypes.c_uint32), ("tree_height", ctypes.c_uint32), ("version", ctypes.c_uint64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("updates_calls", ctypes.c_uint64), ("effective_updates", ctypes.c_uint64),
+                ("windows_per_level", ctypes.c_uint64 * 32), ("slots_per_level", ctypes.c_uint64 * 32),
+                ("rebuilds", ctypes.c_uint64), ("rebuild_slots", ctypes.c_uint64), ("marker_moves", ctypes.c_uint64),
+                ("alg_bytes_wave", ctypes.c_uint64), ("alg_bytes_block", ctypes.c_uint64),
+                ("alg_bytes_large", ctypes.c_uint64)]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("sort_ms", ctypes.c_double), ("locate_ms", ctypes.c_double), ("elect_ms", ctypes.c_double),
+                ("rebalance_wave_ms", ctypes.c_double), ("rebalance_block_ms", ctypes.c_double),
+                ("large_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double), ("pyramid_ms", ctypes.c_double),
+                ("launches_wave", ctypes.c_uint64), ("launches_block", ctypes.c_uint64)]
+
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+
+# (name, restype, argtypes) for every entry point of include/lpma.h
+SIGNATURES = [
+    ("lpma_config_default", None, [ctypes.POINTER(Config)]),
+    ("lpma_create", _int, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(Config), _vp]),
+    ("lpma_destroy", _int, [_vp]),
+    ("lpma_update", _int, [_vp, _vp, _u64, _int, _vp]),
+    ("lpma_update_mixed", _int, [_vp, _vp, _u64, _vp, _u64, _vp]),
+    ("lpma_bulk_load", _int, [_vp, _vp, _u64, _vp]),
+    ("lpma_get_view", _int, [_vp, ctypes.POINTER(View)]),
+    ("lpma_copy_view", _int, [_vp, _vp, _vp, _vp]),
+    ("lpma_export_compacted", _int, [_vp, _vp, _vp, _vp]),
+    ("lpma_export_data", _int, [_vp, _vp, _vp]),
+    ("lpma_to_csr", _int, [_vp, _vp, _vp, _vp]),
+    ("lpma_edge_query", _int, [_vp, _vp, _u64, _vp, _vp]),
+    ("lpma_neighbor_count", _int, [_vp, _vp, _u64, _vp, _vp]),
+    ("lpma_neighbor_gather", _int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    ("lpma_check", _int, [_vp, ctypes.POINTER(_u64), _vp]),
+    ("lpma_get_stats", _int, [_vp, ctypes.POINTER(Stats)]),
+    ("lpma_reset_stats", _int, [_vp]),
+    ("lpma_set_profiling", _int, [_vp, _int]),
+    ("lpma_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
+    ("lpma_set_pool_budget", _int, [_u64]),
+    ("lpma_pool_trim", _int, []),
+    ("lpma_pool_reserved_bytes", _u64, []),
+    ("lpma_strerror", ctypes.c_char_p, [_int]),
+    ("lpma_gen_uniform", _int, [_u32, _u64, _u64, _vp]),
+    ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
+    ("lpma_shuffle", _int, [_vp, _u64, _u64]),
+    ("lpma_gen_absent", _int, [_u32, _vp, _u64, _u64, _u64, _vp]),
+    ("lpma_read_edgelist", _int, [ctypes.c_char_p, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _u64, _vp]),
+    ("lpma_mt64_fill", _int, [_u64, _u64, _vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load liblpma.so; raises when it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m lpma_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code, what):
+    if code != 0:
+        raise LpmaError(code, what)
+
+
+# ----------------------------------------------------------------- inputs
+
+def _np_u64(n):
+    return np.empty(int(n), dtype=np.uint64)
+
+
+def gen_uniform(V, n, seed):
+    out = _np_u64(n)
+    _check(lib().lpma_gen_uniform(V, n, seed, out.ctypes.data), "gen_uniform")
+    return out
+
+
+def gen_rmat(scale, n, a=0.57, b=0.19, c=0.19, seed=2):
+    out = _np_u64(n)
+    _check(lib().lpma_gen_rmat(scale, n, a, b, c, seed, out.ctypes.data), "gen_rmat")
+    return out
+
+
+def shuffle(keys, seed):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    _check(lib().lpma_shuffle(keys.ctypes.data, keys.size, seed), "shuffle")
+    return keys
+
+
+def gen_absent(V, present_sorted, n, seed):
+    p = np.ascontiguousarray(present_sorted, dtype=np.uint64)
+    out = _np_u64(n)
+    _check(lib().lpma_gen_absent(V, p.ctypes.data, p.size, n, seed, out.ctypes.data), "gen_absent")
+    return out
+
+
+def read_edgelist(path, max_edges=0):
+    V = _u32(0)
+    E = _u64(0)
+    _check(lib().lpma_read_edgelist(path.encode(), ctypes.byref(V), ctypes.byref(E), max_edges, None), "edgelist")
+    out = _np_u64(E.value)
+    _check(lib().lpma_read_edgelist(path.encode(), ctypes.byref(V), ctypes.byref(E), max_edges, out.ctypes.data),
+           "edgelist")
+    return V.value, out
+
+
+def mt64(seed, n):
+    out = _np_u64(n)
+    _check(lib().lpma_mt64_fill(seed, n, out.ctypes.data), "mt64")
+    return out
+
+
+# ------------------------------------------------------------ device side
+
+def _torch():
+    import torch
+    return torch
+
+
+def to_device(keys, device="cuda"):
+    """uint64 numpy keys -> int64 device tensor with the same bits."""
+    torch = _torch()
+    a = np.ascontiguousarray(keys, dtype=np.uint64).view(np.int64)
+    return torch.from_numpy(a).to(device)
+
+
+def to_host_u64(t):
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class PMA:
+    """One packed-memory-array edge store on a GPU (a handle of the C ABI)."""
+
+    def __init__(self, num_vertices, segment_slots=32, density_lower=0.42, density_upper=1.0,
+                 density_lower_leaf=None, density_upper_leaf=None, device=None, stream=None):
+        torch = _torch()
+        L = lib()
+        cfg = Config()
+        L.lpma_config_default(ctypes.byref(cfg))
+        cfg.segment_slots = segment_slots
+        cfg.density_lower_root = density_lower
+        cfg.density_upper_root = density_upper
+        cfg.density_lower_leaf = density_lower if density_lower_leaf is None else density_lower_leaf
+        cfg.density_upper_leaf = density_upper if density_upper_leaf is None else density_upper_leaf
+        if device is None:
+            device = torch.cuda.current_device()
+        cfg.device = int(device)
+        self.device = torch.device("cuda", int(device))
+        self.stream = stream
+        self.num_vertices = int(num_vertices)
+        h = ctypes.c_void_p()
+        _check(L.lpma_create(ctypes.byref(h), self.num_vertices, ctypes.byref(cfg), _stream_ptr(stream)), "create")
+        self._h = h
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lpma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- updates
+    def _dev_keys(self, keys):
+        torch = _torch()
+        if isinstance(keys, np.ndarray) or isinstance(keys, (list, tuple)):
+            keys = to_device(np.asarray(keys, dtype=np.uint64), self.device)
+        if keys.dtype != torch.int64 or not keys.is_cuda:
+            raise TypeError("keys must be an int64 device tensor (bit pattern of src<<32|dst)")
+        return keys.contiguous()
+
+    def update(self, keys, op):
+        k = self._dev_keys(keys)
+        _check(lib().lpma_update(self._h, k.data_ptr(), k.numel(), int(op), _stream_ptr(self.stream)), "update")
+
+    def insert(self, keys):
+        self.update(keys, OP_INSERT)
+
+    def delete(self, keys):
+        self.update(keys, OP_DELETE)
+
+    def update_mixed(self, ins, dels):
+        i = self._dev_keys(ins)
+        d = self._dev_keys(dels)
+        _check(lib().lpma_update_mixed(self._h, i.data_ptr(), i.numel(), d.data_ptr(), d.numel(),
+                                       _stream_ptr(self.stream)), "update_mixed")
+
+    def bulk_load(self, keys):
+        k = self._dev_keys(keys)
+        _check(lib().lpma_bulk_load(self._h, k.data_ptr(), k.numel(), _stream_ptr(self.stream)), "bulk_load")
+
+    # -- views
+    def view(self):
+        v = View()
+        _check(lib().lpma_get_view(self._h, ctypes.byref(v)), "view")
+        return v
+
+    @property
+    def edges(self):
+        return int(self.view().edges)
+
+    def copy_view(self):
+        """(gapped keys, marker row offsets) as new device tensors."""
+        torch = _torch()
+        v = self.view()
+        keys = torch.empty(int(v.capacity), dtype=torch.int64, device=self.device)
+        ro = torch.empty(self.num_vertices + 1, dtype=torch.int32, device=self.device)
+        _check(lib().lpma_copy_view(self._h, keys.data_ptr(), ro.data_ptr(), _stream_ptr(self.stream)), "copy_view")
+        return keys, ro
+
+    def export(self):
+        """(edge keys, standard CSR row offsets) as device tensors -- the parity surface."""
+        torch = _torch()
+        v = self.view()
+        keys = torch.empty(max(int(v.edges), 1), dtype=torch.int64, device=self.device)
+        ro = torch.empty(self.num_vertices + 1, dtype=torch.int32, device=self.device)
+        _check(lib().lpma_export_compacted(self._h, keys.data_ptr(), ro.data_ptr(), _stream_ptr(self.stream)),
+               "export")
+        return keys[: int(v.edges)], ro
+
+    def get_data(self):
+        torch = _torch()
+        v = self.view()
+        keys = torch.empty(int(v.live_keys), dtype=torch.int64, device=self.device)
+        _check(lib().lpma_export_data(self._h, keys.data_ptr(), _stream_ptr(self.stream)), "get_data")
+        return keys
+
+    def to_csr(self):
+        torch = _torch()
+        v = self.view()
+        ro = torch.empty(self.num_vertices + 1, dtype=torch.int32, device=self.device)
+        col = torch.empty(int(v.live_keys), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_to_csr(self._h, ro.data_ptr(), col.data_ptr(), _stream_ptr(self.stream)), "to_csr")
+        return ro, col
+
+    def edge_query(self, keys):
+        torch = _torch()
+        k = self._dev_keys(keys)
+        res = torch.empty(max(k.numel(), 1), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_edge_query(self._h, k.data_ptr(), k.numel(), res.data_ptr(), _stream_ptr(self.stream)),
+               "edge_query")
+        return res[: k.numel()]
+
+    def neighbors(self, srcs):
+        """(offsets[n+1], neighbor dst ids) for a device int32 tensor of vertex ids."""
+        torch = _torch()
+        srcs = srcs.to(self.device, torch.int32).contiguous()
+        n = srcs.numel()
+        ln = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_neighbor_count(self._h, srcs.data_ptr(), n, ln.data_ptr(), _stream_ptr(self.stream)),
+               "neighbor_count")
+        offs = torch.zeros(n + 1, dtype=torch.int64, device=self.device)
+        if n:
+            offs[1:] = torch.cumsum(ln[:n].to(torch.int64), 0)
+        total = int(offs[-1].item())
+        offs32 = offs.to(torch.int32)
+        out = torch.empty(max(total, 1), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_neighbor_gather(self._h, srcs.data_ptr(), n, offs32.data_ptr(), out.data_ptr(),
+                                          _stream_ptr(self.stream)), "neighbor_gather")
+        return offs32, out[:total]
+
+    def check(self):
+        v = _u64(0)
+        _check(lib().lpma_check(self._h, ctypes.byref(v), _stream_ptr(self.stream)), "check")
+        return int(v.value)
+
+    def stats(self):
+        s = Stats()
+        _check(lib().lpma_get_stats(self._h, ctypes.byref(s)), "stats")
+        return s
+
+    def reset_stats(self):
+        _check(lib().lpma_reset_stats(self._h), "reset_stats")
+
+    def set_profiling(self, on=True):
+        _check(lib().lpma_set_profiling(self._h, 1 if on else 0), "set_profiling")
+
+    def profile(self):
+        p = Profile()
+        _check(lib().lpma_get_profile(self._h, ctypes.byref(p)), "profile")
+        return p
+
+
+# ------------------------------------------- reference-named free functions
+
+def init_csr_rpma(num_vertices, **kw):
+    return PMA(num_vertices, **kw)
+
+
+def update_rpma(pma, keys, op=OP_INSERT):
+    pma.update(keys, op)
+
+
+def get_data_rpma(pma):
+    return pma.get_data()
+
+
+def to_csr_rpma(pma):
+    return pma.to_csr()
+
+
+def edge_query_rpma(pma, keys):
+    return pma.edge_query(keys)
+
+
+def query_rpma(pma, srcs):
+    return pma.neighbors(srcs)
+
+
+def destroy_rpma(pma):
+    pma.close()

@@ -1,0 +1,242 @@
+// common.hpp -- shared types, device state and wave64 helpers for the
+// MI355X (gfx950) packed-memory-array edge store.
+//
+// Layout in HBM (one flat logical array, see the reference's logical slot
+// space GET_LINEAR_IDX, rpma.cuh:132-134):
+//   keys[S << H]    u64 slots, KEY_NONE gaps; segment s = slots [s*S, s*S+S),
+//                   its live keys front-packed and sorted.
+//   cnt[nseg]       u32 live keys per segment (level 0 of the count pyramid).
+//   pyr[nseg-1]     u32 live keys per window for levels 1..H
+//                   (level l holds nseg>>l windows at offset nseg-(nseg>>(l-1))).
+//   sep[nseg]       u64 separator per segment: every key of segment s lies in
+//                   [sep[s], sep[s+1]); sep[0] = 0.  A key routes to the last
+//                   segment whose separator is <= key.
+//   owner[2nseg-1]  u32 scratch for window election, all SEG_NONE between batches.
+//   row_off[V+1]    u32, row_off[r+1] = slot of row r's marker key (r<<32|~0u)
+//                   (GPMA convention, gpma.cuh:340-344), row_off[0] = 0.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pma {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef int32_t i32;
+typedef int64_t i64;
+
+static constexpr u64 KEY_NONE = ~0ull;   // empty slot (rpma.cuh:38)
+static constexpr u32 SEG_NONE = 0xFFFFFFFFu;
+static constexpr u32 OP_BIT = 0x80000000u;
+
+// status flags raised on the device during one update
+static constexpr u32 FLAG_INVALID_KEY = 1u;   // src or dst >= V
+static constexpr u32 FLAG_ROOT = 2u;          // a run chose the root window
+static constexpr u32 FLAG_RESIZE = 4u;        // the root window does not fit
+static constexpr u32 FLAG_LARGE = 8u;         // a chosen window exceeds the LDS path
+
+// rebalance work classes
+static constexpr int CLASS_WAVE = 0;   // one wave64 per window
+static constexpr int CLASS_BLOCK = 1;  // one 256-thread workgroup per window
+static constexpr int CLASS_LARGE = 2;  // multi-workgroup global path
+static constexpr int NUM_CLASSES = 3;
+
+// LDS capacities of the two in-LDS classes: E = old live + updates, and the
+// number of segments in the window
+static constexpr u32 WAVE_E_CAP = 256;
+static constexpr u32 WAVE_SEG_CAP = 64;
+static constexpr u32 BLOCK_E_CAP = 2048;
+static constexpr u32 BLOCK_SEG_CAP = 1024;
+
+// device statistics slots (u64 counters)
+enum StatSlot {
+    ST_WINDOWS = 0,        // [0, 32): windows rebalanced per level
+    ST_REBUILD_SLOTS = 32, // slots written by root rebalances and resizes
+    ST_REBUILDS = 33,      // root rebalances and resizes
+    ST_UPDATES = 64,       // effective updates applied by window rebalances
+    ST_MARKERS = 65,       // marker (row offset) writes
+    ST_ALG_BYTES_WAVE = 66,   // algorithmic bytes, per rebalance class
+    ST_ALG_BYTES_BLOCK = 67,
+    ST_ALG_BYTES_LARGE = 68,
+    ST_NUM = 80,
+};
+
+struct Status {
+    u32 flags;
+    u32 n_eff;        // effective updates after dedup / presence check
+    i32 delta;        // inserts - deletes among them
+    u32 nwork[NUM_CLASSES];
+    u32 pad[2];
+};
+
+struct Work {
+    u32 a;    // first segment of the window
+    u32 L;    // level: the window holds 1 << L segments
+    u32 u0;   // compacted update range [u0, u1)
+    u32 u1;
+    u32 C;    // live keys in the window before the batch
+    u32 M;    // live keys after the batch
+};
+
+struct DevState {
+    u64 *keys;
+    u32 *cnt;
+    u32 *pyr;
+    u64 *sep;
+    u32 *owner;
+    u32 *row_off;
+    u32 log2S;
+    u32 H;
+    u32 V;
+    u32 vbits;
+};
+
+struct Density {
+    u64 lower[32];
+    u64 upper[32];
+    u64 root_shrink_below;  // root also fits below lower[H] when m > this
+};
+
+__host__ __device__ __forceinline__ u64 nseg_of(const DevState &d) { return (u64)1 << d.H; }
+
+// offset of level l (1..H) inside pyr
+__host__ __device__ __forceinline__ u64 pyr_off(u64 nseg, u32 l) { return nseg - (nseg >> (l - 1)); }
+
+__host__ __device__ __forceinline__ u64 owner_idx(u64 nseg, u32 l, u64 w) {
+    return l == 0 ? w : nseg + pyr_off(nseg, l) + w;
+}
+
+__device__ __forceinline__ u32 *count_ptr(const DevState &d, u32 l, u64 w) {
+    return l == 0 ? d.cnt + w : d.pyr + pyr_off(nseg_of(d), l) + w;
+}
+
+__host__ __device__ __forceinline__ bool is_marker(u64 k) {
+    return (u32)k == 0xFFFFFFFFu && (u32)(k >> 32) != 0xFFFFFFFFu;
+}
+
+__host__ __device__ __forceinline__ u64 marker_key(u32 r) { return ((u64)r << 32) | 0xFFFFFFFFull; }
+
+// first rank placed in segment j of a window of nsegw = 1<<L segments that
+// holds m keys: ceil(j*m / nsegw) (even spread, rpma.cuh:676-679)
+__host__ __device__ __forceinline__ u64 spread_first(u64 j, u64 m, u32 L) {
+    return (j * m + ((1ull << L) - 1)) >> L;
+}
+
+__device__ __forceinline__ bool window_fits(const Density &dn, u32 l, u32 H, u64 m) {
+    if (l < H) return m >= dn.lower[l] && m <= dn.upper[l];
+    if (m > dn.upper[H]) return false;
+    return m >= dn.lower[H] || m > dn.root_shrink_below;
+}
+
+__device__ __forceinline__ bool lds_class_ok(u64 E, u32 L) {
+    return E <= BLOCK_E_CAP && (1ull << L) <= BLOCK_SEG_CAP;
+}
+
+__device__ __forceinline__ int work_class(u64 E, u32 L) {
+    if (E <= WAVE_E_CAP && (1ull << L) <= WAVE_SEG_CAP) return CLASS_WAVE;
+    if (lds_class_ok(E, L)) return CLASS_BLOCK;
+    return CLASS_LARGE;
+}
+
+// ---------------------------------------------------------------- searches
+
+// first index in [lo, hi) with a[i] >= x
+template <typename T>
+__device__ __forceinline__ u64 lower_bound_dev(const T *a, u64 lo, u64 hi, T x) {
+    while (lo < hi) {
+        u64 mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// route a key: last segment s with sep[s] <= key (sep[0] = 0, nseg = 2^H)
+__device__ __forceinline__ u32 locate_segment(const u64 *__restrict__ sep, u32 H, u64 key) {
+    u32 base = 0;
+    for (u32 step = (1u << H) >> 1; step; step >>= 1)
+        if (sep[base + step] <= key) base += step;
+    return base;
+}
+
+// is `key` among the live (front-packed) keys of segment s
+__device__ __forceinline__ bool segment_holds(const DevState &d, u32 s, u64 key) {
+    const u64 *seg = d.keys + ((u64)s << d.log2S);
+    u32 c = d.cnt[s];
+    u64 i = lower_bound_dev<u64>(seg, 0, c, key);
+    return i < c && seg[i] == key;
+}
+
+// --------------------------------------------------------- wave64 scans
+
+__device__ __forceinline__ u32 lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = (int)lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// exclusive scan across a block of NT threads (NT multiple of 64).
+// `tmp` needs NT/64 + 1 entries of LDS.  Returns the exclusive prefix and
+// writes the block total to `total`.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T &total, T *tmp) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    T inc = wave_incl_scan(v);
+    if (NW == 1) {
+        total = __shfl(inc, 63, 64);
+        return inc - v;
+    }
+    if ((tid & 63) == 63) tmp[w] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        T run = 0;
+        for (int i = 0; i < NW; ++i) { T t = tmp[i]; tmp[i] = run; run += t; }
+        tmp[NW] = run;
+    }
+    __syncthreads();
+    T res = tmp[w] + inc - v;
+    total = tmp[NW];
+    __syncthreads();
+    return res;
+}
+
+// in-place exclusive scan of an LDS array of n entries (n may exceed NT);
+// arr[n] receives the total.  Every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ void block_scan_array(u32 *arr, u32 n, u32 *tmp) {
+    u32 carry = 0;
+    for (u32 base = 0; base < n; base += NT) {
+        u32 i = base + threadIdx.x;
+        u32 v = i < n ? arr[i] : 0;
+        u32 tot;
+        u32 ex = block_excl_scan<NT, u32>(v, tot, tmp);
+        if (i < n) arr[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) arr[n] = carry;
+    __syncthreads();
+}
+
+inline u32 grid_for(u64 n, u32 tpb, u32 cap = 1u << 20) {
+    u64 g = (n + tpb - 1) / tpb;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (u32)g;
+}
+
+}  // namespace pma

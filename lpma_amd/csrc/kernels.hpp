@@ -1,0 +1,56 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (one per phase).
+#pragma once
+#include "common.hpp"
+
+namespace pma {
+
+// sort.hip
+u64 scan_scratch_words(u64 n);
+hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
+u64 sort_scratch_words(u64 n);
+hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, u32 *flags, hipStream_t st);
+u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st);
+
+// update.hip
+struct BatchBufs {
+    u64 *ukey;     // n: decoded keys of the sorted batch
+    u32 *useg;     // n: segment | OP_BIT for effective updates, else SEG_NONE
+    u64 *ckey;     // n: compacted effective updates (sorted)
+    u32 *cseg;     // n: their segments
+    i32 *pd;       // n+1: exclusive prefix of +1 (insert) / -1 (delete)
+    u32 *rlevel;   // n: chosen level per run start
+    i32 *part;     // 2 * nblocks + 2: block partials (count, delta)
+    Work *work[NUM_CLASSES];  // n each
+    Status *status;
+    u64 *stats;
+};
+
+u32 locate_blocks(u64 n);
+hipError_t batch_begin(Status *status, hipStream_t st);
+hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
+hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st);
+hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st);
+hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
+
+// global (multi-workgroup) window path, in place, and the full rebuild
+hipError_t large_window(const DevState &d, const Work &w, const BatchBufs &b, u64 *merged, u32 *gpre,
+                        u32 *scan_scratch, hipStream_t st);
+hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
+                        u32 *scan_scratch, hipStream_t st);
+hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);
+hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
+                         hipStream_t st);
+hipError_t rebuild_pyramid(const DevState &d, hipStream_t st);
+hipError_t make_markers(u64 *out, u32 V, hipStream_t st);
+hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st);
+hipError_t fill_u64(u64 *p, u64 n, u64 v, hipStream_t st);
+
+// views.hip
+hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64 *edge_keys, u32 *row_std,
+                            u64 *all_keys, u32 *csr_row, u32 *csr_col, hipStream_t st);
+hipError_t edge_query(const DevState &d, const u64 *q, u64 n, u32 *res, hipStream_t st);
+hipError_t neighbor_count(const DevState &d, const u32 *srcs, u64 n, u32 *len, hipStream_t st);
+hipError_t neighbor_gather(const DevState &d, const u32 *srcs, u64 n, const u32 *offs, u32 *out, hipStream_t st);
+hipError_t check_invariants(const DevState &d, u32 *err, hipStream_t st);
+
+}  // namespace pma

@@ -1,0 +1,641 @@
+// update.hip -- the batched edge-update path on gfx950.
+//
+// Reference pipeline (update_rpma, rpma.cuh:1549-1718) and what replaces it:
+//   sort (:1557)                      -> sort.hip (packed keys, LDS radix)
+//   locate_leaf_kernel (:253-316)     -> k_locate: dedup + separator search
+//                                        + presence check (handle_del_mod,
+//                                        gpma.cuh:162-175)
+//   remove_if / RLE / scan (:447-511) -> k_locate + k_compact: one block
+//                                        scan of (count, delta) pairs
+//   per-level host loop (:1611-1657)  -> k_density: every run picks the
+//                                        lowest level whose window fits, in
+//                                        one pass, from the count pyramid
+//   window grouping per level          -> k_claim / k_leaders: the maximal
+//                                        chosen windows are elected, no host
+//                                        round trip per level
+//   block_rebalancing_kernel (:765-847) and rebalancing_kernel (:1127-1211)
+//                                     -> k_rebalance_{wave,block} (LDS) and
+//                                        the multi-workgroup global path
+//   resize / rebuild (:1301-1530)     -> rebuild_push + rebuild_spread
+//   row-offset fixup (:652-657, gpma.cuh:340-344) -> fused into the spread.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pma {
+
+static constexpr int LOC_TPB = 256;
+
+u32 locate_blocks(u64 n) { return (u32)((n + LOC_TPB - 1) / LOC_TPB); }
+
+__global__ void k_batch_begin(Status *s) {
+    if (threadIdx.x == 0) {
+        s->flags = 0;
+        s->n_eff = 0;
+        s->delta = 0;
+        for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
+    }
+}
+
+hipError_t batch_begin(Status *status, hipStream_t st) {
+    k_batch_begin<<<1, 64, 0, st>>>(status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ dedup, locate, presence
+
+__global__ __launch_bounds__(LOC_TPB) void k_locate(DevState d, const u64 *__restrict__ pk, u64 n,
+                                                    u64 *__restrict__ ukey, u32 *__restrict__ useg,
+                                                    i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
+    u64 i = (u64)blockIdx.x * LOC_TPB + threadIdx.x;
+    i32 eff = 0, dl = 0;
+    if (i < n) {
+        u64 p = pk[i];
+        u64 kk = p >> 1;
+        u32 op = (u32)(p & 1);
+        u32 sg = SEG_NONE;
+        bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == kk);
+        if (!dup) {
+            u32 vmask = (u32)((1ull << d.vbits) - 1);
+            u64 key = ((kk >> d.vbits) << 32) | (u64)((u32)kk & vmask);
+            u32 s = locate_segment(d.sep, d.H, key);
+            u32 present = segment_holds(d, s, key) ? 1u : 0u;
+            if (op != present) {
+                eff = 1;
+                dl = op ? 1 : -1;
+                sg = s | (op ? OP_BIT : 0u);
+            }
+            ukey[i] = key;
+        }
+        useg[i] = sg;
+    }
+    __shared__ i32 tmp[2][LOC_TPB / 64];
+    i32 ce = wave_sum(eff), cd = wave_sum(dl);
+    if (lane_id() == 0) { tmp[0][threadIdx.x >> 6] = ce; tmp[1][threadIdx.x >> 6] = cd; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        i32 a = 0, b = 0;
+        for (int w = 0; w < LOC_TPB / 64; ++w) { a += tmp[0][w]; b += tmp[1][w]; }
+        part_cnt[blockIdx.x] = a;
+        part_dl[blockIdx.x] = b;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_parts(i32 *__restrict__ pc, i32 *__restrict__ pdl, u32 nb,
+                                                     Status *__restrict__ status, i32 *__restrict__ pd) {
+    __shared__ i32 tmp[1024 / 64 + 1];
+    i32 carry_c = 0, carry_d = 0;
+    for (u32 b = 0; b < nb; b += 1024) {
+        u32 i = b + threadIdx.x;
+        i32 vc = i < nb ? pc[i] : 0;
+        i32 vd = i < nb ? pdl[i] : 0;
+        i32 tc, td;
+        i32 ec = block_excl_scan<1024, i32>(vc, tc, tmp);
+        i32 ed = block_excl_scan<1024, i32>(vd, td, tmp);
+        if (i < nb) { pc[i] = carry_c + ec; pdl[i] = carry_d + ed; }
+        carry_c += tc;
+        carry_d += td;
+    }
+    if (threadIdx.x == 0) {
+        status->n_eff = (u32)carry_c;
+        status->delta = carry_d;
+        pd[carry_c] = carry_d;
+    }
+}
+
+__global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u32 *__restrict__ useg, u64 n,
+                                                     const i32 *__restrict__ pc, const i32 *__restrict__ pdl,
+                                                     u64 *__restrict__ ckey, u32 *__restrict__ cseg,
+                                                     i32 *__restrict__ pd) {
+    u64 i = (u64)blockIdx.x * LOC_TPB + threadIdx.x;
+    u32 sg = i < n ? useg[i] : SEG_NONE;
+    i32 eff = sg != SEG_NONE ? 1 : 0;
+    i32 dl = eff ? ((sg & OP_BIT) ? 1 : -1) : 0;
+    __shared__ i32 tmp[LOC_TPB / 64 + 1];
+    i32 t1, t2;
+    i32 ex = block_excl_scan<LOC_TPB, i32>(eff, t1, tmp);
+    i32 dx = block_excl_scan<LOC_TPB, i32>(dl, t2, tmp);
+    if (eff) {
+        u64 j = (u64)pc[blockIdx.x] + ex;
+        ckey[j] = ukey[i];
+        cseg[j] = sg & ~OP_BIT;
+        pd[j] = pdl[blockIdx.x] + dx;
+    }
+}
+
+hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st) {
+    u32 nb = locate_blocks(n);
+    i32 *pc = b.part, *pdl = b.part + nb + 1;
+    k_locate<<<nb, LOC_TPB, 0, st>>>(d, pk, n, b.ukey, b.useg, pc, pdl);
+    k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
+    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, pc, pdl, b.ckey, b.cseg, b.pd);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ density check
+
+// first index >= start with a[idx] >= target, given a[start-1] < target
+__device__ __forceinline__ u32 gallop_fwd(const u32 *__restrict__ a, u32 start, u32 n, u64 target) {
+    u32 lo = start;
+    if (lo >= n || a[lo] >= target) return lo;
+    u32 step = 1;
+    u32 hi = lo + 1;
+    while (hi < n && a[hi] < target) {
+        lo = hi;
+        step <<= 1;
+        hi = (n - lo > step) ? lo + step : n;
+    }
+    // a[lo] < target, hi == n or a[hi] >= target
+    u32 l = lo + 1, h = hi;
+    while (l < h) {
+        u32 m = (l + h) >> 1;
+        if (a[m] < target) l = m + 1; else h = m;
+    }
+    return l;
+}
+
+// smallest index <= start whose suffix [idx, start] is >= target, given a[start] >= target
+__device__ __forceinline__ u32 gallop_back(const u32 *__restrict__ a, u32 start, u64 target) {
+    if (start == 0 || a[start - 1] < target) return start;
+    u32 hi = start - 1;  // a[hi] >= target
+    u32 step = 1;
+    u32 lo;
+    while (true) {
+        lo = hi >= step ? hi - step : 0;
+        if (a[lo] < target) break;
+        if (lo == 0) return 0;
+        hi = lo;
+        step <<= 1;
+    }
+    // a[lo] < target <= a[hi]
+    u32 l = lo + 1, h = hi;
+    while (l < h) {
+        u32 m = (l + h) >> 1;
+        if (a[m] < target) l = m + 1; else h = m;
+    }
+    return l;
+}
+
+__global__ void k_density(DevState d, Density dn, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+                          const Status *__restrict__ status_in, u32 *__restrict__ rlevel, Status *status) {
+    u32 n_eff = status_in->n_eff;
+    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    u32 local_flags = 0;
+    if (j < n_eff) {
+        u32 s = cseg[j];
+        if (j == 0 || cseg[j - 1] != s) {
+            u32 u0 = j, u1 = j + 1;
+            u32 l = 0;
+            u64 C = 0;
+            for (; l <= d.H; ++l) {
+                u64 w = (u64)s >> l;
+                u64 a = w << l;
+                u64 b = a + (1ull << l);
+                u0 = gallop_back(cseg, u0, a);
+                u1 = gallop_fwd(cseg, u1, n_eff, b);
+                C = *count_ptr(d, l, w);
+                i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
+                if (window_fits(dn, l, d.H, (u64)m)) break;
+            }
+            rlevel[j] = l;
+            if (l >= d.H) local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
+            else if (work_class(C + (u1 - u0), l) == CLASS_LARGE) local_flags = FLAG_LARGE;
+        }
+    }
+    // one atomic per wave
+    u32 f = local_flags;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
+    if (lane_id() == 0 && f) atomicOr(&status->flags, f);
+}
+
+hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st) {
+    k_density<<<grid_for(n, 256), 256, 0, st>>>(d, dn, b.cseg, b.pd, b.status, b.rlevel, b.status);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- window election
+
+__global__ void k_claim(DevState d, const u32 *__restrict__ cseg, const u32 *__restrict__ rlevel, u32 n_eff) {
+    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_eff) return;
+    u32 s = cseg[j];
+    if (j > 0 && cseg[j - 1] == s) return;
+    u32 L = rlevel[j];
+    atomicMin(&d.owner[owner_idx(nseg_of(d), L, (u64)s >> L)], j);
+}
+
+__global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+                          const u32 *__restrict__ rlevel, u32 n_eff, Work *w0, Work *w1, Work *w2,
+                          Status *status) {
+    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_eff) return;
+    u32 s = cseg[j];
+    if (j > 0 && cseg[j - 1] == s) return;
+    u32 L = rlevel[j];
+    u64 nseg = nseg_of(d);
+    u64 w = (u64)s >> L;
+    if (d.owner[owner_idx(nseg, L, w)] != j) return;
+    for (u32 l2 = L + 1; l2 < d.H; ++l2)
+        if (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] != SEG_NONE) return;  // inside a larger window
+    u64 a = w << L;
+    u32 u0 = gallop_back(cseg, j, a);
+    u32 u1 = gallop_fwd(cseg, j + 1, n_eff, a + (1ull << L));
+    u32 C = *count_ptr(d, L, w);
+    Work wk;
+    wk.a = (u32)a;
+    wk.L = L;
+    wk.u0 = u0;
+    wk.u1 = u1;
+    wk.C = C;
+    wk.M = (u32)((i32)C + pd[u1] - pd[u0]);
+    int cls = work_class((u64)C + (u1 - u0), L);
+    u32 idx = atomicAdd(&status->nwork[cls], 1u);
+    Work *dst = cls == CLASS_WAVE ? w0 : (cls == CLASS_BLOCK ? w1 : w2);
+    dst[idx] = wk;
+}
+
+__global__ void k_reset_owner(DevState d, const u32 *__restrict__ cseg, const u32 *__restrict__ rlevel, u32 n_eff) {
+    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_eff) return;
+    u32 s = cseg[j];
+    if (j > 0 && cseg[j - 1] == s) return;
+    u32 L = rlevel[j];
+    d.owner[owner_idx(nseg_of(d), L, (u64)s >> L)] = SEG_NONE;
+}
+
+hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st) {
+    u32 g = grid_for(n_eff, 256);
+    k_claim<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
+    k_leaders<<<g, 256, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.status);
+    k_reset_owner<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- LDS window rebalance
+//
+// One group of NT threads rebalances one window entirely in LDS:
+//   1. prefix of the window's segment counts,
+//   2. coalesced load of the live keys (compacted) and of the updates,
+//   3. merge by rank: each old key / insert finds its output rank with one
+//      binary search over the other list (deleted keys drop out),
+//   4. coalesced store of every slot of the window after the even spread
+//      (rpma.cuh:676-679), KEY_NONE in the gaps, marker slots into row_off,
+//   5. per-segment counts and separators.
+
+struct BlockStats {
+    u64 updates, markers, bytes;
+    u32 *windows;  // LDS, per level
+};
+
+template <int NT, u32 ECAP, u32 SEGCAP>
+__device__ void rebalance_window(const DevState &d, const Work &wk, const u64 *__restrict__ ckey,
+                                 const i32 *__restrict__ pd, u64 *A, u64 *B, i32 *P, u32 *pre, u32 *stmp,
+                                 BlockStats &bs) {
+    const u32 tid = threadIdx.x;
+    const u32 L = wk.L;
+    const u64 a = wk.a;
+    const u32 nsegw = 1u << L;
+    const u32 C = wk.C;
+    const u32 k = wk.u1 - wk.u0;
+    const u32 log2S = d.log2S;
+    const u32 Smask = (1u << log2S) - 1;
+    const u64 base = a << log2S;
+    const u32 W = nsegw << log2S;
+
+    for (u32 x = tid; x < nsegw; x += NT) pre[x] = d.cnt[a + x];
+    __syncthreads();
+    block_scan_array<NT>(pre, nsegw, stmp);
+
+    for (u32 t = tid; t < W; t += NT) {
+        u32 seg = t >> log2S, lane = t & Smask;
+        u32 p0 = pre[seg];
+        if (lane < pre[seg + 1] - p0) A[p0 + lane] = d.keys[base + t];
+    }
+    for (u32 q = tid; q < k; q += NT) A[C + q] = ckey[wk.u0 + q];
+    const i32 pbase = pd[wk.u0];
+    for (u32 q = tid; q <= k; q += NT) P[q] = pd[wk.u0 + q] - pbase;
+    __syncthreads();
+
+    const u32 M = (u32)((i32)C + P[k]);
+    const u64 *U = A + C;
+    for (u32 r = tid; r < C; r += NT) {
+        u64 x = A[r];
+        u32 lo = 0, hi = k;
+        while (lo < hi) {
+            u32 m = (lo + hi) >> 1;
+            if (U[m] < x) lo = m + 1; else hi = m;
+        }
+        if (lo < k && U[lo] == x) continue;  // deleted
+        B[r + P[lo]] = x;
+    }
+    for (u32 q = tid; q < k; q += NT) {
+        if (P[q + 1] - P[q] != 1) continue;  // deletes produce nothing
+        u64 y = U[q];
+        u32 lo = 0, hi = C;
+        while (lo < hi) {
+            u32 m = (lo + hi) >> 1;
+            if (A[m] < y) lo = m + 1; else hi = m;
+        }
+        B[lo + P[q]] = y;
+    }
+    __syncthreads();
+
+    u32 markers = 0;
+    for (u32 t = tid; t < W; t += NT) {
+        u32 seg = t >> log2S, lane = t & Smask;
+        u64 f = spread_first(seg, M, L);
+        u64 g = spread_first(seg + 1, M, L);
+        u64 key = (lane < g - f) ? B[f + lane] : KEY_NONE;
+        d.keys[base + t] = key;
+        if (is_marker(key) && (u32)(key >> 32) < d.V) {
+            d.row_off[(u32)(key >> 32) + 1] = (u32)(base + t);
+            ++markers;
+        }
+    }
+    const u64 sep_a = d.sep[a];
+    for (u32 x = tid; x < nsegw; x += NT) {
+        u64 f = spread_first(x, M, L);
+        u64 g = spread_first(x + 1, M, L);
+        d.cnt[a + x] = (u32)(g - f);
+        if (x > 0) d.sep[a + x] = (g > f) ? B[f] : (f > 0 ? B[f - 1] + 1 : sep_a);
+    }
+    markers = wave_sum(markers);
+    if (lane_id() == 0) bs.markers += markers;
+    if (tid == 0) {
+        bs.windows[L] += 1;
+        bs.updates += k;
+        bs.bytes += 16ull * W + 8ull * k;
+    }
+    __syncthreads();
+}
+
+__device__ void flush_stats(BlockStats &bs, u64 *stats, int cls_slot, u32 tid) {
+    // markers were accumulated by lane 0 of every wave
+    if ((tid & 63) == 0 && bs.markers) {
+        atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)bs.markers);
+        atomicAdd((unsigned long long *)&stats[cls_slot], (unsigned long long)(4ull * bs.markers));
+    }
+    if (tid == 0) {
+        for (int l = 0; l < 32; ++l)
+            if (bs.windows[l]) atomicAdd((unsigned long long *)&stats[ST_WINDOWS + l], (unsigned long long)bs.windows[l]);
+        if (bs.bytes) {
+            atomicAdd((unsigned long long *)&stats[ST_UPDATES], (unsigned long long)bs.updates);
+            atomicAdd((unsigned long long *)&stats[cls_slot], (unsigned long long)bs.bytes);
+        }
+    }
+}
+
+template <int NT, u32 ECAP, u32 SEGCAP>
+__device__ void rebalance_loop(const DevState &d, const Work *__restrict__ work, const u32 *nwork,
+                               const u64 *__restrict__ ckey, const i32 *__restrict__ pd, u64 *stats, int cls_slot) {
+    __shared__ u64 A[ECAP];
+    __shared__ u64 B[ECAP];
+    __shared__ i32 P[ECAP + 1];
+    __shared__ u32 pre[SEGCAP + 1];
+    __shared__ u32 stmp[NT / 64 + 1];
+    __shared__ u32 wl[32];
+    BlockStats bs;
+    bs.updates = bs.markers = bs.bytes = 0;
+    bs.windows = wl;
+    if (threadIdx.x < 32) wl[threadIdx.x] = 0;
+    __syncthreads();
+    const u32 nw = *nwork;
+    for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
+        Work wk = work[i];
+        rebalance_window<NT, ECAP, SEGCAP>(d, wk, ckey, pd, A, B, P, pre, stmp, bs);
+    }
+    __syncthreads();
+    flush_stats(bs, stats, cls_slot, threadIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_rebalance_wave(DevState d, const Work *__restrict__ work,
+                                                       const Status *__restrict__ status,
+                                                       const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
+                                                       u64 *stats) {
+    rebalance_loop<64, WAVE_E_CAP, WAVE_SEG_CAP>(d, work, &status->nwork[CLASS_WAVE], ckey, pd, stats,
+                                                 ST_ALG_BYTES_WAVE);
+}
+
+__global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work *__restrict__ work,
+                                                         const Status *__restrict__ status,
+                                                         const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
+                                                         u64 *stats) {
+    rebalance_loop<256, BLOCK_E_CAP, BLOCK_SEG_CAP>(d, work, &status->nwork[CLASS_BLOCK], ckey, pd, stats,
+                                                    ST_ALG_BYTES_BLOCK);
+}
+
+hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
+    if (cls == CLASS_WAVE)
+        k_rebalance_wave<<<grid, 64, 0, st>>>(d, b.work[CLASS_WAVE], b.status, b.ckey, b.pd, b.stats);
+    else
+        k_rebalance_block<<<grid, 256, 0, st>>>(d, b.work[CLASS_BLOCK], b.status, b.ckey, b.pd, b.stats);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------- global window path
+//
+// Windows too large for one workgroup's LDS, the root window, and resizes.
+// Push: every live key of the source window and every insert computes its
+// merged rank and writes it to a scratch array.  Pull: every destination
+// slot reads the key the even spread assigns to it.  Source and destination
+// are the same window (in place) or the whole old and new arrays (resize).
+
+__global__ void k_push_old(DevState s, u64 a, u32 L, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
+                           const i32 *__restrict__ pd, u32 u0, u32 u1, u64 *__restrict__ merged, u64 m) {
+    const u64 W = (1ull << L) << s.log2S;
+    const u64 base = a << s.log2S;
+    const u32 Smask = (1u << s.log2S) - 1;
+    const i32 pbase = pd[u0];
+    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < W; t += (u64)gridDim.x * blockDim.x) {
+        u64 seg = t >> s.log2S;
+        u32 lane = (u32)t & Smask;
+        if (lane >= s.cnt[a + seg]) continue;
+        u64 x = s.keys[base + t];
+        u64 r = (u64)gpre[seg] + lane;
+        u64 u = lower_bound_dev<u64>(ckey, u0, u1, x);
+        if (u < u1 && ckey[u] == x) continue;
+        u64 o = r + (i64)(pd[u] - pbase);
+        if (o < m) merged[o] = x;
+    }
+}
+
+__global__ void k_push_ins(DevState s, u64 a, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
+                           const u32 *__restrict__ cseg, const i32 *__restrict__ pd, u32 u0, u32 u1,
+                           u64 *__restrict__ merged, u64 m) {
+    const i32 pbase = pd[u0];
+    for (u64 q = (u64)u0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; q < u1; q += (u64)gridDim.x * blockDim.x) {
+        if (pd[q + 1] - pd[q] != 1) continue;
+        u64 y = ckey[q];
+        u64 seg = cseg[q];
+        const u64 *sk = s.keys + (seg << s.log2S);
+        u64 o = (u64)gpre[seg - a] + lower_bound_dev<u64>(sk, 0, s.cnt[seg], y) + (i64)(pd[q] - pbase);
+        if (o < m) merged[o] = y;
+    }
+}
+
+__global__ void k_pull(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ merged, u64 *stats, int cls_slot) {
+    const u64 W = (1ull << L) << t.log2S;
+    const u64 base = a << t.log2S;
+    const u32 Smask = (1u << t.log2S) - 1;
+    u32 markers = 0;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < W; i += (u64)gridDim.x * blockDim.x) {
+        u64 seg = i >> t.log2S;
+        u32 lane = (u32)i & Smask;
+        u64 f = spread_first(seg, m, L);
+        u64 g = spread_first(seg + 1, m, L);
+        u64 key = (lane < g - f) ? merged[f + lane] : KEY_NONE;
+        t.keys[base + i] = key;
+        if (is_marker(key) && (u32)(key >> 32) < t.V) {
+            t.row_off[(u32)(key >> 32) + 1] = (u32)(base + i);
+            ++markers;
+        }
+    }
+    markers = wave_sum(markers);
+    if (lane_id() == 0 && markers && stats) {
+        atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)markers);
+        atomicAdd((unsigned long long *)&stats[cls_slot], (unsigned long long)(4ull * markers));
+    }
+}
+
+__global__ void k_meta(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ merged) {
+    const u64 nsegw = 1ull << L;
+    const u64 sep_a = (a == 0) ? 0 : t.sep[a];
+    for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < nsegw; x += (u64)gridDim.x * blockDim.x) {
+        u64 f = spread_first(x, m, L);
+        u64 g = spread_first(x + 1, m, L);
+        t.cnt[a + x] = (u32)(g - f);
+        if (x == 0) {
+            if (a == 0) t.sep[0] = 0;
+        } else {
+            t.sep[a + x] = (g > f) ? merged[f] : (f > 0 ? merged[f - 1] + 1 : sep_a);
+        }
+    }
+}
+
+hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
+                         hipStream_t st);
+
+hipError_t large_window(const DevState &d, const Work &w, const BatchBufs &b, u64 *merged, u32 *gpre,
+                        u32 *scan_scratch, hipStream_t st) {
+    u64 nsegw = 1ull << w.L;
+    u64 W = nsegw << d.log2S;
+    device_excl_scan(d.cnt + w.a, nsegw, gpre, scan_scratch, st);
+    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(d, w.a, w.L, gpre, b.ckey, b.pd, w.u0, w.u1, merged, w.M);
+    if (w.u1 > w.u0)
+        k_push_ins<<<grid_for(w.u1 - w.u0, 256, 16384), 256, 0, st>>>(d, w.a, gpre, b.ckey, b.cseg, b.pd, w.u0, w.u1,
+                                                                       merged, w.M);
+    return window_spread(d, w.a, w.L, merged, w.M, b.stats, ST_ALG_BYTES_LARGE, st);
+}
+
+// full push of all live keys and all effective updates (root / resize)
+hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
+                        u32 *scan_scratch, hipStream_t st) {
+    u64 nseg = nseg_of(src);
+    u64 W = nseg << src.log2S;
+    device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
+    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, 0, (u32)n_eff, merged, m);
+    if (n_eff)
+        k_push_ins<<<grid_for(n_eff, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, 0, (u32)n_eff,
+                                                                merged, m);
+    return hipGetLastError();
+}
+
+// spread m merged keys over the window (a, L) of t
+hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
+                         hipStream_t st) {
+    u64 nsegw = 1ull << L;
+    u64 W = nsegw << t.log2S;
+    k_pull<<<grid_for(W, 256, 16384), 256, 0, st>>>(t, a, L, m, merged, stats, cls_slot);
+    k_meta<<<grid_for(nsegw, 256, 16384), 256, 0, st>>>(t, a, L, m, merged);
+    return hipGetLastError();
+}
+
+hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st) {
+    return window_spread(dst, 0, dst.H, merged, m, nullptr, ST_ALG_BYTES_LARGE, st);
+}
+
+// ------------------------------------------------------------- pyramid
+//
+// Window counts for levels 1..H are rebuilt from the segment counts after
+// every batch that changed them: one workgroup reduces a chunk of 2^11
+// segments through LDS (levels 1..11), a single workgroup finishes the
+// levels above.  O(nseg) streaming work, no atomics.
+
+static constexpr u32 PYR_CHL = 11;
+static constexpr u32 PYR_CH = 1u << PYR_CHL;
+
+__global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) {
+    __shared__ u32 v0[PYR_CH];
+    __shared__ u32 v1[PYR_CH / 2];
+    const u64 nseg = nseg_of(d);
+    const u32 levels = d.H < PYR_CHL ? d.H : PYR_CHL;
+    const u32 n = (u32)(nseg < PYR_CH ? nseg : PYR_CH);
+    const u64 c0 = (u64)blockIdx.x * n;
+    for (u32 i = threadIdx.x; i < n; i += 256) v0[i] = d.cnt[c0 + i];
+    __syncthreads();
+    u32 *in = v0, *tmp = v1;
+    for (u32 l = 1; l <= levels; ++l) {
+        u32 nw = n >> l;
+        u32 *out = d.pyr + pyr_off(nseg, l) + (c0 >> l);
+        for (u32 x = threadIdx.x; x < nw; x += 256) {
+            u32 s = in[2 * x] + in[2 * x + 1];
+            tmp[x] = s;
+            out[x] = s;
+        }
+        __syncthreads();
+        u32 *sw = in; in = tmp; tmp = sw;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_pyr_top(DevState d) {
+    const u64 nseg = nseg_of(d);
+    for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
+        u64 nw = nseg >> l;
+        const u32 *in = d.pyr + pyr_off(nseg, l - 1);
+        u32 *out = d.pyr + pyr_off(nseg, l);
+        for (u64 x = threadIdx.x; x < nw; x += 1024) out[x] = in[2 * x] + in[2 * x + 1];
+        __syncthreads();
+    }
+}
+
+hipError_t rebuild_pyramid(const DevState &d, hipStream_t st) {
+    if (d.H == 0) return hipSuccess;
+    u64 nseg = nseg_of(d);
+    u32 nb = (u32)(nseg < PYR_CH ? 1 : nseg / PYR_CH);
+    k_pyr_chunk<<<nb, 256, 0, st>>>(d);
+    if (d.H > PYR_CHL) k_pyr_top<<<1, 1024, 0, st>>>(d);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- fills
+
+__global__ void k_markers(u64 *out, u32 V) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (u64)gridDim.x * blockDim.x)
+        out[i] = marker_key((u32)i);
+}
+
+hipError_t make_markers(u64 *out, u32 V, hipStream_t st) {
+    k_markers<<<grid_for(V, 256, 8192), 256, 0, st>>>(out, V);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(u32 *p, u64 n, u32 v) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(u64 *p, u64 n, u64 v) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill_u32<<<grid_for(n, 256, 8192), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(u64 *p, u64 n, u64 v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill_u64<<<grid_for(n, 256, 8192), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace pma
