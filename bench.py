@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched edge-update path on one MI355X.
+
+Workload (the headline configuration): R-MAT graph, 2^20 vertices,
+2^24 distinct edges (a,b,c,d = 0.57,0.19,0.19,0.05, seed 2, self loops and
+repeats rejected), shuffled with seed 12; the first half is bulk-loaded, the
+second half is streamed as insert batches of B edges, then the same volume of
+live edges (a seeded uniform sample without replacement) is deleted in
+batches of B.  A step is one update batch; the stream is cyclic (deleted
+edges are re-inserted) when more steps are asked for than it holds.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+
+Prints one JSON line.  `value` is edge updates per second over the K timed
+batches (inputs resident in HBM, wall clock between device synchronisations).
+`roofline` describes the dominant rebalance kernel (algorithmic bytes per
+launch / average launch time, from HIP events the library records around
+each launch on the update stream in a second, profiled pass).
+`cpu_baseline` times the CPU oracle (sorted merge + row offsets, pthreads)
+on a bounded sample of the same batches.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(scale, edges, seed, shuffle_seed, B, del_seed):
+    import lpma_amd as L
+    t = time.time()
+    keys = L.gen_rmat(scale, edges, 0.57, 0.19, 0.19, seed)
+    keys = L.shuffle(keys, shuffle_seed)
+    half = edges // 2
+    bulk = keys[:half]
+    rest = keys[half:]
+    ins_batches = [rest[i:i + B] for i in range(0, rest.size, B)]
+    rng = np.random.default_rng(del_seed)
+    dels = keys[rng.permutation(edges)[: rest.size]]  # live = all edges once inserts finished
+    del_batches = [dels[i:i + B] for i in range(0, dels.size, B)]
+    log(f"workload generated in {time.time() - t:.1f}s: bulk {bulk.size}, {len(ins_batches)} insert + "
+        f"{len(del_batches)} delete batches of {B}")
+    return bulk, ins_batches, del_batches
+
+
+def step_list(ins_batches, del_batches, n):
+    """(op, batch index) for n steps of the cyclic stream."""
+    seq = [(1, i) for i in range(len(ins_batches))] + [(0, i) for i in range(len(del_batches))]
+    cyc = [(1, i) for i in range(len(del_batches))]  # re-insert the deleted edges
+    out = []
+    k = 0
+    while len(out) < n:
+        if k < len(seq):
+            out.append(seq[k])
+        else:
+            j = (k - len(seq)) % (2 * len(del_batches))
+            out.append((1, j) if j < len(del_batches) else (0, j - len(del_batches)))
+        k += 1
+    return out
+
+
+def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile):
+    import torch
+    import lpma_amd as L
+    stream = torch.cuda.current_stream()
+    pma = L.PMA(V, segment_slots=seg)
+    t = time.time()
+    pma.bulk_load(L.to_device(bulk))
+    torch.cuda.synchronize()
+    log(f"bulk load of {bulk.size} edges: {time.time() - t:.3f}s, capacity {pma.view().capacity}")
+
+    def batch_of(op, i, reinserted):
+        if op == 1 and not reinserted:
+            return ins_d[i]
+        return del_d[i]
+
+    def do(step_idx, st):
+        op, i = st
+        reins = step_idx >= len(ins_d) + len(del_d)
+        pma.update(batch_of(op, i, reins), op)
+
+    for k in range(warmup):
+        do(k, steps_seq[k])
+    torch.cuda.synchronize()
+    pma.reset_stats()
+    if profile:
+        pma.set_profiling(True)
+        pma.profile()  # clear
+    timed = steps_seq[warmup:]
+    n_updates = 0
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in timed]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, st in enumerate(timed):
+        k = warmup + j
+        op, i = st
+        reins = k >= len(ins_d) + len(del_d)
+        b = batch_of(op, i, reins)
+        ev[j][0].record(stream)
+        pma.update(b, op)
+        ev[j][1].record(stream)
+        n_updates += b.numel()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    stats = pma.stats()
+    prof = pma.profile() if profile else None
+    # correctness spot check of the final state: sorted, unique, consistent
+    assert pma.check() == 0
+    res = dict(wall=wall, n_updates=n_updates, lat_ms=lat, stats=stats, prof=prof, edges=pma.edges,
+               capacity=int(pma.view().capacity), height=int(pma.view().tree_height))
+    pma.close()
+    return res
+
+
+def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
+    import oracle
+    S = oracle.sort_unique(bulk)
+    times = []
+    n = 0
+    empty = np.empty(0, dtype=np.uint64)
+    for i in range(n_ins):
+        t = time.perf_counter()
+        S = oracle.apply(S, ins_batches[i], empty, threads=threads)
+        oracle.row_offsets(S, V)
+        times.append(time.perf_counter() - t)
+        n += ins_batches[i].size
+    # deletes sampled from the edges live at this point
+    rng = np.random.default_rng(77)
+    for i in range(n_del):
+        d = S[rng.choice(S.size, del_batches[i].size, replace=False)]
+        t = time.perf_counter()
+        S = oracle.apply(S, empty, d, threads=threads)
+        oracle.row_offsets(S, V)
+        times.append(time.perf_counter() - t)
+        n += d.size
+    return n / sum(times), times
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=160)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=100000)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--edges", type=int, default=1 << 24)
+    ap.add_argument("--segment-slots", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-batches", type=int, default=15)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the store is "
+              "sharded by key range across GPUs only by design (no multi-GPU run is part of this build).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lpma_amd as L
+    L.lib()
+    V = 1 << args.scale
+    B = args.batch
+    bulk, ins_b, del_b = make_workload(args.scale, args.edges, 2, 12, B, 13)
+    dev = torch.device("cuda", 0)
+    ins_d = [L.to_device(b, dev) for b in ins_b]
+    del_d = [L.to_device(b, dev) for b in del_b]
+    total = args.warmup + args.steps
+    seq = step_list(ins_b, del_b, total)
+
+    # pass 1: throughput and per-batch latency, no profiling events
+    r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False)
+    # pass 2: same stream with per-launch HIP events for the kernel roofline
+    rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True)
+
+    value = r["n_updates"] / r["wall"]
+    lat = r["lat_ms"]
+    st, pf = rp["stats"], rp["prof"]
+    # dominant rebalance kernel by time
+    kern = {
+        "k_rebalance_wave": (pf.rebalance_wave_ms, st.alg_bytes_wave, pf.launches_wave),
+        "k_rebalance_block": (pf.rebalance_block_ms, st.alg_bytes_block, pf.launches_block),
+    }
+    name = max(kern, key=lambda k: kern[k][0])
+    ms, byts, launches = kern[name]
+    avg_s = ms / 1e3 / max(launches, 1)
+    bytes_per_launch = byts / max(launches, 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    reb_ms = pf.rebalance_wave_ms + pf.rebalance_block_ms + pf.large_ms
+    reb_bytes = st.alg_bytes_wave + st.alg_bytes_block + st.alg_bytes_large
+    phases = {k: round(getattr(pf, k), 3) for k in ("sort_ms", "locate_ms", "elect_ms", "rebalance_wave_ms",
+                                                    "rebalance_block_ms", "large_ms", "rebuild_ms", "pyramid_ms")}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        nb = min(args.cpu_batches, len(ins_b), len(del_b))
+        t = time.time()
+        v_cpu, _ = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, threads)
+        log(f"cpu baseline ({threads} threads): {v_cpu:.3e} updates/s in {time.time() - t:.1f}s")
+        cpu = {"value": v_cpu, "unit": "edge updates/s", "cores": threads, "kind": "port",
+               "sample": f"CPU oracle (sorted merge + CSR offsets, pthreads) on the first {nb} insert and {nb} "
+                         f"delete batches of {B} of the same stream, from the same bulk-loaded state"}
+
+    out = {
+        "metric": "edge updates/sec (R-MAT 2^20 V, 16M E, batch 1e5); p99 batch latency; HBM GB/s",
+        "value": value,
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded R-MAT generator)",
+        "config": {"workload": f"rmat{args.scale}_{args.edges}E_half_bulk_stream_insert_then_delete_batch{B}",
+                   "vertices": V, "edges": args.edges, "batch": B, "segment_slots": args.segment_slots,
+                   "density": "lower 0.42 / upper 1.0 (reference)", "parallelism": "single GPU"},
+        "p50_batch_ms": float(np.percentile(lat, 50)),
+        "p99_batch_ms": float(np.percentile(lat, 99)),
+        "rebalance_alg_GBps": reb_bytes / (reb_ms / 1e3) / 1e9 if reb_ms > 0 else 0.0,
+        "phase_ms_total": phases,
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+                     "launches": int(launches)},
+        "cpu_baseline": cpu,
+        "final_state": {"edges": r["edges"], "capacity": r["capacity"], "tree_height": r["height"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

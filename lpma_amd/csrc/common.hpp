@@ -47,6 +47,8 @@ static constexpr u32 WAVE_E_CAP = 256;
 static constexpr u32 WAVE_SEG_CAP = 64;
 static constexpr u32 BLOCK_E_CAP = 2048;
 static constexpr u32 BLOCK_SEG_CAP = 1024;
+// the multi-workgroup path cuts windows into chunks of this many slots
+static constexpr u32 LARGE_CHUNK_LOG2_SLOTS = 12;
 
 // device statistics slots (u64 counters)
 enum StatSlot {
@@ -60,6 +62,9 @@ enum StatSlot {
     ST_ALG_BYTES_LARGE = 68,
     ST_NUM = 80,
 };
+// statistics are spread over ST_ROWS copies (row = block index mod ST_ROWS)
+// so that per-workgroup flushes do not serialise on one address
+static constexpr u32 ST_ROWS = 64;
 
 struct Status {
     u32 flags;

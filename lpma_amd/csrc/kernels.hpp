@@ -21,6 +21,9 @@ struct BatchBufs {
     u32 *rlevel;   // n: chosen level per run start
     i32 *part;     // 2 * nblocks + 2: block partials (count, delta)
     Work *work[NUM_CLASSES];  // n each
+    u32 *lchunk;   // n+1: chunk prefix over large windows
+    u64 *lmbase;   // n: merged-scratch base per large window
+    u32 *lhdr;     // 4: large windows, chunks
     Status *status;
     u64 *stats;
 };
@@ -33,8 +36,7 @@ hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipSt
 hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
 
 // global (multi-workgroup) window path, in place, and the full rebuild
-hipError_t large_window(const DevState &d, const Work &w, const BatchBufs &b, u64 *merged, u32 *gpre,
-                        u32 *scan_scratch, hipStream_t st);
+hipError_t rebalance_large(const DevState &d, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st);
 hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
                         u32 *scan_scratch, hipStream_t st);
 hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);

@@ -128,6 +128,7 @@ struct DBuf {
 
 struct lpma {
     int device = 0;
+    hipStream_t cur_stream = nullptr;  // stream of the call in progress
     lpma_config_t cfg;
     u32 V = 0, vbits = 1, log2S = 5;
     u64 live = 0;
@@ -141,7 +142,8 @@ struct lpma {
 
     // batch scratch
     DBuf<u64> pk0, pk1, ukey, ckey, merged;
-    DBuf<u32> sort_scr, useg, cseg, rlevel, gpre, scan_scr;
+    DBuf<u32> sort_scr, useg, cseg, rlevel, gpre, scan_scr, lchunk, lhdr;
+    DBuf<u64> lmbase;
     DBuf<i32> pd, part;
     DBuf<Work> work[NUM_CLASSES];
     Status *d_status = nullptr;
@@ -194,7 +196,12 @@ int ensure(lpma *h, DBuf<T> &b, u64 n) {
         if (r != LPMA_OK) return r;
         want = n;
     }
-    pool_free(h->device, b.p);
+    if (b.p) {
+        // queued kernels may still read the old block; it must not reach
+        // another stream through the shared pool before they finish
+        hipStreamSynchronize(h->cur_stream);
+        pool_free(h->device, b.p);
+    }
     b.p = (T *)p;
     b.n = want;
     return LPMA_OK;
@@ -328,6 +335,9 @@ BatchBufs batch_bufs(lpma *h) {
     b.rlevel = h->rlevel.p;
     b.part = h->part.p;
     for (int c = 0; c < NUM_CLASSES; ++c) b.work[c] = h->work[c].p;
+    b.lchunk = h->lchunk.p;
+    b.lmbase = h->lmbase.p;
+    b.lhdr = h->lhdr.p;
     b.status = h->d_status;
     b.stats = h->d_stats;
     return b;
@@ -376,6 +386,7 @@ int respread_into_new(lpma *h, u32 H, u64 m, hipStream_t st) {
 
 int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32 opb, hipStream_t st) {
     const u64 n = na + nb;
+    h->cur_stream = st;
     h->calls++;
     if (n == 0) {
         h->version++;
@@ -441,35 +452,24 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
         }
         {
             PhaseTimer t(h, st, PH_WAVE);
-            u32 g = (u32)std::min<u64>(grid_runs, 256 * 16);
+            u32 g = (u32)std::min<u64>(grid_runs, 65535);
             HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_WAVE, g, st));
             h->launches_wave++;
         }
         {
             PhaseTimer t(h, st, PH_BLOCK);
-            u32 g = (u32)std::min<u64>(grid_runs, 256 * 3);
+            u32 g = (u32)std::min<u64>(grid_runs, 256 * 8);
             HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_BLOCK, g, st));
             h->launches_block++;
         }
         if (s.flags & FLAG_LARGE) {
-            HIPCHECK(hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
-            u32 nl = h->h_status->nwork[CLASS_LARGE];
-            std::vector<Work> lw(nl);
-            if (nl) {
-                HIPCHECK(hipMemcpy(lw.data(), h->work[CLASS_LARGE].p, nl * sizeof(Work), hipMemcpyDeviceToHost));
-            }
-            u64 maxM = 1, maxSeg = 1;
-            for (auto &w : lw) {
-                maxM = std::max<u64>(maxM, w.M);
-                maxSeg = std::max<u64>(maxSeg, 1ull << w.L);
-            }
-            CHECKED(ensure(h, h->merged, maxM + 1));
-            CHECKED(ensure(h, h->gpre, maxSeg + 1));
-            CHECKED(ensure(h, h->scan_scr, scan_scratch_words(maxSeg) + 8));
+            CHECKED(ensure(h, h->merged, M + 1));
+            CHECKED(ensure(h, h->lchunk, n_eff + 1));
+            CHECKED(ensure(h, h->lmbase, n_eff));
+            CHECKED(ensure(h, h->lhdr, 4));
             bb = batch_bufs(h);
             PhaseTimer t(h, st, PH_LARGE);
-            for (auto &w : lw) HIPCHECK(large_window(h->st, w, bb, h->merged.p, h->gpre.p, h->scan_scr.p, st));
+            HIPCHECK(rebalance_large(h->st, bb, h->merged.p, 256 * 8, st));
         }
         {
             PhaseTimer t(h, st, PH_PYR);
@@ -537,6 +537,7 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
 
     lpma *h = new (std::nothrow) lpma();
     if (!h) return LPMA_ERR_OUT_OF_MEMORY;
+    h->cur_stream = st;
     h->device = dev;
     h->cfg = cfg;
     h->cfg.device = dev;
@@ -553,12 +554,12 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
     int r = LPMA_OK;
     if (hipMalloc(&h->d_status, sizeof(Status)) != hipSuccess ||
         hipHostMalloc((void **)&h->h_status, sizeof(Status), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&h->d_stats, ST_NUM * sizeof(u64)) != hipSuccess || hipMalloc(&h->d_err, 16) != hipSuccess) {
+        hipMalloc(&h->d_stats, ST_ROWS * ST_NUM * sizeof(u64)) != hipSuccess || hipMalloc(&h->d_err, 16) != hipSuccess) {
         (void)hipGetLastError();
         lpma_destroy(h);
         return LPMA_ERR_OUT_OF_MEMORY;
     }
-    hipMemsetAsync(h->d_stats, 0, ST_NUM * sizeof(u64), st);
+    hipMemsetAsync(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64), st);
     if ((r = ensure(h, h->row_off, (u64)num_vertices + 1)) != LPMA_OK) {
         lpma_destroy(h);
         return r;
@@ -609,6 +610,7 @@ int lpma_destroy(lpma_t *h) {
     release(h, h->pk0); release(h, h->pk1); release(h, h->ukey); release(h, h->ckey); release(h, h->merged);
     release(h, h->sort_scr); release(h, h->useg); release(h, h->cseg); release(h, h->rlevel);
     release(h, h->gpre); release(h, h->scan_scr); release(h, h->pd); release(h, h->part);
+    release(h, h->lchunk); release(h, h->lhdr); release(h, h->lmbase);
     for (int c = 0; c < NUM_CLASSES; ++c) release(h, h->work[c]);
     if (h->d_status) hipFree(h->d_status);
     if (h->h_status) hipHostFree(h->h_status);
@@ -667,6 +669,7 @@ static int export_common(lpma_t *h, u64 *edges, u32 *row_std, u64 *all, u32 *csr
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     hipStream_t st = (hipStream_t)stream;
+    h->cur_stream = st;
     u64 nseg = 1ull << h->st.H;
     CHECKED(ensure(h, h->gpre, nseg + 1));
     CHECKED(ensure(h, h->scan_scr, scan_scratch_words(nseg) + 8));
@@ -726,9 +729,12 @@ int lpma_check(lpma_t *h, uint64_t *violations, void *stream) {
 int lpma_get_stats(lpma_t *h, lpma_stats_t *out) {
     if (!h || !out) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    u64 s[ST_NUM];
+    std::vector<u64> rows((size_t)ST_ROWS * ST_NUM);
     HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemcpy(s, h->d_stats, sizeof(s), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(rows.data(), h->d_stats, rows.size() * sizeof(u64), hipMemcpyDeviceToHost));
+    u64 s[ST_NUM] = {0};
+    for (u32 r = 0; r < ST_ROWS; ++r)
+        for (u32 i = 0; i < ST_NUM; ++i) s[i] += rows[(size_t)r * ST_NUM + i];
     memset(out, 0, sizeof(*out));
     out->updates_calls = h->calls;
     out->effective_updates = h->effective;
@@ -749,7 +755,7 @@ int lpma_reset_stats(lpma_t *h) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemset(h->d_stats, 0, ST_NUM * sizeof(u64)));
+    HIPCHECK(hipMemset(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64)));
     h->calls = h->effective = h->rebuilds = h->rebuild_slots = 0;
     return LPMA_OK;
 }

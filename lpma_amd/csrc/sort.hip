@@ -15,10 +15,11 @@ namespace pma {
 static constexpr int RS_BITS = 8;
 static constexpr int RS_RADIX = 1 << RS_BITS;
 static constexpr int RS_TPB = 256;
-static constexpr int RS_ITEMS = 16;
-static constexpr int RS_TILE = RS_TPB * RS_ITEMS;
-static constexpr int RS_SMALL_ITEMS = 8;
-static constexpr int RS_SMALL_CAP = RS_TPB * RS_SMALL_ITEMS;  // single-workgroup sort
+static constexpr int RS_ITEMS_BIG = 16;   // keys per thread per tile, large inputs
+static constexpr int RS_ITEMS_SMALL = 4;  // smaller tiles for more workgroups on mid-size batches
+static constexpr int RS_SMALL_TPB = 1024;
+static constexpr int RS_SMALL_CAP = 8192;  // single-workgroup sort (2 x 64 KiB of LDS)
+static constexpr u32 RS_PREFIX_MAX_TILES = 512;  // scatter derives its offsets itself up to here
 
 __global__ void k_pack(const u64 *__restrict__ in, u64 n, u32 op, u32 V, u32 vbits,
                        u64 *__restrict__ out, u32 *__restrict__ flags) {
@@ -35,17 +36,25 @@ __global__ void k_pack(const u64 *__restrict__ in, u64 n, u32 op, u32 V, u32 vbi
 
 // ------------------------------------------------------------ stable ranking
 //
-// One round ranks RS_TPB consecutive keys of a tile by their digit: lanes
-// with equal digits are found with 8 wave ballots (a wave64 "match any"),
-// a lane's rank inside its wave is the popcount of its lower peers, and
+// One round ranks NT consecutive keys of a tile by their digit: lanes with
+// equal digits are found with 8 wave ballots (a wave64 "match any"), a
+// lane's rank inside its wave is the popcount of its lower peers, and
 // per-wave digit counts in LDS give the offset of the waves before it.
 
+template <int NT>
 struct RankLds {
     u32 running[RS_RADIX];
-    u32 wcnt[RS_TPB / 64][RS_RADIX];
+    u32 wcnt[NT / 64][RS_RADIX];
 };
 
-__device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds &s, u32 &pos) {
+template <int NT>
+__device__ __forceinline__ void rank_init(RankLds<NT> &s) {
+    for (u32 i = threadIdx.x; i < RS_RADIX; i += NT) s.running[i] = 0;
+    for (u32 i = threadIdx.x; i < (NT / 64) * RS_RADIX; i += NT) (&s.wcnt[0][0])[i] = 0;
+}
+
+template <int NT>
+__device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s, u32 &pos) {
     const u32 lane = lane_id();
     const u32 w = threadIdx.x >> 6;
     u64 peers = __ballot(valid);
@@ -64,88 +73,110 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds &s, u3
         pos = off + rank;
     }
     __syncthreads();
-    {
-        u32 d = threadIdx.x;  // RS_TPB == RS_RADIX
+    for (u32 d = threadIdx.x; d < RS_RADIX; d += NT) {
         u32 add = 0;
 #pragma unroll
-        for (int v = 0; v < RS_TPB / 64; ++v) { add += s.wcnt[v][d]; s.wcnt[v][d] = 0; }
+        for (int v = 0; v < NT / 64; ++v) { add += s.wcnt[v][d]; s.wcnt[v][d] = 0; }
         s.running[d] += add;
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys, u64 n, u32 shift,
-                                                     u32 *__restrict__ hist, u32 nblocks) {
+// per-tile digit histogram; layout block-major (hist[b*RADIX + d]) when
+// `block_major`, digit-major (hist[d*nb + b]) otherwise
+__global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys, u64 n, u32 shift, u32 items,
+                                                     u32 *__restrict__ hist, u32 nblocks, int block_major) {
     __shared__ u32 h[RS_RADIX];
     h[threadIdx.x] = 0;
     __syncthreads();
-    u64 base = (u64)blockIdx.x * RS_TILE;
-#pragma unroll 4
-    for (int r = 0; r < RS_ITEMS; ++r) {
+    u64 base = (u64)blockIdx.x * items * RS_TPB;
+    for (u32 r = 0; r < items; ++r) {
         u64 i = base + (u64)r * RS_TPB + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_RADIX - 1)], 1u);
     }
     __syncthreads();
-    hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
+    else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Scatter of one pass.  With block-major histograms (few tiles) every
+// workgroup derives its digit offsets itself: digit totals and the prefix
+// over earlier tiles from the histogram rows, then a scan over the digits;
+// otherwise hist_scan holds the device-wide exclusive scan (digit-major).
 __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ in, u64 *__restrict__ out, u64 n,
-                                                        u32 shift, const u32 *__restrict__ hist_scan,
-                                                        u32 nblocks) {
-    __shared__ RankLds s;
+                                                        u32 shift, u32 items, const u32 *__restrict__ hist,
+                                                        u32 nblocks, int block_major) {
+    __shared__ RankLds<RS_TPB> s;
     __shared__ u32 gofs[RS_RADIX];
-    gofs[threadIdx.x] = hist_scan[(u64)threadIdx.x * nblocks + blockIdx.x];
-    s.running[threadIdx.x] = 0;
-#pragma unroll
-    for (int v = 0; v < RS_TPB / 64; ++v) s.wcnt[v][threadIdx.x] = 0;
+    __shared__ u32 stmp[RS_TPB / 64 + 1];
+    const u32 d = threadIdx.x;
+    if (block_major) {
+        u32 tot = 0, pre = 0;
+#pragma unroll 16
+        for (u32 b = 0; b < nblocks; ++b) {
+            u32 v = hist[(u64)b * RS_RADIX + d];
+            tot += v;
+            pre += b < blockIdx.x ? v : 0u;
+        }
+        u32 all;
+        u32 dbase = block_excl_scan<RS_TPB, u32>(tot, all, stmp);
+        gofs[d] = dbase + pre;
+    } else {
+        gofs[d] = hist[(u64)d * nblocks + blockIdx.x];
+    }
+    rank_init<RS_TPB>(s);
     __syncthreads();
-    u64 base = (u64)blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
+    u64 base = (u64)blockIdx.x * items * RS_TPB;
+    for (u32 r = 0; r < items; ++r) {
         u64 i = base + (u64)r * RS_TPB + threadIdx.x;
         bool valid = i < n;
         u64 key = valid ? in[i] : 0;
         u32 digit = (u32)(key >> shift) & (RS_RADIX - 1);
         u32 pos = 0;
-        rank_round(valid, digit, s, pos);
+        rank_round<RS_TPB>(valid, digit, s, pos);
         if (valid) out[gofs[digit] + pos] = key;
     }
 }
 
 // Whole sort inside one workgroup's LDS for n <= RS_SMALL_CAP.
-__global__ __launch_bounds__(RS_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, u32 nbits) {
+__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, u32 nbits) {
     __shared__ u64 buf[2][RS_SMALL_CAP];
-    __shared__ RankLds s;
+    __shared__ RankLds<RS_SMALL_TPB> s;
     __shared__ u32 h[RS_RADIX];
-    __shared__ u32 scan_tmp[RS_TPB / 64 + 1];
-    for (u32 i = threadIdx.x; i < n; i += RS_TPB) buf[0][i] = keys[i];
+    __shared__ u32 scan_tmp[RS_RADIX / 64 + 1];
+    for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) buf[0][i] = keys[i];
     int cur = 0;
     for (u32 shift = 0; shift < nbits; shift += RS_BITS) {
-        h[threadIdx.x] = 0;
-        s.running[threadIdx.x] = 0;
-#pragma unroll
-        for (int v = 0; v < RS_TPB / 64; ++v) s.wcnt[v][threadIdx.x] = 0;
+        for (u32 i = threadIdx.x; i < RS_RADIX; i += RS_SMALL_TPB) h[i] = 0;
+        rank_init<RS_SMALL_TPB>(s);
         __syncthreads();
-        for (u32 i = threadIdx.x; i < n; i += RS_TPB)
+        for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB)
             atomicAdd(&h[(buf[cur][i] >> shift) & (RS_RADIX - 1)], 1u);
         __syncthreads();
-        u32 tot;
-        u32 ex = block_excl_scan<RS_TPB, u32>(h[threadIdx.x], tot, scan_tmp);
+        if (threadIdx.x < 64) {
+            // one wave scans the 256 digit counts
+            u32 v[RS_RADIX / 64];
+            u32 sum = 0;
+#pragma unroll
+            for (int r = 0; r < RS_RADIX / 64; ++r) { v[r] = h[threadIdx.x * (RS_RADIX / 64) + r]; sum += v[r]; }
+            u32 ex = wave_incl_scan(sum) - sum;
+#pragma unroll
+            for (int r = 0; r < RS_RADIX / 64; ++r) { h[threadIdx.x * (RS_RADIX / 64) + r] = ex; ex += v[r]; }
+        }
         __syncthreads();
-        h[threadIdx.x] = ex;
-        __syncthreads();
-        for (u32 r = 0; r * RS_TPB < n; ++r) {
-            u32 i = r * RS_TPB + threadIdx.x;
+        for (u32 r = 0; r * RS_SMALL_TPB < n; ++r) {
+            u32 i = r * RS_SMALL_TPB + threadIdx.x;
             bool valid = i < n;
             u64 key = valid ? buf[cur][i] : 0;
             u32 digit = (u32)(key >> shift) & (RS_RADIX - 1);
             u32 pos = 0;
-            rank_round(valid, digit, s, pos);
+            rank_round<RS_SMALL_TPB>(valid, digit, s, pos);
             if (valid) buf[cur ^ 1][h[digit] + pos] = key;
         }
         __syncthreads();
         cur ^= 1;
     }
-    for (u32 i = threadIdx.x; i < n; i += RS_TPB) keys[i] = buf[cur][i];
+    for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) keys[i] = buf[cur][i];
 }
 
 // ------------------------------------------------------ device-wide scan
@@ -226,8 +257,10 @@ hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStr
 
 // ------------------------------------------------------------ radix driver
 
+static u32 rs_items(u64 n) { return n <= (1ull << 20) ? RS_ITEMS_SMALL : RS_ITEMS_BIG; }
+
 u64 sort_scratch_words(u64 n) {
-    u64 nb = (n + RS_TILE - 1) / RS_TILE;
+    u64 nb = (n + (u64)rs_items(n) * RS_TPB - 1) / ((u64)rs_items(n) * RS_TPB);
     return (u64)RS_RADIX * nb + 1 + scan_scratch_words((u64)RS_RADIX * nb);
 }
 
@@ -242,17 +275,20 @@ hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, 
 u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st) {
     if (n <= 1) return keys;
     if (n <= (u64)RS_SMALL_CAP) {
-        k_rs_small<<<1, RS_TPB, 0, st>>>(keys, n, nbits);
+        k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(keys, n, nbits);
         return keys;
     }
-    u32 nb = (u32)((n + RS_TILE - 1) / RS_TILE);
+    const u32 items = rs_items(n);
+    const u64 tile = (u64)items * RS_TPB;
+    const u32 nb = (u32)((n + tile - 1) / tile);
+    const int block_major = nb <= RS_PREFIX_MAX_TILES;
     u32 *hist = scratch;
     u32 *sscr = scratch + (u64)RS_RADIX * nb + 1;
     u64 *src = keys, *dst = alt;
     for (u32 shift = 0; shift < nbits; shift += RS_BITS) {
-        k_rs_hist<<<nb, RS_TPB, 0, st>>>(src, n, shift, hist, nb);
-        device_excl_scan(hist, (u64)RS_RADIX * nb, hist, sscr, st);
-        k_rs_scatter<<<nb, RS_TPB, 0, st>>>(src, dst, n, shift, hist, nb);
+        k_rs_hist<<<nb, RS_TPB, 0, st>>>(src, n, shift, items, hist, nb, block_major);
+        if (!block_major) device_excl_scan(hist, (u64)RS_RADIX * nb, hist, sscr, st);
+        k_rs_scatter<<<nb, RS_TPB, 0, st>>>(src, dst, n, shift, items, hist, nb, block_major);
         u64 *t = src; src = dst; dst = t;
     }
     return src;

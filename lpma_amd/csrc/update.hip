@@ -228,30 +228,47 @@ __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *_
                           const u32 *__restrict__ rlevel, u32 n_eff, Work *w0, Work *w1, Work *w2,
                           Status *status) {
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_eff) return;
-    u32 s = cseg[j];
-    if (j > 0 && cseg[j - 1] == s) return;
-    u32 L = rlevel[j];
-    u64 nseg = nseg_of(d);
-    u64 w = (u64)s >> L;
-    if (d.owner[owner_idx(nseg, L, w)] != j) return;
-    for (u32 l2 = L + 1; l2 < d.H; ++l2)
-        if (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] != SEG_NONE) return;  // inside a larger window
-    u64 a = w << L;
-    u32 u0 = gallop_back(cseg, j, a);
-    u32 u1 = gallop_fwd(cseg, j + 1, n_eff, a + (1ull << L));
-    u32 C = *count_ptr(d, L, w);
+    int cls = -1;
     Work wk;
-    wk.a = (u32)a;
-    wk.L = L;
-    wk.u0 = u0;
-    wk.u1 = u1;
-    wk.C = C;
-    wk.M = (u32)((i32)C + pd[u1] - pd[u0]);
-    int cls = work_class((u64)C + (u1 - u0), L);
-    u32 idx = atomicAdd(&status->nwork[cls], 1u);
-    Work *dst = cls == CLASS_WAVE ? w0 : (cls == CLASS_BLOCK ? w1 : w2);
-    dst[idx] = wk;
+    if (j < n_eff) {
+        u32 s = cseg[j];
+        bool leader = (j == 0 || cseg[j - 1] != s);
+        u32 L = leader ? rlevel[j] : 0;
+        u64 nseg = nseg_of(d);
+        u64 w = (u64)s >> L;
+        if (leader && d.owner[owner_idx(nseg, L, w)] != j) leader = false;
+        for (u32 l2 = L + 1; leader && l2 < d.H; ++l2)
+            if (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] != SEG_NONE) leader = false;  // inside a larger window
+        if (leader) {
+            u64 a = w << L;
+            u32 u0 = gallop_back(cseg, j, a);
+            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, a + (1ull << L));
+            u32 C = *count_ptr(d, L, w);
+            wk.a = (u32)a;
+            wk.L = L;
+            wk.u0 = u0;
+            wk.u1 = u1;
+            wk.C = C;
+            wk.M = (u32)((i32)C + pd[u1] - pd[u0]);
+            cls = work_class((u64)C + (u1 - u0), L);
+        }
+    }
+    // one atomic per wave and class for the list appends
+    const u32 lane = lane_id();
+    const u64 lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int c = 0; c < NUM_CLASSES; ++c) {
+        u64 m = __ballot(cls == c);
+        if (!m) continue;
+        u32 first = (u32)__builtin_ctzll(m);
+        u32 base = 0;
+        if (lane == first) base = atomicAdd(&status->nwork[c], (u32)__popcll(m));
+        base = __shfl(base, first, 64);
+        if (cls == c) {
+            Work *dst = c == CLASS_WAVE ? w0 : (c == CLASS_BLOCK ? w1 : w2);
+            dst[base + __popcll(m & lt)] = wk;
+        }
+    }
 }
 
 __global__ void k_reset_owner(DevState d, const u32 *__restrict__ cseg, const u32 *__restrict__ rlevel, u32 n_eff) {
@@ -369,7 +386,8 @@ __device__ void rebalance_window(const DevState &d, const Work &wk, const u64 *_
     __syncthreads();
 }
 
-__device__ void flush_stats(BlockStats &bs, u64 *stats, int cls_slot, u32 tid) {
+__device__ void flush_stats(BlockStats &bs, u64 *stats_all, int cls_slot, u32 tid) {
+    u64 *stats = stats_all + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
     // markers were accumulated by lane 0 of every wave
     if ((tid & 63) == 0 && bs.markers) {
         atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)bs.markers);
@@ -408,12 +426,119 @@ __device__ void rebalance_loop(const DevState &d, const Work *__restrict__ work,
     flush_stats(bs, stats, cls_slot, threadIdx.x);
 }
 
+// One wave64 per window (E <= WAVE_E_CAP, <= WAVE_SEG_CAP segments).  The
+// live keys are found with a ballot over the window's slots (segments are
+// front-packed, gaps hold KEY_NONE), so neither the segment counts nor a scan
+// sit on the dependency chain: the slot loads and the update loads are
+// issued together, then one LDS merge, then the coalesced store.
+__device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u64 *__restrict__ ckey,
+                                      const i32 *__restrict__ pd, u64 *A, u64 *B, i32 *P, BlockStats &bs) {
+    const u32 lane = lane_id();
+    const u64 lt = (1ull << lane) - 1ull;
+    const u32 L = wk.L;
+    const u64 a = wk.a;
+    const u32 nsegw = 1u << L;
+    const u32 C = wk.C;
+    const u32 k = wk.u1 - wk.u0;
+    const u32 log2S = d.log2S;
+    const u32 Smask = (1u << log2S) - 1;
+    const u64 base = a << log2S;
+    const u32 W = nsegw << log2S;
+
+    const i32 pbase = pd[wk.u0];
+    for (u32 q = lane; q < k; q += 64) A[C + q] = ckey[wk.u0 + q];
+    for (u32 q = lane; q <= k; q += 64) P[q] = pd[wk.u0 + q] - pbase;
+    u32 off = 0;
+    for (u32 t0 = 0; t0 < W; t0 += 256) {
+        u64 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            u32 t = t0 + r * 64 + lane;
+            v[r] = t < W ? d.keys[base + t] : KEY_NONE;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            bool live = v[r] != KEY_NONE;
+            u64 m = __ballot(live);
+            if (live) A[off + __popcll(m & lt)] = v[r];
+            off += __popcll(m);
+        }
+    }
+    __syncthreads();
+
+    const u32 M = (u32)((i32)C + P[k]);
+    const u64 *U = A + C;
+    for (u32 r = lane; r < C; r += 64) {
+        u64 x = A[r];
+        u32 lo = 0, hi = k;
+        while (lo < hi) {
+            u32 m = (lo + hi) >> 1;
+            if (U[m] < x) lo = m + 1; else hi = m;
+        }
+        if (lo < k && U[lo] == x) continue;  // deleted
+        B[r + P[lo]] = x;
+    }
+    for (u32 q = lane; q < k; q += 64) {
+        if (P[q + 1] - P[q] != 1) continue;
+        u64 y = U[q];
+        u32 lo = 0, hi = C;
+        while (lo < hi) {
+            u32 m = (lo + hi) >> 1;
+            if (A[m] < y) lo = m + 1; else hi = m;
+        }
+        B[lo + P[q]] = y;
+    }
+    __syncthreads();
+
+    u32 markers = 0;
+    for (u32 t = lane; t < W; t += 64) {
+        u32 seg = t >> log2S, ln = t & Smask;
+        u64 f = spread_first(seg, M, L);
+        u64 g = spread_first(seg + 1, M, L);
+        u64 key = (ln < g - f) ? B[f + ln] : KEY_NONE;
+        d.keys[base + t] = key;
+        if (is_marker(key) && (u32)(key >> 32) < d.V) {
+            d.row_off[(u32)(key >> 32) + 1] = (u32)(base + t);
+            ++markers;
+        }
+    }
+    const u64 sep_a = d.sep[a];
+    for (u32 x = lane; x < nsegw; x += 64) {
+        u64 f = spread_first(x, M, L);
+        u64 g = spread_first(x + 1, M, L);
+        d.cnt[a + x] = (u32)(g - f);
+        if (x > 0) d.sep[a + x] = (g > f) ? B[f] : (f > 0 ? B[f - 1] + 1 : sep_a);
+    }
+    markers = wave_sum(markers);
+    if (lane == 0) {
+        bs.markers += markers;
+        bs.windows[L] += 1;
+        bs.updates += k;
+        bs.bytes += 16ull * W + 8ull * k;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(64) void k_rebalance_wave(DevState d, const Work *__restrict__ work,
                                                        const Status *__restrict__ status,
                                                        const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
                                                        u64 *stats) {
-    rebalance_loop<64, WAVE_E_CAP, WAVE_SEG_CAP>(d, work, &status->nwork[CLASS_WAVE], ckey, pd, stats,
-                                                 ST_ALG_BYTES_WAVE);
+    __shared__ u64 A[WAVE_E_CAP];
+    __shared__ u64 B[WAVE_E_CAP];
+    __shared__ i32 P[WAVE_E_CAP + 1];
+    __shared__ u32 wl[32];
+    BlockStats bs;
+    bs.updates = bs.markers = bs.bytes = 0;
+    bs.windows = wl;
+    if (threadIdx.x < 32) wl[threadIdx.x] = 0;
+    __syncthreads();
+    const u32 nw = status->nwork[CLASS_WAVE];
+    for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
+        Work wk = work[i];
+        rebalance_window_wave(d, wk, ckey, pd, A, B, P, bs);
+    }
+    __syncthreads();
+    flush_stats(bs, stats, ST_ALG_BYTES_WAVE, threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work *__restrict__ work,
@@ -492,8 +617,9 @@ __global__ void k_pull(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ 
     }
     markers = wave_sum(markers);
     if (lane_id() == 0 && markers && stats) {
-        atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)markers);
-        atomicAdd((unsigned long long *)&stats[cls_slot], (unsigned long long)(4ull * markers));
+        u64 *row = stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+        atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)markers);
+        atomicAdd((unsigned long long *)&row[cls_slot], (unsigned long long)(4ull * markers));
     }
 }
 
@@ -512,20 +638,210 @@ __global__ void k_meta(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ 
     }
 }
 
+// ---------------------------------------------- batched large windows
+//
+// All windows beyond the LDS classes of one batch are handled by three
+// launches, whatever their number and size.  A window of 2^L segments is cut
+// into chunks of 2^cl segments (4096 slots); chunk j's rank offset inside
+// its window is read off the count pyramid by a walk over the levels (the
+// counts of the aligned blocks left of it), so no scan is needed.  Each
+// window gets a private slice of the merged scratch (windows are disjoint,
+// so the slices add up to at most the live key count).
+
+__device__ __forceinline__ u32 chunk_level(const DevState &d, u32 L) {
+    u32 cl = d.log2S >= LARGE_CHUNK_LOG2_SLOTS ? 0 : LARGE_CHUNK_LOG2_SLOTS - d.log2S;
+    return L < cl ? L : cl;
+}
+
+// live keys of the window's segments [a, x), x - a a multiple of 2^cl
+__device__ __forceinline__ u64 rank_in_window(const DevState &d, u64 a, u32 L, u64 x, u32 cl) {
+    u64 rel = x - a, pos = a, off = 0;
+    for (int l = (int)L - 1; l >= (int)cl; --l) {
+        if ((rel >> l) & 1ull) {
+            off += *count_ptr(d, (u32)l, pos >> l);
+            pos += 1ull << l;
+        }
+    }
+    return off;
+}
+
+__global__ __launch_bounds__(1024) void k_large_prep(const DevState d, const Work *__restrict__ work,
+                                                     const Status *__restrict__ status, u32 *__restrict__ chunk_base,
+                                                     u64 *__restrict__ mbase, u32 *__restrict__ hdr) {
+    __shared__ u32 tmp[1024 / 64 + 1];
+    __shared__ u64 tmp64[1024 / 64 + 1];
+    const u32 n = status->nwork[CLASS_LARGE];
+    u32 carry = 0;
+    u64 mcarry = 0;
+    for (u32 b = 0; b < n; b += 1024) {
+        u32 i = b + threadIdx.x;
+        u32 nch = 0;
+        u64 m = 0;
+        if (i < n) {
+            Work w = work[i];
+            nch = 1u << (w.L - chunk_level(d, w.L));
+            m = w.M;
+        }
+        u32 tot;
+        u64 mtot;
+        u32 ex = block_excl_scan<1024, u32>(nch, tot, tmp);
+        u64 mex = block_excl_scan<1024, u64>(m, mtot, tmp64);
+        if (i < n) {
+            chunk_base[i] = carry + ex;
+            mbase[i] = mcarry + mex;
+        }
+        carry += tot;
+        mcarry += mtot;
+    }
+    if (threadIdx.x == 0) {
+        chunk_base[n] = carry;
+        hdr[0] = n;
+        hdr[1] = carry;
+    }
+}
+
+__device__ __forceinline__ u32 window_of_chunk(const u32 *chunk_base, u32 n, u32 c) {
+    // last i with chunk_base[i] <= c
+    u32 lo = 0, hi = n;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (chunk_base[mid] <= c) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+static constexpr int LG_TPB = 256;
+static constexpr u32 LG_SEG_MAX = 1u << LARGE_CHUNK_LOG2_SLOTS;  // segments per chunk at S = 1
+
+__global__ __launch_bounds__(LG_TPB) void k_large_push(DevState d, const Work *__restrict__ work,
+                                                        const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
+                                                        const u32 *__restrict__ hdr, const u64 *__restrict__ ckey,
+                                                        const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+                                                        u64 *__restrict__ merged) {
+    __shared__ u32 pre[LG_SEG_MAX + 1];
+    __shared__ u32 stmp[LG_TPB / 64 + 1];
+    __shared__ u32 qr[2];
+    const u32 nwin = hdr[0], nch = hdr[1];
+    const u32 log2S = d.log2S;
+    const u32 Smask = (1u << log2S) - 1;
+    for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
+        const u32 i = window_of_chunk(chunk_base, nwin, c);
+        const Work wk = work[i];
+        const u32 cl = chunk_level(d, wk.L);
+        const u64 j = c - chunk_base[i];
+        const u64 s0 = (u64)wk.a + (j << cl);
+        const u32 nsc = 1u << cl;
+        const u64 obase = rank_in_window(d, wk.a, wk.L, s0, cl);
+        const u64 mb = mbase[i];
+        const i32 pbase = pd[wk.u0];
+        for (u32 x = threadIdx.x; x < nsc; x += LG_TPB) pre[x] = d.cnt[s0 + x];
+        if (threadIdx.x == 0) {
+            qr[0] = (u32)lower_bound_dev<u32>(cseg, wk.u0, wk.u1, (u32)s0);
+            qr[1] = (u32)lower_bound_dev<u32>(cseg, wk.u0, wk.u1, (u32)(s0 + nsc));
+        }
+        __syncthreads();
+        block_scan_array<LG_TPB>(pre, nsc, stmp);
+        const u64 slot0 = s0 << log2S;
+        const u32 W = nsc << log2S;
+        for (u32 t = threadIdx.x; t < W; t += LG_TPB) {
+            u32 seg = t >> log2S, lane = t & Smask;
+            u32 p0 = pre[seg];
+            if (lane >= pre[seg + 1] - p0) continue;
+            u64 x = d.keys[slot0 + t];
+            u64 u = lower_bound_dev<u64>(ckey, wk.u0, wk.u1, x);
+            if (u < wk.u1 && ckey[u] == x) continue;  // deleted
+            u64 o = obase + p0 + lane + (i64)(pd[u] - pbase);
+            if (o < wk.M) merged[mb + o] = x;
+        }
+        for (u32 q = qr[0] + threadIdx.x; q < qr[1]; q += LG_TPB) {
+            if (pd[q + 1] - pd[q] != 1) continue;
+            u64 y = ckey[q];
+            u64 sg = cseg[q];
+            u64 o = obase + pre[sg - s0] + lower_bound_dev<u64>(d.keys + (sg << log2S), 0, d.cnt[sg], y) +
+                    (i64)(pd[q] - pbase);
+            if (o < wk.M) merged[mb + o] = y;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(LG_TPB) void k_large_pull(DevState d, const Work *__restrict__ work,
+                                                        const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
+                                                        const u32 *__restrict__ hdr, const u64 *__restrict__ merged,
+                                                        u64 *stats_all) {
+    const u32 nwin = hdr[0], nch = hdr[1];
+    const u32 log2S = d.log2S;
+    const u32 Smask = (1u << log2S) - 1;
+    u64 bytes = 0, markers = 0, upd = 0;
+    __shared__ u32 windows[32];
+    bool any = false;
+    if (threadIdx.x < 32) windows[threadIdx.x] = 0;
+    __syncthreads();
+    for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
+        const u32 i = window_of_chunk(chunk_base, nwin, c);
+        const Work wk = work[i];
+        const u32 cl = chunk_level(d, wk.L);
+        const u64 j = c - chunk_base[i];
+        const u64 rel0 = j << cl;
+        const u32 nsc = 1u << cl;
+        const u64 mb = mbase[i];
+        const u64 M = wk.M;
+        const u64 slot0 = ((u64)wk.a + rel0) << log2S;
+        const u32 W = nsc << log2S;
+        for (u32 t = threadIdx.x; t < W; t += LG_TPB) {
+            u64 seg = rel0 + (t >> log2S);
+            u32 lane = t & Smask;
+            u64 f = spread_first(seg, M, wk.L);
+            u64 g = spread_first(seg + 1, M, wk.L);
+            u64 key = (lane < g - f) ? merged[mb + f + lane] : KEY_NONE;
+            d.keys[slot0 + t] = key;
+            if (is_marker(key) && (u32)(key >> 32) < d.V) {
+                d.row_off[(u32)(key >> 32) + 1] = (u32)(slot0 + t);
+                ++markers;
+            }
+        }
+        const u64 sep_a = d.sep[wk.a];
+        for (u32 x = threadIdx.x; x < nsc; x += LG_TPB) {
+            u64 seg = rel0 + x;
+            u64 f = spread_first(seg, M, wk.L);
+            u64 g = spread_first(seg + 1, M, wk.L);
+            d.cnt[wk.a + seg] = (u32)(g - f);
+            if (seg > 0) d.sep[wk.a + seg] = (g > f) ? merged[mb + f] : (f > 0 ? merged[mb + f - 1] + 1 : sep_a);
+        }
+        if (threadIdx.x == 0) {
+            bytes += 16ull * W;
+            if (j == 0) {
+                windows[wk.L & 31] += 1u;
+                upd += wk.u1 - wk.u0;
+                bytes += 8ull * (wk.u1 - wk.u0);
+            }
+            any = true;
+        }
+    }
+    markers = wave_sum(markers);
+    u64 *stats = stats_all + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    if (lane_id() == 0 && markers) {
+        atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)markers);
+        atomicAdd((unsigned long long *)&stats[ST_ALG_BYTES_LARGE], (unsigned long long)(4ull * markers));
+    }
+    if (threadIdx.x == 0 && any) {
+        atomicAdd((unsigned long long *)&stats[ST_ALG_BYTES_LARGE], (unsigned long long)bytes);
+        atomicAdd((unsigned long long *)&stats[ST_UPDATES], (unsigned long long)upd);
+        for (int l = 0; l < 32; ++l)
+            if (windows[l]) atomicAdd((unsigned long long *)&stats[ST_WINDOWS + l], (unsigned long long)windows[l]);
+    }
+}
+
+hipError_t rebalance_large(const DevState &d, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st) {
+    k_large_prep<<<1, 1024, 0, st>>>(d, b.work[CLASS_LARGE], b.status, b.lchunk, b.lmbase, b.lhdr);
+    k_large_push<<<grid, LG_TPB, 0, st>>>(d, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
+                                          merged);
+    k_large_pull<<<grid, LG_TPB, 0, st>>>(d, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, merged, b.stats);
+    return hipGetLastError();
+}
+
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);
-
-hipError_t large_window(const DevState &d, const Work &w, const BatchBufs &b, u64 *merged, u32 *gpre,
-                        u32 *scan_scratch, hipStream_t st) {
-    u64 nsegw = 1ull << w.L;
-    u64 W = nsegw << d.log2S;
-    device_excl_scan(d.cnt + w.a, nsegw, gpre, scan_scratch, st);
-    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(d, w.a, w.L, gpre, b.ckey, b.pd, w.u0, w.u1, merged, w.M);
-    if (w.u1 > w.u0)
-        k_push_ins<<<grid_for(w.u1 - w.u0, 256, 16384), 256, 0, st>>>(d, w.a, gpre, b.ckey, b.cseg, b.pd, w.u0, w.u1,
-                                                                       merged, w.M);
-    return window_spread(d, w.a, w.L, merged, w.M, b.stats, ST_ALG_BYTES_LARGE, st);
-}
 
 // full push of all live keys and all effective updates (root / resize)
 hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,

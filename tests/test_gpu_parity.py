@@ -225,3 +225,19 @@ def test_queries():
     for r in (0, 1, V // 2, V - 1):
         assert col[ro[r + 1] - 1] == r  # marker slot holds its row id
     t.close()
+
+
+@pytest.mark.parametrize("n", [9000, 300000, 700000, 3000000])
+def test_batch_sizes_cover_sort_paths(n):
+    """Batches that exercise every radix-sort path: one workgroup (<= 8192),
+    self-offset tiles, and tiles offset by a device-wide scan."""
+    say(f"batch of {n}")
+    rng = np.random.default_rng(n)
+    V = 1 << 16
+    t = Triple(V, model=n <= 300000)
+    k = rand_keys(rng, V, n)
+    t.insert(np.concatenate([k, k[: n // 10]]))
+    t.compare()
+    t.mixed(rand_keys(rng, V, n // 2), t.ref[rng.integers(0, t.ref.size, n // 2)])
+    t.compare()
+    t.close()
