@@ -185,6 +185,7 @@ def main():
     st, pf = rp["stats"], rp["prof"]
     # dominant rebalance kernel by time
     kern = {
+        "k_rebalance_seg": (pf.rebalance_seg_ms, st.alg_bytes_seg, pf.launches_seg),
         "k_rebalance_wave": (pf.rebalance_wave_ms, st.alg_bytes_wave, pf.launches_wave),
         "k_rebalance_block": (pf.rebalance_block_ms, st.alg_bytes_block, pf.launches_block),
     }
@@ -193,10 +194,12 @@ def main():
     avg_s = ms / 1e3 / max(launches, 1)
     bytes_per_launch = byts / max(launches, 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    reb_ms = pf.rebalance_wave_ms + pf.rebalance_block_ms + pf.large_ms
-    reb_bytes = st.alg_bytes_wave + st.alg_bytes_block + st.alg_bytes_large
-    phases = {k: round(getattr(pf, k), 3) for k in ("sort_ms", "locate_ms", "elect_ms", "rebalance_wave_ms",
-                                                    "rebalance_block_ms", "large_ms", "rebuild_ms", "pyramid_ms")}
+    reb_ms = pf.rebalance_seg_ms + pf.rebalance_wave_ms + pf.rebalance_block_ms + pf.large_ms
+    reb_bytes = st.alg_bytes_seg + st.alg_bytes_wave + st.alg_bytes_block + st.alg_bytes_large
+    phases = {k: round(getattr(pf, k), 3) for k in ("sort_ms", "locate_ms", "elect_ms", "rebalance_seg_ms",
+                                                    "rebalance_wave_ms", "rebalance_block_ms", "large_ms",
+                                                    "rebuild_ms", "pyramid_ms")}
+    windows = {str(l): int(st.windows_per_level[l]) for l in range(32) if st.windows_per_level[l]}
 
     cpu = None
     if not args.no_cpu:
@@ -234,6 +237,8 @@ def main():
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
                      "launches": int(launches)},
         "cpu_baseline": cpu,
+        "windows_per_level": windows,
+        "rebuilds": int(st.rebuilds),
         "final_state": {"edges": r["edges"], "capacity": r["capacity"], "tree_height": r["height"]},
     }
     print(json.dumps(out))

@@ -84,6 +84,7 @@ typedef struct lpma_stats {
     uint64_t alg_bytes_wave;         /* algorithmic bytes, 16W + 8k + 4m, per rebalance class */
     uint64_t alg_bytes_block;
     uint64_t alg_bytes_large;
+    uint64_t alg_bytes_seg;
 } lpma_stats_t;
 
 /* Per-phase device time (ms) accumulated while profiling is on. */
@@ -96,8 +97,10 @@ typedef struct lpma_profile {
     double large_ms;
     double rebuild_ms;
     double pyramid_ms;
+    double rebalance_seg_ms;
     uint64_t launches_wave;
     uint64_t launches_block;
+    uint64_t launches_seg;
 } lpma_profile_t;
 
 /* Fills the reference defaults (32 slots, 0.42 / 1.0). */

@@ -69,14 +69,15 @@ class Stats(ctypes.Structure):
                 ("windows_per_level", ctypes.c_uint64 * 32), ("slots_per_level", ctypes.c_uint64 * 32),
                 ("rebuilds", ctypes.c_uint64), ("rebuild_slots", ctypes.c_uint64), ("marker_moves", ctypes.c_uint64),
                 ("alg_bytes_wave", ctypes.c_uint64), ("alg_bytes_block", ctypes.c_uint64),
-                ("alg_bytes_large", ctypes.c_uint64)]
+                ("alg_bytes_large", ctypes.c_uint64), ("alg_bytes_seg", ctypes.c_uint64)]
 
 
 class Profile(ctypes.Structure):
     _fields_ = [("sort_ms", ctypes.c_double), ("locate_ms", ctypes.c_double), ("elect_ms", ctypes.c_double),
                 ("rebalance_wave_ms", ctypes.c_double), ("rebalance_block_ms", ctypes.c_double),
                 ("large_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double), ("pyramid_ms", ctypes.c_double),
-                ("launches_wave", ctypes.c_uint64), ("launches_block", ctypes.c_uint64)]
+                ("rebalance_seg_ms", ctypes.c_double), ("launches_wave", ctypes.c_uint64),
+                ("launches_block", ctypes.c_uint64), ("launches_seg", ctypes.c_uint64)]
 
 
 _vp = ctypes.c_void_p

@@ -39,7 +39,8 @@ static constexpr u32 FLAG_LARGE = 8u;         // a chosen window exceeds the LDS
 static constexpr int CLASS_WAVE = 0;   // one wave64 per window
 static constexpr int CLASS_BLOCK = 1;  // one 256-thread workgroup per window
 static constexpr int CLASS_LARGE = 2;  // multi-workgroup global path
-static constexpr int NUM_CLASSES = 3;
+static constexpr int CLASS_SEG = 3;    // single-segment window, S/2 lanes (several per wave)
+static constexpr int NUM_CLASSES = 4;
 
 // LDS capacities of the two in-LDS classes: E = old live + updates, and the
 // number of segments in the window
@@ -60,6 +61,7 @@ enum StatSlot {
     ST_ALG_BYTES_WAVE = 66,   // algorithmic bytes, per rebalance class
     ST_ALG_BYTES_BLOCK = 67,
     ST_ALG_BYTES_LARGE = 68,
+    ST_ALG_BYTES_SEG = 69,
     ST_NUM = 80,
 };
 // statistics are spread over ST_ROWS copies (row = block index mod ST_ROWS)
@@ -137,7 +139,8 @@ __device__ __forceinline__ bool lds_class_ok(u64 E, u32 L) {
     return E <= BLOCK_E_CAP && (1ull << L) <= BLOCK_SEG_CAP;
 }
 
-__device__ __forceinline__ int work_class(u64 E, u32 L) {
+__device__ __forceinline__ int work_class(u64 E, u32 L, u32 log2S) {
+    if (L == 0 && E <= (2ull << log2S)) return CLASS_SEG;
     if (E <= WAVE_E_CAP && (1ull << L) <= WAVE_SEG_CAP) return CLASS_WAVE;
     if (lds_class_ok(E, L)) return CLASS_BLOCK;
     return CLASS_LARGE;

@@ -162,13 +162,13 @@ struct lpma {
     };
     std::vector<Ev> events;
     std::vector<hipEvent_t> ev_pool;
-    double phase_ms[8] = {0};
-    u64 launches_wave = 0, launches_block = 0;
+    double phase_ms[9] = {0};
+    u64 launches_wave = 0, launches_block = 0, launches_seg = 0;
 };
 
 namespace {
 
-enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR };
+enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR, PH_SEG };
 
 struct DevGuard {
     int prev = -1;
@@ -446,9 +446,15 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
             CHECKED(ensure(h, h->work[0], n_eff));
             CHECKED(ensure(h, h->work[1], n_eff));
             CHECKED(ensure(h, h->work[2], n_eff));
+            CHECKED(ensure(h, h->work[3], n_eff));
             bb = batch_bufs(h);
             PhaseTimer t(h, st, PH_ELECT);
             HIPCHECK(elect_windows(h->st, n_eff, bb, st));
+        }
+        {
+            PhaseTimer t(h, st, PH_SEG);
+            HIPCHECK(rebalance_seg(h->st, bb, n_eff, st));
+            h->launches_seg++;
         }
         {
             PhaseTimer t(h, st, PH_WAVE);
@@ -748,6 +754,7 @@ int lpma_get_stats(lpma_t *h, lpma_stats_t *out) {
     out->alg_bytes_wave = s[ST_ALG_BYTES_WAVE];
     out->alg_bytes_block = s[ST_ALG_BYTES_BLOCK];
     out->alg_bytes_large = s[ST_ALG_BYTES_LARGE];
+    out->alg_bytes_seg = s[ST_ALG_BYTES_SEG];
     return LPMA_OK;
 }
 
@@ -778,10 +785,12 @@ int lpma_get_profile(lpma_t *h, lpma_profile_t *out) {
     out->large_ms = h->phase_ms[PH_LARGE];
     out->rebuild_ms = h->phase_ms[PH_REBUILD];
     out->pyramid_ms = h->phase_ms[PH_PYR];
+    out->rebalance_seg_ms = h->phase_ms[PH_SEG];
     out->launches_wave = h->launches_wave;
     out->launches_block = h->launches_block;
+    out->launches_seg = h->launches_seg;
     for (double &v : h->phase_ms) v = 0;
-    h->launches_wave = h->launches_block = 0;
+    h->launches_wave = h->launches_block = h->launches_seg = 0;
     return LPMA_OK;
 }
 

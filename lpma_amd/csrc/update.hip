@@ -198,7 +198,7 @@ __global__ void k_density(DevState d, Density dn, const u32 *__restrict__ cseg, 
             }
             rlevel[j] = l;
             if (l >= d.H) local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
-            else if (work_class(C + (u1 - u0), l) == CLASS_LARGE) local_flags = FLAG_LARGE;
+            else if (work_class(C + (u1 - u0), l, d.log2S) == CLASS_LARGE) local_flags = FLAG_LARGE;
         }
     }
     // one atomic per wave
@@ -226,7 +226,7 @@ __global__ void k_claim(DevState d, const u32 *__restrict__ cseg, const u32 *__r
 
 __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, u32 n_eff, Work *w0, Work *w1, Work *w2,
-                          Status *status) {
+                          Work *w3, Status *status) {
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
     Work wk;
@@ -250,7 +250,7 @@ __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *_
             wk.u1 = u1;
             wk.C = C;
             wk.M = (u32)((i32)C + pd[u1] - pd[u0]);
-            cls = work_class((u64)C + (u1 - u0), L);
+            cls = work_class((u64)C + (u1 - u0), L, d.log2S);
         }
     }
     // one atomic per wave and class for the list appends
@@ -265,7 +265,7 @@ __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *_
         if (lane == first) base = atomicAdd(&status->nwork[c], (u32)__popcll(m));
         base = __shfl(base, first, 64);
         if (cls == c) {
-            Work *dst = c == CLASS_WAVE ? w0 : (c == CLASS_BLOCK ? w1 : w2);
+            Work *dst = c == CLASS_WAVE ? w0 : (c == CLASS_BLOCK ? w1 : (c == CLASS_LARGE ? w2 : w3));
             dst[base + __popcll(m & lt)] = wk;
         }
     }
@@ -283,7 +283,8 @@ __global__ void k_reset_owner(DevState d, const u32 *__restrict__ cseg, const u3
 hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st) {
     u32 g = grid_for(n_eff, 256);
     k_claim<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
-    k_leaders<<<g, 256, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.status);
+    k_leaders<<<g, 256, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.work[3],
+                                 b.status);
     k_reset_owner<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
     return hipGetLastError();
 }
@@ -547,6 +548,110 @@ __global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work 
                                                          u64 *stats) {
     rebalance_loop<256, BLOCK_E_CAP, BLOCK_SEG_CAP>(d, work, &status->nwork[CLASS_BLOCK], ckey, pd, stats,
                                                     ST_ALG_BYTES_BLOCK);
+}
+
+// Single-segment windows (level 0, E <= 2S): a group of G = S/2 lanes
+// (at most 64) rebalances one segment, so a wave works on 64/G segments at
+// once and no workgroup barrier is needed.  With one segment the even spread
+// is the merged order itself, front-packed, so every old key and every
+// insert stores straight to its rank; the old keys were staged in the
+// group's LDS slice first, so the in-place stores cannot overwrite a key
+// another lane has yet to read.
+static constexpr int SEG_TPB = 256;
+
+__global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Work *__restrict__ work,
+                                                           const Status *__restrict__ status,
+                                                           const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
+                                                           u64 *stats_all, u32 log2G) {
+    extern __shared__ u64 lds[];
+    const u32 G = 1u << log2G;
+    const u32 S = 1u << d.log2S;
+    const u32 SPL = S >> log2G;  // slots per lane
+    const u32 lane_g = threadIdx.x & (G - 1);
+    const u32 grp = threadIdx.x >> log2G;
+    const u32 groups_per_block = SEG_TPB >> log2G;
+    u64 *buf = lds + (u64)grp * (2 * S);                                   // old keys then updates
+    i32 *P = (i32 *)(lds + (u64)groups_per_block * 2 * S) + (u64)grp * (2 * S + 2);  // update prefix
+    const u32 nw = status->nwork[CLASS_SEG];
+    u64 bytes = 0, upd = 0, markers = 0, wins = 0;
+    for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += gridDim.x * groups_per_block) {
+        const Work wk = work[w];
+        const u32 C = wk.C;
+        const u32 k = wk.u1 - wk.u0;
+        const u64 base = (u64)wk.a << d.log2S;
+        for (u32 r = 0; r < SPL; ++r) {
+            u32 t = lane_g * SPL + r;
+            if (t < C) buf[t] = d.keys[base + t];
+        }
+        const i32 pbase = pd[wk.u0];
+        for (u32 q = lane_g; q < k; q += G) buf[C + q] = ckey[wk.u0 + q];
+        for (u32 q = lane_g; q <= k; q += G) P[q] = pd[wk.u0 + q] - pbase;
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
+        const u32 M = (u32)((i32)C + P[k]);
+        const u64 *U = buf + C;
+        for (u32 r = 0; r < SPL; ++r) {
+            u32 t = lane_g * SPL + r;
+            if (t >= C) break;
+            u64 x = buf[t];
+            u32 lo = 0, hi = k;
+            while (lo < hi) {
+                u32 m = (lo + hi) >> 1;
+                if (U[m] < x) lo = m + 1; else hi = m;
+            }
+            if (lo < k && U[lo] == x) continue;  // deleted
+            u32 o = t + P[lo];
+            if (o < S) {
+                d.keys[base + o] = x;
+                if (is_marker(x) && (u32)(x >> 32) < d.V) {
+                    d.row_off[(u32)(x >> 32) + 1] = (u32)(base + o);
+                    ++markers;
+                }
+            }
+        }
+        for (u32 q = lane_g; q < k; q += G) {
+            if (P[q + 1] - P[q] != 1) continue;
+            u64 y = U[q];
+            u32 lo = 0, hi = C;
+            while (lo < hi) {
+                u32 m = (lo + hi) >> 1;
+                if (buf[m] < y) lo = m + 1; else hi = m;
+            }
+            u32 o = lo + P[q];
+            if (o < S) d.keys[base + o] = y;  // inserts are never markers
+        }
+        // slots emptied by deletes
+        for (u32 t = M + lane_g; t < C; t += G) d.keys[base + t] = KEY_NONE;
+        if (lane_g == 0) {
+            d.cnt[wk.a] = M;
+            bytes += 16ull * S + 8ull * k;
+            upd += k;
+            wins += 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    markers = wave_sum(markers);
+    bytes = wave_sum(bytes);
+    upd = wave_sum(upd);
+    wins = wave_sum(wins);
+    u64 *row = stats_all + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    if (lane_id() == 0 && wins) {
+        atomicAdd((unsigned long long *)&row[ST_WINDOWS + 0], (unsigned long long)wins);
+        atomicAdd((unsigned long long *)&row[ST_UPDATES], (unsigned long long)upd);
+        atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)markers);
+        atomicAdd((unsigned long long *)&row[ST_ALG_BYTES_SEG], (unsigned long long)(bytes + 4ull * markers));
+    }
+}
+
+hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 max_windows, hipStream_t st) {
+    u32 S = 1u << d.log2S;
+    u32 log2G = d.log2S >= 7 ? 6 : (d.log2S == 0 ? 0 : d.log2S - 1);
+    u32 groups_per_block = SEG_TPB >> log2G;
+    size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
+    u64 blocks = (max_windows + groups_per_block - 1) / groups_per_block;
+    u32 grid = (u32)(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks));
+    k_rebalance_seg<<<grid, SEG_TPB, lds, st>>>(d, b.work[CLASS_SEG], b.status, b.ckey, b.pd, b.stats, log2G);
+    return hipGetLastError();
 }
 
 hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
