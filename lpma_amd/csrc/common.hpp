@@ -11,7 +11,8 @@
 //   sep[nseg]       u64 separator per segment: every key of segment s lies in
 //                   [sep[s], sep[s+1]); sep[0] = 0.  A key routes to the last
 //                   segment whose separator is <= key.
-//   owner[2nseg-1]  u32 scratch for window election, all SEG_NONE between batches.
+//   owner[2nseg]    u64 window claims of the current batch, tagged with the
+//                   batch epoch (epoch << 32 | ~run), so no reset is needed.
 //   row_off[V+1]    u32, row_off[r+1] = slot of row r's marker key (r<<32|~0u)
 //                   (GPMA convention, gpma.cuh:340-344), row_off[0] = 0.
 #pragma once
@@ -69,11 +70,11 @@ enum StatSlot {
 static constexpr u32 ST_ROWS = 64;
 
 struct Status {
+    u32 pyr_ticket;   // last-workgroup-done counter of the pyramid kernel (0 between launches)
     u32 flags;
     u32 n_eff;        // effective updates after dedup / presence check
     i32 delta;        // inserts - deletes among them
     u32 nwork[NUM_CLASSES];
-    u32 pad[2];
 };
 
 struct Work {
@@ -90,12 +91,13 @@ struct DevState {
     u32 *cnt;
     u32 *pyr;
     u64 *sep;
-    u32 *owner;
+    u64 *owner;
     u32 *row_off;
     u32 log2S;
     u32 H;
     u32 V;
     u32 vbits;
+    u32 epoch;  // batch counter tagging window claims
 };
 
 struct Density {

@@ -10,6 +10,8 @@ hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStr
 u64 sort_scratch_words(u64 n);
 hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, u32 *flags, hipStream_t st);
 u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st);
+u64 *pack_and_sort(const u64 *a, u64 na, u32 op_a, const u64 *b, u64 nb, u32 op_b, u32 V, u32 vbits, u32 lo_bit,
+                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
 
 // update.hip
 struct BatchBufs {
@@ -29,7 +31,7 @@ struct BatchBufs {
 };
 
 u32 locate_blocks(u64 n);
-hipError_t batch_begin(Status *status, hipStream_t st);
+hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st);
 hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st);
@@ -43,7 +45,7 @@ hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 
 hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);
-hipError_t rebuild_pyramid(const DevState &d, hipStream_t st);
+hipError_t rebuild_pyramid(const DevState &d, bool full, hipStream_t st);
 hipError_t make_markers(u64 *out, u32 V, hipStream_t st);
 hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st);
 hipError_t fill_u64(u64 *p, u64 n, u64 v, hipStream_t st);

@@ -53,7 +53,7 @@ struct Pool {
     void trim_locked() {
         for (auto &kv : free_blocks)
             for (void *p : kv.second) {
-                hipFree(p);
+                (void)hipFree(p);
                 reserved -= kv.first;
                 cached -= kv.first;
             }
@@ -137,7 +137,8 @@ struct lpma {
     Density dn;
 
     DBuf<u64> keys;
-    DBuf<u32> cnt, pyr, owner, row_off;
+    DBuf<u32> cnt, pyr, row_off;
+    DBuf<u64> owner;
     DBuf<u64> sep;
 
     // batch scratch
@@ -173,13 +174,13 @@ enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUI
 struct DevGuard {
     int prev = -1;
     explicit DevGuard(int dev) {
-        hipGetDevice(&prev);
-        if (prev != dev) hipSetDevice(dev);
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
     }
     ~DevGuard() {
         int cur;
-        hipGetDevice(&cur);
-        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+        (void)hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
     }
 };
 
@@ -199,7 +200,7 @@ int ensure(lpma *h, DBuf<T> &b, u64 n) {
     if (b.p) {
         // queued kernels may still read the old block; it must not reach
         // another stream through the shared pool before they finish
-        hipStreamSynchronize(h->cur_stream);
+        (void)hipStreamSynchronize(h->cur_stream);
         pool_free(h->device, b.p);
     }
     b.p = (T *)p;
@@ -221,7 +222,7 @@ hipEvent_t take_event(lpma *h) {
         return e;
     }
     hipEvent_t e;
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
     return e;
 }
 
@@ -233,13 +234,13 @@ struct PhaseTimer {
     PhaseTimer(lpma *h_, hipStream_t s, int p) : h(h_), st(s), phase(p) {
         if (h->profiling) {
             a = take_event(h);
-            hipEventRecord(a, st);
+            (void)hipEventRecord(a, st);
         }
     }
     ~PhaseTimer() {
         if (a) {
             hipEvent_t b = take_event(h);
-            hipEventRecord(b, st);
+            (void)hipEventRecord(b, st);
             h->events.push_back({a, b, phase});
         }
     }
@@ -247,9 +248,9 @@ struct PhaseTimer {
 
 void collect_events(lpma *h) {
     for (auto &e : h->events) {
-        hipEventSynchronize(e.b);
+        (void)hipEventSynchronize(e.b);
         float ms = 0;
-        hipEventElapsedTime(&ms, e.a, e.b);
+        (void)hipEventElapsedTime(&ms, e.a, e.b);
         h->phase_ms[e.phase] += ms;
         h->ev_pool.push_back(e.a);
         h->ev_pool.push_back(e.b);
@@ -285,8 +286,8 @@ u32 fit_height(const lpma *h, u64 m) {
 bool height_ok(const lpma *h, u32 H) { return H <= 31 && ((u64)h->cfg.segment_slots << H) <= (1ull << 32); }
 
 struct Geometry {
-    DBuf<u64> keys, sep;
-    DBuf<u32> cnt, pyr, owner;
+    DBuf<u64> keys, sep, owner;
+    DBuf<u32> cnt, pyr;
 };
 
 int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
@@ -298,11 +299,13 @@ int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
         release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
         return r;
     }
-    fill_u64(g.keys.p, cap, KEY_NONE, st);
-    fill_u32(g.cnt.p, nseg, 0, st);
-    fill_u32(g.pyr.p, nseg, 0, st);
-    fill_u64(g.sep.p, nseg, 0, st);
-    fill_u32(g.owner.p, 2 * nseg, SEG_NONE, st);
+    if (fill_u64(g.keys.p, cap, KEY_NONE, st) != hipSuccess || fill_u32(g.cnt.p, nseg, 0, st) != hipSuccess ||
+        fill_u32(g.pyr.p, nseg, 0, st) != hipSuccess || fill_u64(g.sep.p, nseg, 0, st) != hipSuccess ||
+        fill_u64(g.owner.p, 2 * nseg, 0, st) != hipSuccess) {
+        (void)hipGetLastError();
+        release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
+        return LPMA_ERR_DEVICE;
+    }
     return LPMA_OK;
 }
 
@@ -376,7 +379,7 @@ int respread_into_new(lpma *h, u32 H, u64 m, hipStream_t st) {
     CHECKED(alloc_geometry(h, H, g, st));
     DevState d2 = state_with(h, H, g);
     HIPCHECK(rebuild_spread(d2, h->merged.p, m, st));
-    HIPCHECK(rebuild_pyramid(d2, st));
+    HIPCHECK(rebuild_pyramid(d2, true, st));
     // old arrays may still be read by queued kernels: wait before they
     // return to the pool
     HIPCHECK(hipStreamSynchronize(st));
@@ -394,15 +397,21 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
     }
     if (n > 0x7FFFFFF0ull) return LPMA_ERR_TOO_LARGE;
     CHECKED(ensure_batch(h, n));
+    if (++h->st.epoch == 0) {
+        // claims are tagged with the epoch: after a wrap, clear the old tags
+        h->st.epoch = 1;
+        HIPCHECK(fill_u64(h->owner.p, 2 * (1ull << h->st.H), 0, st));
+    }
     BatchBufs bb = batch_bufs(h);
-    const u32 nbits = 2 * h->vbits + 1;
+    // bit 0 of a packed key is the operation: sorted only when the batch mixes them
+    const u32 lo_bit = (na && nb && opa != opb) ? 0 : 1;
+    const u32 hi_bit = 2 * h->vbits + 1;
     u64 *pk;
     {
         PhaseTimer t(h, st, PH_SORT);
-        HIPCHECK(batch_begin(h->d_status, st));
-        HIPCHECK(pack_batch(a, na, opa, h->V, h->vbits, h->pk0.p, &h->d_status->flags, st));
-        HIPCHECK(pack_batch(b, nb, opb, h->V, h->vbits, h->pk0.p + na, &h->d_status->flags, st));
-        pk = radix_sort(h->pk0.p, h->pk1.p, n, nbits, h->sort_scr.p, st);
+        HIPCHECK(batch_begin(h->d_status, h->st, st));
+        pk = pack_and_sort(a, na, opa, b, nb, opb, h->V, h->vbits, lo_bit, hi_bit, h->pk0.p, h->pk1.p,
+                           h->sort_scr.p, &h->d_status->flags, st);
         HIPCHECK(hipGetLastError());
     }
     {
@@ -436,7 +445,7 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
             CHECKED(respread_into_new(h, newH, M, st));
         } else {
             HIPCHECK(rebuild_spread(h->st, h->merged.p, M, st));
-            HIPCHECK(rebuild_pyramid(h->st, st));
+            HIPCHECK(rebuild_pyramid(h->st, true, st));
         }
         h->rebuilds++;
         h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
@@ -479,7 +488,7 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
         }
         {
             PhaseTimer t(h, st, PH_PYR);
-            HIPCHECK(rebuild_pyramid(h->st, st));
+            HIPCHECK(rebuild_pyramid(h->st, false, st));
         }
     }
     h->live = M;
@@ -536,7 +545,7 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
         return LPMA_ERR_DEVICE;
     }
     int dev = cfg.device;
-    if (dev < 0) hipGetDevice(&dev);
+    if (dev < 0) (void)hipGetDevice(&dev);
     if (dev >= ndev) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(dev);
     hipStream_t st = (hipStream_t)stream;
@@ -565,12 +574,21 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
         lpma_destroy(h);
         return LPMA_ERR_OUT_OF_MEMORY;
     }
-    hipMemsetAsync(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64), st);
+    if (hipMemsetAsync(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64), st) != hipSuccess ||
+        hipMemsetAsync(h->d_status, 0, sizeof(Status), st) != hipSuccess) {
+        (void)hipGetLastError();
+        lpma_destroy(h);
+        return LPMA_ERR_DEVICE;
+    }
     if ((r = ensure(h, h->row_off, (u64)num_vertices + 1)) != LPMA_OK) {
         lpma_destroy(h);
         return r;
     }
-    fill_u32(h->row_off.p, (u64)num_vertices + 1, 0, st);
+    if (fill_u32(h->row_off.p, (u64)num_vertices + 1, 0, st) != hipSuccess) {
+        (void)hipGetLastError();
+        lpma_destroy(h);
+        return LPMA_ERR_DEVICE;
+    }
     h->st.row_off = h->row_off.p;
 
     // one marker per row, spread at the smallest fitting height
@@ -608,9 +626,9 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
 int lpma_destroy(lpma_t *h) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
     collect_events(h);
-    for (auto e : h->ev_pool) hipEventDestroy(e);
+    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
     release(h, h->pk0); release(h, h->pk1); release(h, h->ukey); release(h, h->ckey); release(h, h->merged);
@@ -618,10 +636,10 @@ int lpma_destroy(lpma_t *h) {
     release(h, h->gpre); release(h, h->scan_scr); release(h, h->pd); release(h, h->part);
     release(h, h->lchunk); release(h, h->lhdr); release(h, h->lmbase);
     for (int c = 0; c < NUM_CLASSES; ++c) release(h, h->work[c]);
-    if (h->d_status) hipFree(h->d_status);
-    if (h->h_status) hipHostFree(h->h_status);
-    if (h->d_stats) hipFree(h->d_stats);
-    if (h->d_err) hipFree(h->d_err);
+    if (h->d_status) (void)hipFree(h->d_status);
+    if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->d_stats) (void)hipFree(h->d_stats);
+    if (h->d_err) (void)hipFree(h->d_err);
     delete h;
     return LPMA_OK;
 }

@@ -64,8 +64,10 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s
         u64 bb = __ballot(bit);
         peers &= bit ? bb : ~bb;
     }
-    u32 rank = __popcll(peers & ((1ull << lane) - 1ull));
-    if (valid && rank == 0) s.wcnt[w][digit] = __popcll(peers);
+    const u32 rank = __popcll(peers & ((1ull << lane) - 1ull));
+    const bool leader = valid && rank == 0;
+    const u32 cnt = __popcll(peers);
+    if (leader) s.wcnt[w][digit] = cnt;
     __syncthreads();
     if (valid) {
         u32 off = s.running[digit];
@@ -73,13 +75,42 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s
         pos = off + rank;
     }
     __syncthreads();
-    for (u32 d = threadIdx.x; d < RS_RADIX; d += NT) {
-        u32 add = 0;
-#pragma unroll
-        for (int v = 0; v < NT / 64; ++v) { add += s.wcnt[v][d]; s.wcnt[v][d] = 0; }
-        s.running[d] += add;
+    // only the entries written this round change: their leaders fold them in
+    if (leader) {
+        atomicAdd(&s.running[digit], cnt);
+        s.wcnt[w][digit] = 0;
     }
     __syncthreads();
+}
+
+// Packs the batch (a with op_a, then b with op_b) into sortable keys and
+// counts the first radix digit of every tile in the same pass.
+__global__ __launch_bounds__(RS_TPB) void k_pack_hist(const u64 *__restrict__ a, u64 na, u32 op_a,
+                                                       const u64 *__restrict__ b, u64 nb, u32 op_b, u32 V, u32 vbits,
+                                                       u64 *__restrict__ out, u32 *__restrict__ flags, u32 shift,
+                                                       u32 items, u32 *__restrict__ hist, u32 nblocks,
+                                                       int block_major) {
+    __shared__ u32 h[RS_RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const u64 n = na + nb;
+    u64 base = (u64)blockIdx.x * items * RS_TPB;
+    bool bad = false;
+    for (u32 r = 0; r < items; ++r) {
+        u64 i = base + (u64)r * RS_TPB + threadIdx.x;
+        if (i >= n) break;
+        u64 k = i < na ? a[i] : b[i - na];
+        u32 op = i < na ? op_a : op_b;
+        u32 src = (u32)(k >> 32), dst = (u32)k;
+        if (src >= V || dst >= V) { bad = true; src = 0; dst = 0; }
+        u64 p = (((((u64)src) << vbits) | dst) << 1) | op;
+        out[i] = p;
+        atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
+    }
+    if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
+    __syncthreads();
+    if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
+    else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
 // per-tile digit histogram; layout block-major (hist[b*RADIX + d]) when
@@ -139,14 +170,14 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ i
 }
 
 // Whole sort inside one workgroup's LDS for n <= RS_SMALL_CAP.
-__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, u32 nbits) {
+__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, u32 lo_bit, u32 hi_bit) {
     __shared__ u64 buf[2][RS_SMALL_CAP];
     __shared__ RankLds<RS_SMALL_TPB> s;
     __shared__ u32 h[RS_RADIX];
     __shared__ u32 scan_tmp[RS_RADIX / 64 + 1];
     for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) buf[0][i] = keys[i];
     int cur = 0;
-    for (u32 shift = 0; shift < nbits; shift += RS_BITS) {
+    for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
         for (u32 i = threadIdx.x; i < RS_RADIX; i += RS_SMALL_TPB) h[i] = 0;
         rank_init<RS_SMALL_TPB>(s);
         __syncthreads();
@@ -270,12 +301,12 @@ hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, 
     return hipGetLastError();
 }
 
-// Sorts keys[0..n) on bits [0, nbits).  Returns the buffer that holds the
-// result (keys or alt).
+// Sorts keys[0..n) on bits [lo_bit, hi_bit).  Returns the buffer that holds
+// the result (keys or alt).
 u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st) {
     if (n <= 1) return keys;
     if (n <= (u64)RS_SMALL_CAP) {
-        k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(keys, n, nbits);
+        k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(keys, n, 0, nbits);
         return keys;
     }
     const u32 items = rs_items(n);
@@ -287,8 +318,40 @@ u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t
     u64 *src = keys, *dst = alt;
     for (u32 shift = 0; shift < nbits; shift += RS_BITS) {
         k_rs_hist<<<nb, RS_TPB, 0, st>>>(src, n, shift, items, hist, nb, block_major);
-        if (!block_major) device_excl_scan(hist, (u64)RS_RADIX * nb, hist, sscr, st);
+        if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nb, hist, sscr, st);
         k_rs_scatter<<<nb, RS_TPB, 0, st>>>(src, dst, n, shift, items, hist, nb, block_major);
+        u64 *t = src; src = dst; dst = t;
+    }
+    return src;
+}
+
+// Packs a batch (a: op_a, b: op_b) and sorts the packed keys on bits
+// [lo_bit, hi_bit): the first histogram is taken while packing.
+u64 *pack_and_sort(const u64 *a, u64 na, u32 op_a, const u64 *b, u64 nb, u32 op_b, u32 V, u32 vbits, u32 lo_bit,
+                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
+    const u64 n = na + nb;
+    if (n == 0) return k0;
+    if (n <= (u64)RS_SMALL_CAP) {
+        (void)pack_batch(a, na, op_a, V, vbits, k0, flags, st);
+        (void)pack_batch(b, nb, op_b, V, vbits, k0 + na, flags, st);
+        if (n > 1) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, lo_bit, hi_bit);
+        return k0;
+    }
+    const u32 items = rs_items(n);
+    const u64 tile = (u64)items * RS_TPB;
+    const u32 nblk = (u32)((n + tile - 1) / tile);
+    const int block_major = nblk <= RS_PREFIX_MAX_TILES;
+    u32 *hist = scratch;
+    u32 *sscr = scratch + (u64)RS_RADIX * nblk + 1;
+    u64 *src = k0, *dst = k1;
+    for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
+        if (shift == lo_bit)
+            k_pack_hist<<<nblk, RS_TPB, 0, st>>>(a, na, op_a, b, nb, op_b, V, vbits, k0, flags, shift, items, hist,
+                                                 nblk, block_major);
+        else
+            k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, shift, items, hist, nblk, block_major);
+        if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);
+        k_rs_scatter<<<nblk, RS_TPB, 0, st>>>(src, dst, n, shift, items, hist, nblk, block_major);
         u64 *t = src; src = dst; dst = t;
     }
     return src;

@@ -24,48 +24,72 @@
 namespace pma {
 
 static constexpr int LOC_TPB = 256;
+static constexpr int LOC_ITEMS = 1;
+static constexpr int LOC_TILE = LOC_TPB * LOC_ITEMS;
 
-u32 locate_blocks(u64 n) { return (u32)((n + LOC_TPB - 1) / LOC_TPB); }
+u32 locate_blocks(u64 n) { return (u32)((n + LOC_TILE - 1) / LOC_TILE); }
 
-__global__ void k_batch_begin(Status *s) {
-    if (threadIdx.x == 0) {
-        s->flags = 0;
-        s->n_eff = 0;
-        s->delta = 0;
-        for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
-    }
-}
+// Zeroes the batch status and finishes the count pyramid left by the
+// previous batch: the levels above the per-workgroup chunks of k_pyr_chunk
+// (nseg >> 11 entries and fewer), so no separate launch is needed for them.
+__global__ __launch_bounds__(1024) void k_batch_begin(Status *s, DevState d);
 
-hipError_t batch_begin(Status *status, hipStream_t st) {
-    k_batch_begin<<<1, 64, 0, st>>>(status);
+hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st) {
+    k_batch_begin<<<1, 1024, 0, st>>>(status, d);
     return hipGetLastError();
 }
 
 // ------------------------------------------------ dedup, locate, presence
 
+// Dedup (keep the last of equal keys: deletes sort before inserts), route
+// each key to its segment, and resolve it against the keys present there.
 __global__ __launch_bounds__(LOC_TPB) void k_locate(DevState d, const u64 *__restrict__ pk, u64 n,
                                                     u64 *__restrict__ ukey, u32 *__restrict__ useg,
                                                     i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
-    u64 i = (u64)blockIdx.x * LOC_TPB + threadIdx.x;
-    i32 eff = 0, dl = 0;
-    if (i < n) {
-        u64 p = pk[i];
-        u64 kk = p >> 1;
-        u32 op = (u32)(p & 1);
-        u32 sg = SEG_NONE;
-        bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == kk);
-        if (!dup) {
-            u32 vmask = (u32)((1ull << d.vbits) - 1);
-            u64 key = ((kk >> d.vbits) << 32) | (u64)((u32)kk & vmask);
-            u32 s = locate_segment(d.sep, d.H, key);
-            u32 present = segment_holds(d, s, key) ? 1u : 0u;
-            if (op != present) {
-                eff = 1;
-                dl = op ? 1 : -1;
-                sg = s | (op ? OP_BIT : 0u);
-            }
-            ukey[i] = key;
+    const u32 vmask = (u32)((1ull << d.vbits) - 1);
+    const u64 i0 = (u64)blockIdx.x * LOC_TILE + (u64)threadIdx.x * LOC_ITEMS;
+    u64 key[LOC_ITEMS];
+    u32 op[LOC_ITEMS], b[LOC_ITEMS];
+    bool act[LOC_ITEMS];
+#pragma unroll
+    for (int r = 0; r < LOC_ITEMS; ++r) {
+        u64 i = i0 + r;
+        act[r] = false;
+        key[r] = 0;
+        op[r] = 0;
+        b[r] = 0;
+        if (i < n) {
+            u64 p = pk[i];
+            u64 kk = p >> 1;
+            op[r] = (u32)(p & 1);
+            bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == kk);
+            act[r] = !dup;
+            key[r] = ((kk >> d.vbits) << 32) | (u64)((u32)kk & vmask);
+            if (dup) useg[i] = SEG_NONE;
         }
+    }
+    for (u32 step = (1u << d.H) >> 1; step; step >>= 1) {
+        u64 v[LOC_ITEMS];
+#pragma unroll
+        for (int r = 0; r < LOC_ITEMS; ++r) v[r] = d.sep[b[r] + step];
+#pragma unroll
+        for (int r = 0; r < LOC_ITEMS; ++r)
+            if (v[r] <= key[r]) b[r] += step;
+    }
+    i32 eff = 0, dl = 0;
+#pragma unroll
+    for (int r = 0; r < LOC_ITEMS; ++r) {
+        u64 i = i0 + r;
+        if (!act[r]) continue;
+        u32 s = b[r];
+        u32 present = segment_holds(d, s, key[r]) ? 1u : 0u;
+        u32 sg = SEG_NONE;
+        if (op[r] != present) {
+            eff += 1;
+            dl += op[r] ? 1 : -1;
+            sg = s | (op[r] ? OP_BIT : 0u);
+        }
+        ukey[i] = key[r];
         useg[i] = sg;
     }
     __shared__ i32 tmp[2][LOC_TPB / 64];
@@ -73,10 +97,10 @@ __global__ __launch_bounds__(LOC_TPB) void k_locate(DevState d, const u64 *__res
     if (lane_id() == 0) { tmp[0][threadIdx.x >> 6] = ce; tmp[1][threadIdx.x >> 6] = cd; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        i32 a = 0, b = 0;
-        for (int w = 0; w < LOC_TPB / 64; ++w) { a += tmp[0][w]; b += tmp[1][w]; }
+        i32 a = 0, c = 0;
+        for (int w = 0; w < LOC_TPB / 64; ++w) { a += tmp[0][w]; c += tmp[1][w]; }
         part_cnt[blockIdx.x] = a;
-        part_dl[blockIdx.x] = b;
+        part_dl[blockIdx.x] = c;
     }
 }
 
@@ -102,23 +126,65 @@ __global__ __launch_bounds__(1024) void k_scan_parts(i32 *__restrict__ pc, i32 *
     }
 }
 
+static constexpr u32 COMPACT_SELF_PREFIX_MAX = 2048;  // blocks
+
+// Stream compaction of the effective updates.  With few blocks every block
+// derives its prefix (and the totals) from the per-block partials itself;
+// otherwise k_scan_parts has already scanned them.
 __global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u32 *__restrict__ useg, u64 n,
-                                                     const i32 *__restrict__ pc, const i32 *__restrict__ pdl,
-                                                     u64 *__restrict__ ckey, u32 *__restrict__ cseg,
-                                                     i32 *__restrict__ pd) {
-    u64 i = (u64)blockIdx.x * LOC_TPB + threadIdx.x;
-    u32 sg = i < n ? useg[i] : SEG_NONE;
-    i32 eff = sg != SEG_NONE ? 1 : 0;
-    i32 dl = eff ? ((sg & OP_BIT) ? 1 : -1) : 0;
+                                                     const i32 *__restrict__ pc, const i32 *__restrict__ pdl, u32 nb,
+                                                     int self_prefix, u64 *__restrict__ ckey, u32 *__restrict__ cseg,
+                                                     i32 *__restrict__ pd, Status *__restrict__ status) {
     __shared__ i32 tmp[LOC_TPB / 64 + 1];
+    __shared__ i32 red[4][LOC_TPB / 64];
+    i32 base_c, base_d;
+    if (self_prefix) {
+        i32 a = 0, b = 0, ta = 0, tb = 0;
+        for (u32 x = threadIdx.x; x < nb; x += LOC_TPB) {
+            i32 v = pc[x], w = pdl[x];
+            ta += v;
+            tb += w;
+            if (x < blockIdx.x) { a += v; b += w; }
+        }
+        a = wave_sum(a); b = wave_sum(b); ta = wave_sum(ta); tb = wave_sum(tb);
+        if (lane_id() == 0) {
+            red[0][threadIdx.x >> 6] = a; red[1][threadIdx.x >> 6] = b;
+            red[2][threadIdx.x >> 6] = ta; red[3][threadIdx.x >> 6] = tb;
+        }
+        __syncthreads();
+        base_c = 0; base_d = 0; ta = 0; tb = 0;
+        for (int w = 0; w < LOC_TPB / 64; ++w) {
+            base_c += red[0][w]; base_d += red[1][w]; ta += red[2][w]; tb += red[3][w];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->n_eff = (u32)ta;
+            status->delta = tb;
+            pd[ta] = tb;
+        }
+    } else {
+        base_c = pc[blockIdx.x];
+        base_d = pdl[blockIdx.x];
+    }
+    // blocked: thread t owns entries i0 .. i0 + LOC_ITEMS - 1
+    const u64 i0 = (u64)blockIdx.x * LOC_TILE + (u64)threadIdx.x * LOC_ITEMS;
+    u32 sg[LOC_ITEMS];
+    i32 ce = 0, cd = 0;
+#pragma unroll
+    for (int r = 0; r < LOC_ITEMS; ++r) {
+        sg[r] = (i0 + r) < n ? useg[i0 + r] : SEG_NONE;
+        if (sg[r] != SEG_NONE) { ce += 1; cd += (sg[r] & OP_BIT) ? 1 : -1; }
+    }
     i32 t1, t2;
-    i32 ex = block_excl_scan<LOC_TPB, i32>(eff, t1, tmp);
-    i32 dx = block_excl_scan<LOC_TPB, i32>(dl, t2, tmp);
-    if (eff) {
-        u64 j = (u64)pc[blockIdx.x] + ex;
-        ckey[j] = ukey[i];
-        cseg[j] = sg & ~OP_BIT;
-        pd[j] = pdl[blockIdx.x] + dx;
+    i32 ex = block_excl_scan<LOC_TPB, i32>(ce, t1, tmp) + base_c;
+    i32 dx = block_excl_scan<LOC_TPB, i32>(cd, t2, tmp) + base_d;
+#pragma unroll
+    for (int r = 0; r < LOC_ITEMS; ++r) {
+        if (sg[r] == SEG_NONE) continue;
+        ckey[ex] = ukey[i0 + r];
+        cseg[ex] = sg[r] & ~OP_BIT;
+        pd[ex] = dx;
+        ex += 1;
+        dx += (sg[r] & OP_BIT) ? 1 : -1;
     }
 }
 
@@ -126,8 +192,9 @@ hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const Bat
     u32 nb = locate_blocks(n);
     i32 *pc = b.part, *pdl = b.part + nb + 1;
     k_locate<<<nb, LOC_TPB, 0, st>>>(d, pk, n, b.ukey, b.useg, pc, pdl);
-    k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
-    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, pc, pdl, b.ckey, b.cseg, b.pd);
+    int self_prefix = nb <= COMPACT_SELF_PREFIX_MAX;
+    if (!self_prefix) k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
+    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd, b.status);
     return hipGetLastError();
 }
 
@@ -197,8 +264,14 @@ __global__ void k_density(DevState d, Density dn, const u32 *__restrict__ cseg, 
                 if (window_fits(dn, l, d.H, (u64)m)) break;
             }
             rlevel[j] = l;
-            if (l >= d.H) local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
-            else if (work_class(C + (u1 - u0), l, d.log2S) == CLASS_LARGE) local_flags = FLAG_LARGE;
+            if (l >= d.H) {
+                local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
+            } else {
+                if (work_class(C + (u1 - u0), l, d.log2S) == CLASS_LARGE) local_flags = FLAG_LARGE;
+                // claim the window; the smallest run index wins (atomic max of ~j)
+                atomicMax((unsigned long long *)&d.owner[owner_idx(nseg_of(d), l, (u64)s >> l)],
+                          (unsigned long long)(((u64)d.epoch << 32) | (u64)(~j)));
+            }
         }
     }
     // one atomic per wave
@@ -215,16 +288,7 @@ hipError_t density_check(const DevState &d, const Density &dn, u64 n, const Batc
 
 // ---------------------------------------------------------- window election
 
-__global__ void k_claim(DevState d, const u32 *__restrict__ cseg, const u32 *__restrict__ rlevel, u32 n_eff) {
-    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_eff) return;
-    u32 s = cseg[j];
-    if (j > 0 && cseg[j - 1] == s) return;
-    u32 L = rlevel[j];
-    atomicMin(&d.owner[owner_idx(nseg_of(d), L, (u64)s >> L)], j);
-}
-
-__global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+__global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, u32 n_eff, Work *w0, Work *w1, Work *w2,
                           Work *w3, Status *status) {
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,9 +300,17 @@ __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *_
         u32 L = leader ? rlevel[j] : 0;
         u64 nseg = nseg_of(d);
         u64 w = (u64)s >> L;
-        if (leader && d.owner[owner_idx(nseg, L, w)] != j) leader = false;
-        for (u32 l2 = L + 1; leader && l2 < d.H; ++l2)
-            if (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] != SEG_NONE) leader = false;  // inside a larger window
+        const u64 tag = (u64)d.epoch << 32;
+        if (leader && d.owner[owner_idx(nseg, L, w)] != (tag | (u64)(~j))) leader = false;
+        if (leader) {
+            // inside a larger chosen window?  the loads are independent, so
+            // they are issued together rather than as a dependent chain
+            bool dom = false;
+#pragma unroll 8
+            for (u32 l2 = L + 1; l2 < d.H; ++l2)
+                dom |= (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] >> 32) == (u64)d.epoch;
+            if (dom) leader = false;
+        }
         if (leader) {
             u64 a = w << L;
             u32 u0 = gallop_back(cseg, j, a);
@@ -253,39 +325,27 @@ __global__ void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *_
             cls = work_class((u64)C + (u1 - u0), L, d.log2S);
         }
     }
-    // one atomic per wave and class for the list appends
-    const u32 lane = lane_id();
-    const u64 lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int c = 0; c < NUM_CLASSES; ++c) {
-        u64 m = __ballot(cls == c);
-        if (!m) continue;
-        u32 first = (u32)__builtin_ctzll(m);
-        u32 base = 0;
-        if (lane == first) base = atomicAdd(&status->nwork[c], (u32)__popcll(m));
-        base = __shfl(base, first, 64);
-        if (cls == c) {
-            Work *dst = c == CLASS_WAVE ? w0 : (c == CLASS_BLOCK ? w1 : (c == CLASS_LARGE ? w2 : w3));
-            dst[base + __popcll(m & lt)] = wk;
-        }
+    // list appends: LDS counters per class, then one global atomic per
+    // workgroup and class (global atomics on one address serialise in L2)
+    __shared__ u32 ccount[NUM_CLASSES], cbase[NUM_CLASSES];
+    if (threadIdx.x < NUM_CLASSES) ccount[threadIdx.x] = 0;
+    __syncthreads();
+    u32 off = 0;
+    if (cls >= 0) off = atomicAdd(&ccount[cls], 1u);
+    __syncthreads();
+    if (threadIdx.x < NUM_CLASSES && ccount[threadIdx.x])
+        cbase[threadIdx.x] = atomicAdd(&status->nwork[threadIdx.x], ccount[threadIdx.x]);
+    __syncthreads();
+    if (cls >= 0) {
+        Work *dst = cls == CLASS_WAVE ? w0 : (cls == CLASS_BLOCK ? w1 : (cls == CLASS_LARGE ? w2 : w3));
+        dst[cbase[cls] + off] = wk;
     }
 }
 
-__global__ void k_reset_owner(DevState d, const u32 *__restrict__ cseg, const u32 *__restrict__ rlevel, u32 n_eff) {
-    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_eff) return;
-    u32 s = cseg[j];
-    if (j > 0 && cseg[j - 1] == s) return;
-    u32 L = rlevel[j];
-    d.owner[owner_idx(nseg_of(d), L, (u64)s >> L)] = SEG_NONE;
-}
-
 hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st) {
-    u32 g = grid_for(n_eff, 256);
-    k_claim<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
-    k_leaders<<<g, 256, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.work[3],
+    u32 g = grid_for(n_eff, 1024);
+    k_leaders<<<g, 1024, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.work[3],
                                  b.status);
-    k_reset_owner<<<g, 256, 0, st>>>(d, b.cseg, b.rlevel, (u32)n_eff);
     return hipGetLastError();
 }
 
@@ -550,7 +610,7 @@ __global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work 
                                                     ST_ALG_BYTES_BLOCK);
 }
 
-// Single-segment windows (level 0, E <= 2S): a group of G = S/2 lanes
+// Single-segment windows (level 0, E <= 2S): a group of G = S/4 lanes
 // (at most 64) rebalances one segment, so a wave works on 64/G segments at
 // once and no workgroup barrier is needed.  With one segment the even spread
 // is the merged order itself, front-packed, so every old key and every
@@ -645,7 +705,7 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Wor
 
 hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 max_windows, hipStream_t st) {
     u32 S = 1u << d.log2S;
-    u32 log2G = d.log2S >= 7 ? 6 : (d.log2S == 0 ? 0 : d.log2S - 1);
+    u32 log2G = d.log2S >= 8 ? 6 : (d.log2S <= 2 ? 0 : d.log2S - 2);
     u32 groups_per_block = SEG_TPB >> log2G;
     size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
     u64 blocks = (max_windows + groups_per_block - 1) / groups_per_block;
@@ -953,7 +1013,7 @@ hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 
                         u32 *scan_scratch, hipStream_t st) {
     u64 nseg = nseg_of(src);
     u64 W = nseg << src.log2S;
-    device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
+    (void)device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
     k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, 0, (u32)n_eff, merged, m);
     if (n_eff)
         k_push_ins<<<grid_for(n_eff, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, 0, (u32)n_eff,
@@ -1008,23 +1068,37 @@ __global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) {
     }
 }
 
-__global__ __launch_bounds__(1024) void k_pyr_top(DevState d) {
+__device__ void pyr_top_levels(const DevState &d) {
     const u64 nseg = nseg_of(d);
     for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
         u64 nw = nseg >> l;
         const u32 *in = d.pyr + pyr_off(nseg, l - 1);
         u32 *out = d.pyr + pyr_off(nseg, l);
-        for (u64 x = threadIdx.x; x < nw; x += 1024) out[x] = in[2 * x] + in[2 * x + 1];
+        for (u64 x = threadIdx.x; x < nw; x += blockDim.x) out[x] = in[2 * x] + in[2 * x + 1];
         __syncthreads();
     }
 }
 
-hipError_t rebuild_pyramid(const DevState &d, hipStream_t st) {
+__global__ __launch_bounds__(1024) void k_pyr_top(DevState d) { pyr_top_levels(d); }
+
+__global__ __launch_bounds__(1024) void k_batch_begin(Status *s, DevState d) {
+    if (threadIdx.x == 0) {
+        s->flags = 0;
+        s->n_eff = 0;
+        s->delta = 0;
+        for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
+    }
+    pyr_top_levels(d);
+}
+
+// full: also the levels above the chunks (otherwise the next batch_begin
+// computes them before anything reads them)
+hipError_t rebuild_pyramid(const DevState &d, bool full, hipStream_t st) {
     if (d.H == 0) return hipSuccess;
     u64 nseg = nseg_of(d);
     u32 nb = (u32)(nseg < PYR_CH ? 1 : nseg / PYR_CH);
     k_pyr_chunk<<<nb, 256, 0, st>>>(d);
-    if (d.H > PYR_CHL) k_pyr_top<<<1, 1024, 0, st>>>(d);
+    if (full && d.H > PYR_CHL) k_pyr_top<<<1, 1024, 0, st>>>(d);
     return hipGetLastError();
 }
 

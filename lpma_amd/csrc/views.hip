@@ -45,7 +45,7 @@ __global__ void k_export(DevState d, const u32 *__restrict__ gpre, u64 *__restri
 hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64 *edge_keys, u32 *row_std,
                             u64 *all_keys, u32 *csr_row, u32 *csr_col, hipStream_t st) {
     u64 nseg = nseg_of(d);
-    device_excl_scan(d.cnt, nseg, gpre, scan_scratch, st);
+    (void)device_excl_scan(d.cnt, nseg, gpre, scan_scratch, st);
     k_export<<<grid_for(nseg << d.log2S, 256, 16384), 256, 0, st>>>(d, gpre, edge_keys, row_std, all_keys, csr_row,
                                                                     csr_col);
     return hipGetLastError();
@@ -54,7 +54,8 @@ hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64
 // Exact edge membership (edge_query_rpma / edge_query_kernel,
 // rpma.cuh:2090-2190): 1 when the key is a live edge, 0 otherwise
 // (row markers and out-of-range keys are not edges).
-__global__ void k_edge_query(DevState d, const u64 *__restrict__ q, u64 n, u32 *__restrict__ res) {
+__global__ __launch_bounds__(256) void k_edge_query(DevState d, const u64 *__restrict__ q, u64 n,
+                                                     u32 *__restrict__ res) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
         u64 key = q[i];
         u32 src = (u32)(key >> 32), dst = (u32)key;
