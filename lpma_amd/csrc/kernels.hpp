@@ -34,9 +34,9 @@ u32 locate_blocks(u64 n);
 hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st);
 hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st);
-hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st);
+hipError_t elect_windows(const DevState &d, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
-hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 max_windows, hipStream_t st);
+hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 windows, hipStream_t st);
 
 // global (multi-workgroup) window path, in place, and the full rebuild
 hipError_t rebalance_large(const DevState &d, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st);

@@ -354,6 +354,9 @@ int ensure_batch(lpma *h, u64 n) {
         (r = ensure(h, h->cseg, n)) || (r = ensure(h, h->pd, n + 1)) || (r = ensure(h, h->rlevel, n)) ||
         (r = ensure(h, h->part, 2 * nb + 2)))
         return r;
+    u64 runs = std::min<u64>(n, 1ull << h->st.H);
+    for (int c = 0; c < NUM_CLASSES; ++c)
+        if ((r = ensure(h, h->work[c], runs))) return r;
     return LPMA_OK;
 }
 
@@ -419,6 +422,13 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
         HIPCHECK(locate_and_compact(h->st, pk, n, bb, st));
         HIPCHECK(density_check(h->st, h->dn, n, bb, st));
     }
+    {
+        // window election before the single host synchronisation, so the
+        // host learns the exact work of every rebalance class and sizes each
+        // grid to it (idle workgroups still cost dispatch time)
+        PhaseTimer t(h, st, PH_ELECT);
+        HIPCHECK(elect_windows(h->st, n, bb, st));
+    }
     HIPCHECK(hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     const Status s = *h->h_status;
@@ -450,41 +460,29 @@ int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32
         h->rebuilds++;
         h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
     } else {
-        u64 grid_runs = n_eff;
-        {
-            CHECKED(ensure(h, h->work[0], n_eff));
-            CHECKED(ensure(h, h->work[1], n_eff));
-            CHECKED(ensure(h, h->work[2], n_eff));
-            CHECKED(ensure(h, h->work[3], n_eff));
-            bb = batch_bufs(h);
-            PhaseTimer t(h, st, PH_ELECT);
-            HIPCHECK(elect_windows(h->st, n_eff, bb, st));
-        }
-        {
+        if (s.nwork[CLASS_SEG]) {
             PhaseTimer t(h, st, PH_SEG);
-            HIPCHECK(rebalance_seg(h->st, bb, n_eff, st));
+            HIPCHECK(rebalance_seg(h->st, bb, s.nwork[CLASS_SEG], st));
             h->launches_seg++;
         }
-        {
+        if (s.nwork[CLASS_WAVE]) {
             PhaseTimer t(h, st, PH_WAVE);
-            u32 g = (u32)std::min<u64>(grid_runs, 65535);
-            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_WAVE, g, st));
+            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
             h->launches_wave++;
         }
-        {
+        if (s.nwork[CLASS_BLOCK]) {
             PhaseTimer t(h, st, PH_BLOCK);
-            u32 g = (u32)std::min<u64>(grid_runs, 256 * 8);
-            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_BLOCK, g, st));
+            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
             h->launches_block++;
         }
-        if (s.flags & FLAG_LARGE) {
+        if (s.nwork[CLASS_LARGE]) {
             CHECKED(ensure(h, h->merged, M + 1));
-            CHECKED(ensure(h, h->lchunk, n_eff + 1));
-            CHECKED(ensure(h, h->lmbase, n_eff));
+            CHECKED(ensure(h, h->lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
+            CHECKED(ensure(h, h->lmbase, s.nwork[CLASS_LARGE]));
             CHECKED(ensure(h, h->lhdr, 4));
             bb = batch_bufs(h);
             PhaseTimer t(h, st, PH_LARGE);
-            HIPCHECK(rebalance_large(h->st, bb, h->merged.p, 256 * 8, st));
+            HIPCHECK(rebalance_large(h->st, bb, h->merged.p, 1024, st));
         }
         {
             PhaseTimer t(h, st, PH_PYR);

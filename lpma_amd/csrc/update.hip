@@ -289,8 +289,11 @@ hipError_t density_check(const DevState &d, const Density &dn, u64 n, const Batc
 // ---------------------------------------------------------- window election
 
 __global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
-                          const u32 *__restrict__ rlevel, u32 n_eff, Work *w0, Work *w1, Work *w2,
-                          Work *w3, Status *status) {
+                          const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
+                          Status *status) {
+    // a root rebalance or a resize redoes everything: no windows needed
+    if (status->flags & (FLAG_ROOT | FLAG_RESIZE | FLAG_INVALID_KEY)) return;
+    const u32 n_eff = status->n_eff;
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
     Work wk;
@@ -342,10 +345,11 @@ __global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restr
     }
 }
 
-hipError_t elect_windows(const DevState &d, u64 n_eff, const BatchBufs &b, hipStream_t st) {
-    u32 g = grid_for(n_eff, 1024);
-    k_leaders<<<g, 1024, 0, st>>>(d, b.cseg, b.pd, b.rlevel, (u32)n_eff, b.work[0], b.work[1], b.work[2], b.work[3],
-                                 b.status);
+// n bounds the number of runs (the batch size); the kernel reads the exact
+// count from the status
+hipError_t elect_windows(const DevState &d, u64 n, const BatchBufs &b, hipStream_t st) {
+    u32 g = grid_for(n, 1024);
+    k_leaders<<<g, 1024, 0, st>>>(d, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2], b.work[3], b.status);
     return hipGetLastError();
 }
 
@@ -703,12 +707,12 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Wor
     }
 }
 
-hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 max_windows, hipStream_t st) {
+hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 windows, hipStream_t st) {
     u32 S = 1u << d.log2S;
     u32 log2G = d.log2S >= 8 ? 6 : (d.log2S <= 2 ? 0 : d.log2S - 2);
     u32 groups_per_block = SEG_TPB >> log2G;
     size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
-    u64 blocks = (max_windows + groups_per_block - 1) / groups_per_block;
+    u64 blocks = (windows + groups_per_block - 1) / groups_per_block;
     u32 grid = (u32)(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks));
     k_rebalance_seg<<<grid, SEG_TPB, lds, st>>>(d, b.work[CLASS_SEG], b.status, b.ckey, b.pd, b.stats, log2G);
     return hipGetLastError();
