@@ -145,6 +145,21 @@ def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
     return n / sum(times), times
 
 
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/ (None when not measured for this
+    kernel and batch size)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_rmat20_b{batch}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"].get(kernel)
+        return None if k is None else k["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +171,8 @@ def main():
     ap.add_argument("--segment-slots", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-batches", type=int, default=15)
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the store is "
@@ -179,6 +196,14 @@ def main():
     r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False)
     # pass 2: same stream with per-launch HIP events for the kernel roofline
     rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True)
+
+    if args.copy_calib:
+        src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device=dev)
+        dstc = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dstc.copy_(src)
+        torch.cuda.synchronize()
+        del src, dstc
 
     value = r["n_updates"] / r["wall"]
     lat = r["lat_ms"]
@@ -233,7 +258,7 @@ def main():
         "rebalance_alg_GBps": reb_bytes / (reb_ms / 1e3) / 1e9 if reb_ms > 0 else 0.0,
         "phase_ms_total": phases,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, B),
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
                      "launches": int(launches)},
         "cpu_baseline": cpu,
