@@ -164,6 +164,21 @@ int lpma_neighbor_count(lpma_t *h, const uint32_t *d_srcs, uint64_t n, uint32_t 
 int lpma_neighbor_gather(lpma_t *h, const uint32_t *d_srcs, uint64_t n, const uint32_t *d_offsets, uint32_t *d_out,
                          void *stream);
 
+/* Breadth-first search from `src` over the gapped layout (the reference runs
+ * nvGraph's BFS on a CSR copy, test_algos.cu:136-186).  d_levels (V entries)
+ * receives the level of every vertex, 0xFFFFFFFF when unreached; `depth` the
+ * number of levels, `edges_scanned` the live out-edges of the expanded
+ * vertices.  Synchronises the stream once per level. */
+int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned, void *stream);
+
+/* PageRank over the gapped layout with nvGraph's formulation
+ * (include/pagerank.hxx:66-88): from the uniform vector, until the L1 change
+ * is below `tolerance` or `max_iter` iterations, then L1-normalised.  FP32
+ * values and accumulation.  d_rank has V entries.  Synchronises the stream
+ * once per iteration. */
+int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, float *d_rank, uint32_t *iterations,
+                  float *last_delta, void *stream);
+
 /* Counts layout invariant violations on the device (0 = consistent). */
 int lpma_check(lpma_t *h, uint64_t *violations, void *stream);
 

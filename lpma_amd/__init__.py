@@ -102,6 +102,9 @@ SIGNATURES = [
     ("lpma_neighbor_count", _int, [_vp, _vp, _u64, _vp, _vp]),
     ("lpma_neighbor_gather", _int, [_vp, _vp, _u64, _vp, _vp, _vp]),
     ("lpma_check", _int, [_vp, ctypes.POINTER(_u64), _vp]),
+    ("lpma_bfs", _int, [_vp, _u32, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _vp]),
+    ("lpma_pagerank", _int, [_vp, ctypes.c_float, ctypes.c_float, _u32, _vp, ctypes.POINTER(_u32),
+                             ctypes.POINTER(ctypes.c_float), _vp]),
     ("lpma_get_stats", _int, [_vp, ctypes.POINTER(Stats)]),
     ("lpma_reset_stats", _int, [_vp]),
     ("lpma_set_profiling", _int, [_vp, _int]),
@@ -352,6 +355,26 @@ class PMA:
         _check(lib().lpma_neighbor_gather(self._h, srcs.data_ptr(), n, offs32.data_ptr(), out.data_ptr(),
                                           _stream_ptr(self.stream)), "neighbor_gather")
         return offs32, out[:total]
+
+    def bfs(self, src):
+        """(levels int32 device tensor, depth, live edges scanned); unreached = -1."""
+        torch = _torch()
+        lv = torch.empty(self.num_vertices, dtype=torch.int32, device=self.device)
+        depth = _u32(0)
+        scanned = _u64(0)
+        _check(lib().lpma_bfs(self._h, int(src), lv.data_ptr(), ctypes.byref(depth), ctypes.byref(scanned),
+                              _stream_ptr(self.stream)), "bfs")
+        return lv, int(depth.value), int(scanned.value)
+
+    def pagerank(self, damping=0.85, tolerance=1e-6, max_iter=50):
+        """(rank float32 device tensor, iterations, last L1 change)."""
+        torch = _torch()
+        pr = torch.empty(self.num_vertices, dtype=torch.float32, device=self.device)
+        it = _u32(0)
+        dl = ctypes.c_float(0)
+        _check(lib().lpma_pagerank(self._h, float(damping), float(tolerance), int(max_iter), pr.data_ptr(),
+                                   ctypes.byref(it), ctypes.byref(dl), _stream_ptr(self.stream)), "pagerank")
+        return pr, int(it.value), float(dl.value)
 
     def check(self):
         v = _u64(0)

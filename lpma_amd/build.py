@@ -33,6 +33,7 @@ SOURCES = [
     ("sort.hip", "hip"),
     ("update.hip", "hip"),
     ("views.hip", "hip"),
+    ("analytics.hip", "hip"),
     ("lpma.cpp", "hip"),
     ("gen.cpp", "cxx"),
 ]

@@ -1,0 +1,208 @@
+// analytics.hip -- BFS and PageRank straight over the gapped layout.
+//
+// The reference runs nvGraph on a CSR copy (test_algos.cu:44-90 PageRank,
+// :136-186 BFS; test.cu:170-182 converts with to_csr_rpma first).  Here both
+// read the PMA in place: row r's out-edges are the live slots between the
+// markers of rows r-1 and r, i.e. slots (row_off[r], row_off[r+1]).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pma {
+
+static constexpr u32 INF_LEVEL = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void slot_range(const DevState &d, u32 r, u64 &lo, u64 &hi) {
+    lo = r == 0 ? 0 : (u64)d.row_off[r] + 1;
+    hi = d.row_off[r + 1];
+}
+
+// ------------------------------------------------------------------ BFS
+//
+// Level-synchronous, top-down.  A level expands the frontier's slot ranges
+// with one thread per slot (the range lengths are scanned, each thread finds
+// its vertex by binary search), so hub vertices are split over many threads.
+
+__global__ void k_bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (u64)gridDim.x * blockDim.x)
+        levels[i] = i == src ? 0u : INF_LEVEL;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[0] = src;
+        counters[0] = 0;  // next frontier size
+        counters[1] = 0;  // live edges scanned
+    }
+}
+
+__global__ void k_bfs_len(DevState d, const u32 *__restrict__ q, u32 qn, u32 *__restrict__ len) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < qn; i += (u64)gridDim.x * blockDim.x) {
+        u64 lo, hi;
+        slot_range(d, q[i], lo, hi);
+        len[i] = (u32)(hi - lo);
+    }
+}
+
+__global__ void k_bfs_expand(DevState d, const u32 *__restrict__ q, u32 qn, const u32 *__restrict__ off,
+                             u32 *__restrict__ levels, u32 level, u32 *__restrict__ qnext, u32 *__restrict__ counters) {
+    const u64 total = off[qn];
+    const u32 lane = lane_id();
+    u32 scanned = 0;
+    for (u64 t0 = (u64)blockIdx.x * blockDim.x; t0 < total; t0 += (u64)gridDim.x * blockDim.x) {
+        u64 t = t0 + threadIdx.x;
+        u32 v = INF_LEVEL;
+        bool found = false;
+        if (t < total) {
+            // frontier entry owning slot t: last i with off[i] <= t
+            u32 lo = 0, hi = qn;
+            while (hi - lo > 1) {
+                u32 m = (lo + hi) >> 1;
+                if (off[m] <= t) lo = m; else hi = m;
+            }
+            u64 slo, shi;
+            slot_range(d, q[lo], slo, shi);
+            u64 key = d.keys[slo + (t - off[lo])];
+            if (key != KEY_NONE) {
+                ++scanned;
+                v = (u32)key;
+                if (levels[v] == INF_LEVEL && atomicCAS(&levels[v], INF_LEVEL, level + 1) == INF_LEVEL) found = true;
+            }
+        }
+        u64 m = __ballot(found);
+        if (m) {
+            u32 first = (u32)__builtin_ctzll(m);
+            u32 base = 0;
+            if (lane == first) base = atomicAdd(&counters[0], (u32)__popcll(m));
+            base = __shfl(base, first, 64);
+            if (found) qnext[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+        }
+    }
+    scanned = wave_sum(scanned);
+    if (lane == 0 && scanned) atomicAdd(&counters[1], scanned);
+}
+
+hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipStream_t st) {
+    k_bfs_init<<<grid_for(V, 256, 16384), 256, 0, st>>>(levels, V, src, queue, counters);
+    return hipGetLastError();
+}
+
+hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
+                     u32 *qnext, u32 *counters, hipStream_t st) {
+    k_bfs_len<<<grid_for(qn, 256, 16384), 256, 0, st>>>(d, q, qn, len);
+    (void)device_excl_scan(len, qn, len, scan_scratch, st);
+    k_bfs_expand<<<2048, 256, 0, st>>>(d, q, qn, len, levels, level, qnext, counters);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- PageRank
+//
+// nvGraph's formulation (include/pagerank.hxx:66-88): with a = alpha on
+// dangling vertices and 1-alpha elsewhere,
+//     pr' = alpha * A^T (pr / outdeg) + (sum_u a_u pr_u) / n,
+// iterated from the uniform vector; here until the L1 change is below the
+// tolerance or max_iter, then normalised by the L1 norm.  Values and
+// accumulation are FP32; the push over the slot array uses FP32 atomics.
+
+// out-degree of every row from marker ranks (rank = live slots before)
+__global__ void k_outdeg(DevState d, const u32 *__restrict__ gpre, u32 *__restrict__ outdeg) {
+    const u32 Smask = (1u << d.log2S) - 1;
+    for (u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x; r < d.V; r += (u64)gridDim.x * blockDim.x) {
+        u64 hs = d.row_off[r + 1];
+        u64 rank_hi = (u64)gpre[hs >> d.log2S] + (hs & Smask);
+        u64 rank_lo = 0;
+        if (r > 0) {
+            u64 ls = d.row_off[r];
+            rank_lo = (u64)gpre[ls >> d.log2S] + (ls & Smask) + 1;
+        }
+        outdeg[r] = (u32)(rank_hi - rank_lo);
+    }
+}
+
+// contributions pr/outdeg, the dangling mass, and a cleared accumulator
+__global__ void k_pr_prep(const float *__restrict__ pr, const u32 *__restrict__ outdeg, u32 V, float *__restrict__ c,
+                          float *__restrict__ acc, float *__restrict__ red) {
+    float dang = 0.f;
+    for (u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x; u < V; u += (u64)gridDim.x * blockDim.x) {
+        u32 od = outdeg[u];
+        float p = pr[u];
+        c[u] = od ? p / (float)od : 0.f;
+        if (!od) dang += p;
+        acc[u] = 0.f;
+    }
+    dang = wave_sum(dang);
+    if (lane_id() == 0 && dang != 0.f) atomicAdd(&red[0], dang);
+}
+
+__global__ void k_pr_push(DevState d, const float *__restrict__ c, float *__restrict__ acc) {
+    const u64 W = nseg_of(d) << d.log2S;
+    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < W; t += (u64)gridDim.x * blockDim.x) {
+        u64 key = d.keys[t];
+        if (key == KEY_NONE || is_marker(key)) continue;
+        atomicAdd(&acc[(u32)key], c[(u32)(key >> 32)]);
+    }
+}
+
+// pr' = alpha*acc + (alpha*dangling + (1-alpha)*sum(pr)) / n; L1 change and sum
+__global__ void k_pr_finish(const float *__restrict__ acc, const float *__restrict__ pr, float *__restrict__ out,
+                            u32 V, float alpha, const float *__restrict__ red_in, float *__restrict__ red_out) {
+    const float gamma = (alpha * red_in[0] + (1.f - alpha) * red_in[1]) / (float)V;
+    float dl = 0.f, sum = 0.f;
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x) {
+        float p = alpha * acc[v] + gamma;
+        out[v] = p;
+        dl += fabsf(p - pr[v]);
+        sum += p;
+    }
+    dl = wave_sum(dl);
+    sum = wave_sum(sum);
+    if (lane_id() == 0) {
+        atomicAdd(&red_out[2], dl);
+        atomicAdd(&red_out[1], sum);
+    }
+}
+
+__global__ void k_pr_init(float *pr, u32 V, float *red) {
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x)
+        pr[v] = 1.f / (float)V;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        red[0] = 0.f;  // dangling mass of the current vector
+        red[1] = 1.f;  // its sum
+        red[2] = 0.f;  // L1 change
+    }
+}
+
+__global__ void k_pr_reset(float *red_next) {
+    if (threadIdx.x == 0) { red_next[0] = 0.f; red_next[1] = 0.f; red_next[2] = 0.f; }
+}
+
+__global__ void k_pr_scale(float *pr, u32 V, const float *red) {
+    const float inv = 1.f / red[1];
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x) pr[v] *= inv;
+}
+
+hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outdeg, hipStream_t st) {
+    (void)device_excl_scan(d.cnt, nseg_of(d), gpre, scan_scratch, st);
+    k_outdeg<<<grid_for(d.V, 256, 16384), 256, 0, st>>>(d, gpre, outdeg);
+    return hipGetLastError();
+}
+
+hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st) {
+    k_pr_init<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, V, red);
+    return hipGetLastError();
+}
+
+// one iteration: pr -> out; red holds (dangling, sum) of pr on entry and
+// (dangling of out, sum of out, L1 change) on exit in red_next
+hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, float *c, float *acc, float *out,
+                      float alpha, float *red, float *red_next, hipStream_t st) {
+    const u32 V = d.V;
+    k_pr_reset<<<1, 64, 0, st>>>(red_next);
+    k_pr_prep<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, outdeg, V, c, acc, red);
+    k_pr_push<<<grid_for(nseg_of(d) << d.log2S, 256, 65536), 256, 0, st>>>(d, c, acc);
+    k_pr_finish<<<grid_for(V, 256, 16384), 256, 0, st>>>(acc, pr, out, V, alpha, red, red_next);
+    return hipGetLastError();
+}
+
+hipError_t pr_scale(float *pr, u32 V, const float *red, hipStream_t st) {
+    k_pr_scale<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, V, red);
+    return hipGetLastError();
+}
+
+}  // namespace pma

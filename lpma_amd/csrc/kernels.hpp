@@ -58,4 +58,14 @@ hipError_t neighbor_count(const DevState &d, const u32 *srcs, u64 n, u32 *len, h
 hipError_t neighbor_gather(const DevState &d, const u32 *srcs, u64 n, const u32 *offs, u32 *out, hipStream_t st);
 hipError_t check_invariants(const DevState &d, u32 *err, hipStream_t st);
 
+// analytics.hip
+hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipStream_t st);
+hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
+                     u32 *qnext, u32 *counters, hipStream_t st);
+hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outdeg, hipStream_t st);
+hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st);
+hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, float *c, float *acc, float *out,
+                      float alpha, float *red, float *red_next, hipStream_t st);
+hipError_t pr_scale(float *pr, u32 V, const float *red, hipStream_t st);
+
 }  // namespace pma

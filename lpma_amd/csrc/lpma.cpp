@@ -145,6 +145,9 @@ struct lpma {
     DBuf<u64> pk0, pk1, ukey, ckey, merged;
     DBuf<u32> sort_scr, useg, cseg, rlevel, gpre, scan_scr, lchunk, lhdr;
     DBuf<u64> lmbase;
+    // analytics scratch
+    DBuf<u32> aq0, aq1, alen, acnt;
+    DBuf<float> af0, af1, af2, ared;
     DBuf<i32> pd, part;
     DBuf<Work> work[NUM_CLASSES];
     Status *d_status = nullptr;
@@ -633,6 +636,8 @@ int lpma_destroy(lpma_t *h) {
     release(h, h->sort_scr); release(h, h->useg); release(h, h->cseg); release(h, h->rlevel);
     release(h, h->gpre); release(h, h->scan_scr); release(h, h->pd); release(h, h->part);
     release(h, h->lchunk); release(h, h->lhdr); release(h, h->lmbase);
+    release(h, h->aq0); release(h, h->aq1); release(h, h->alen); release(h, h->acnt);
+    release(h, h->af0); release(h, h->af1); release(h, h->af2); release(h, h->ared);
     for (int c = 0; c < NUM_CLASSES; ++c) release(h, h->work[c]);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
@@ -745,6 +750,79 @@ int lpma_check(lpma_t *h, uint64_t *violations, void *stream) {
     HIPCHECK(hipMemcpyAsync(&e, h->d_err, sizeof(u32), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     *violations = e;
+    return LPMA_OK;
+}
+
+int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned, void *stream) {
+    if (!h || !d_levels) return LPMA_ERR_INVALID_ARGUMENT;
+    if (src >= h->V) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    h->cur_stream = st;
+    const u64 V = h->V;
+    CHECKED(ensure(h, h->aq0, V + 1));
+    CHECKED(ensure(h, h->aq1, V + 1));
+    CHECKED(ensure(h, h->alen, V + 2));
+    CHECKED(ensure(h, h->scan_scr, scan_scratch_words(V + 1) + 8));
+    CHECKED(ensure(h, h->acnt, 4));
+    HIPCHECK(bfs_init(d_levels, h->V, src, h->aq0.p, h->acnt.p, st));
+    u32 *q = h->aq0.p, *qn_buf = h->aq1.p;
+    u32 qn = 1, level = 0;
+    u64 scanned = 0;
+    u32 cnt[2];
+    while (qn) {
+        HIPCHECK(bfs_level(h->st, q, qn, h->alen.p, h->scan_scr.p, d_levels, level, qn_buf, h->acnt.p, st));
+        HIPCHECK(hipMemcpyAsync(cnt, h->acnt.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(fill_u32(h->acnt.p, 2, 0, st));
+        scanned += cnt[1];
+        qn = cnt[0];
+        std::swap(q, qn_buf);
+        ++level;
+    }
+    if (depth) *depth = level;
+    if (edges_scanned) *edges_scanned = scanned;
+    return LPMA_OK;
+}
+
+int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, float *d_rank, uint32_t *iterations,
+                  float *last_delta, void *stream) {
+    if (!h || !d_rank || !(damping >= 0.f && damping <= 1.f) || max_iter == 0) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(h->device);
+    hipStream_t st = (hipStream_t)stream;
+    h->cur_stream = st;
+    const u64 V = h->V;
+    u64 nseg = 1ull << h->st.H;
+    CHECKED(ensure(h, h->gpre, nseg + 1));
+    CHECKED(ensure(h, h->scan_scr, scan_scratch_words(nseg) + 8));
+    CHECKED(ensure(h, h->aq0, V));   // out-degrees
+    CHECKED(ensure(h, h->af0, V));   // contributions
+    CHECKED(ensure(h, h->af1, V));   // accumulator
+    CHECKED(ensure(h, h->af2, V));   // second rank vector
+    CHECKED(ensure(h, h->ared, 8));
+    HIPCHECK(pr_outdeg(h->st, h->gpre.p, h->scan_scr.p, h->aq0.p, st));
+    float *red[2] = {h->ared.p, h->ared.p + 4};
+    float *vec[2] = {d_rank, h->af2.p};
+    HIPCHECK(pr_init(vec[0], h->V, red[0], st));
+    u32 it = 0;
+    float delta = 0.f;
+    int cur = 0;
+    while (it < max_iter) {
+        HIPCHECK(pr_iterate(h->st, vec[cur], h->aq0.p, h->af0.p, h->af1.p, vec[cur ^ 1], damping, red[cur],
+                            red[cur ^ 1], st));
+        cur ^= 1;
+        ++it;
+        float r[4];
+        HIPCHECK(hipMemcpyAsync(r, red[cur], sizeof(r), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        delta = r[2];
+        if (delta < tolerance) break;
+    }
+    HIPCHECK(pr_scale(vec[cur], h->V, red[cur], st));
+    if (vec[cur] != d_rank) HIPCHECK(hipMemcpyAsync(d_rank, vec[cur], V * sizeof(float), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (iterations) *iterations = it;
+    if (last_delta) *last_delta = delta;
     return LPMA_OK;
 }
 

@@ -73,6 +73,11 @@ def lib():
         L.or_pma_windows_at_level.restype = ctypes.c_uint64
         L.or_pma_export.argtypes = [ctypes.c_void_p, _u64p]
         L.or_pma_export.restype = ctypes.c_uint64
+        L.or_bfs.argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        L.or_bfs.restype = ctypes.c_uint32
+        L.or_pagerank.argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_double)]
+        L.or_pagerank.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -109,6 +114,32 @@ def row_offsets(keys, V):
     if r != 0:
         raise ValueError("key with src >= V")
     return ro
+
+
+def csr_of(keys, V):
+    """(row offsets, dst ids) of a sorted edge-key array."""
+    keys = _as_u64(keys)
+    ro = row_offsets(keys, V)
+    col = np.ascontiguousarray((keys & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+    return ro, col
+
+
+def bfs(keys, V, src):
+    """BFS levels (0xFFFFFFFF unreached) over the sorted edge keys."""
+    ro, col = csr_of(keys, V)
+    lv = np.empty(int(V), dtype=np.uint32)
+    lib().or_bfs(ro.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p), int(V), int(src), lv.ctypes.data_as(_u32p))
+    return lv
+
+
+def pagerank(keys, V, alpha=0.85, iters=50):
+    """FP64 PageRank after exactly `iters` iterations (L1-normalised), and the
+    L1 change of the last iteration."""
+    ro, col = csr_of(keys, V)
+    pr = np.empty(int(V), dtype=np.float64)
+    d = lib().or_pagerank(ro.ctypes.data_as(_u32p), col.ctypes.data_as(_u32p), int(V), float(alpha), int(iters),
+                          pr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return pr, d
 
 
 def is_sorted_unique(keys):

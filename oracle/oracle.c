@@ -50,6 +50,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 
 typedef uint64_t u64;
 typedef uint32_t u32;
@@ -548,3 +549,67 @@ u64 or_pma_windows_at_level(const or_pma *p, u32 l) { return l < 64 ? p->windows
 /* Compacted live keys (including row markers), like get_data_rpma
  * (rpma.cuh:1735-1761).  Returns the count. */
 u64 or_pma_export(const or_pma *p, u64 *out) { return pma_gather(p, 0, (u64)1 << p->H, out); }
+
+/* ------------------------------------------------------------------------- */
+/* Analytics oracles over a standard CSR (ro[V+1], col = dst ids).            */
+/* ------------------------------------------------------------------------- */
+
+/* BFS levels from src (0xFFFFFFFF = unreached), level-synchronous queue;
+ * the reference calls nvGraph's BFS on a CSR (test_algos.cu:136-186,
+ * include/bfs.hxx).  Levels do not depend on the traversal order.  Returns the
+ * number of levels (depth + 1). */
+u32 or_bfs(const u32 *ro, const u32 *col, u32 V, u32 src, u32 *levels) {
+    for (u32 v = 0; v < V; ++v) levels[v] = 0xFFFFFFFFu;
+    if (src >= V) return 0;
+    u32 *q = (u32 *)malloc((size_t)V * sizeof(u32));
+    u32 head = 0, tail = 0;
+    q[tail++] = src;
+    levels[src] = 0;
+    u32 depth = 0;
+    while (head < tail) {
+        u32 u = q[head++];
+        for (u32 e = ro[u]; e < ro[u + 1]; ++e) {
+            u32 v = col[e];
+            if (levels[v] == 0xFFFFFFFFu) {
+                levels[v] = levels[u] + 1;
+                if (levels[v] > depth) depth = levels[v];
+                q[tail++] = v;
+            }
+        }
+    }
+    free(q);
+    return depth + 1;
+}
+
+/* PageRank in FP64 with nvGraph's formulation (include/pagerank.hxx:66-88):
+ * pr' = alpha * A^T (pr / outdeg) + (alpha * dangling(pr) + (1-alpha) * sum(pr)) / V,
+ * from the uniform vector, exactly `iters` iterations, then divided by its L1
+ * norm.  Returns the L1 change of the last iteration. */
+double or_pagerank(const u32 *ro, const u32 *col, u32 V, double alpha, u32 iters, double *pr) {
+    double *acc = (double *)malloc((size_t)V * sizeof(double));
+    for (u32 v = 0; v < V; ++v) pr[v] = 1.0 / (double)V;
+    double delta = 0.0;
+    for (u32 it = 0; it < iters; ++it) {
+        double dang = 0.0, sum = 0.0;
+        for (u32 v = 0; v < V; ++v) acc[v] = 0.0;
+        for (u32 u = 0; u < V; ++u) {
+            u32 od = ro[u + 1] - ro[u];
+            sum += pr[u];
+            if (!od) { dang += pr[u]; continue; }
+            double c = pr[u] / (double)od;
+            for (u32 e = ro[u]; e < ro[u + 1]; ++e) acc[col[e]] += c;
+        }
+        double gamma = (alpha * dang + (1.0 - alpha) * sum) / (double)V;
+        delta = 0.0;
+        for (u32 v = 0; v < V; ++v) {
+            double p = alpha * acc[v] + gamma;
+            delta += fabs(p - pr[v]);
+            pr[v] = p;
+        }
+    }
+    double l1 = 0.0;
+    for (u32 v = 0; v < V; ++v) l1 += pr[v];
+    for (u32 v = 0; v < V; ++v) pr[v] /= l1;
+    free(acc);
+    return delta;
+}
