@@ -241,3 +241,23 @@ def test_batch_sizes_cover_sort_paths(n):
     t.mixed(rand_keys(rng, V, n // 2), t.ref[rng.integers(0, t.ref.size, n // 2)])
     t.compare()
     t.close()
+
+
+def test_config1_full_size():
+    """Config 1 at full size: V = 2^16, 10^6 distinct uniform edges (seed 1,
+    no self loops), then 10 insert batches of 10^4 absent edges and 10 delete
+    batches of 10^4 live edges, compared with the oracle after every batch."""
+    say("config 1, full size")
+    V = 1 << 16
+    t = Triple(V, model=False)
+    t.insert(L.gen_uniform(V, 1_000_000, 1))
+    t.compare(layout=False)
+    for b in range(10):
+        t.insert(L.gen_absent(V, t.ref, 10_000, 100 + b))
+        t.compare(layout=False)
+    rng = np.random.default_rng(11)
+    for b in range(10):
+        t.delete(t.ref[rng.choice(t.ref.size, 10_000, replace=False)])
+        t.compare(layout=False)
+    assert t.pma.edges == 1_000_000
+    t.close()
