@@ -45,6 +45,7 @@ typedef enum lpma_status {
 typedef enum lpma_op { LPMA_OP_DELETE = 0, LPMA_OP_INSERT = 1 } lpma_op;
 
 typedef struct lpma lpma_t;
+typedef struct lpma_group lpma_group_t;
 
 /* Structure configuration.  The reference hard-codes 32-slot segments
  * (rpma.cuh:1769) and density bounds lower = 0.42*len, upper = len at every
@@ -196,6 +197,36 @@ int lpma_pool_trim(void);
 uint64_t lpma_pool_reserved_bytes(void);
 
 const char *lpma_strerror(int code);
+
+/* ---------------------------------------------------------------------------
+ * Batched form: many independent PMAs updated by one pipeline run per round.
+ * The reference updates one RPMA per update_rpma call (rpma.cuh:1549); it has
+ * no counterpart for this.  A round sorts, locates, density-checks and elects
+ * windows for every member at once (one host synchronisation), then runs
+ * each rebalance class once for all members.  Each member ends exactly as if
+ * it had been updated alone with lpma_update_mixed(d_ins[i], d_del[i]).
+ *
+ * Members must live on one device and outlive the group; a handle appears
+ * at most once.  Members with different segment sizes, or too many ids to
+ * fit above their sort keys, go to separate partitions (one run each).
+ * An invalid key anywhere fails the whole round and changes no member. */
+int lpma_group_create(lpma_group_t **out, lpma_t *const *members, uint32_t n);
+int lpma_group_destroy(lpma_group_t *g);
+
+/* One round: member i gets n_ins[i] inserts at d_ins[i] and n_del[i] deletes
+ * at d_del[i] (deletes first).  Any of the four arrays may be NULL (no
+ * updates of that kind); pointers with a zero count are ignored. */
+int lpma_group_update(lpma_group_t *g, const uint64_t *const *d_ins, const uint64_t *n_ins,
+                      const uint64_t *const *d_del, const uint64_t *n_del, void *stream);
+
+/* Number of pipeline runs a round takes (partitions of the members). */
+uint32_t lpma_group_partitions(const lpma_group_t *g);
+
+/* Window statistics and phase timing of the group's rounds (the members'
+ * own counters cover only their lpma_update calls). */
+int lpma_group_get_stats(lpma_group_t *g, lpma_stats_t *out);
+int lpma_group_set_profiling(lpma_group_t *g, int on);
+int lpma_group_get_profile(lpma_group_t *g, lpma_profile_t *out);
 
 /* ---------------------------------------------------------------------------
  * Input formats (host memory).  Generators use MT19937-64 with rejection

@@ -113,6 +113,14 @@ SIGNATURES = [
     ("lpma_pool_trim", _int, []),
     ("lpma_pool_reserved_bytes", _u64, []),
     ("lpma_strerror", ctypes.c_char_p, [_int]),
+    ("lpma_group_create", _int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32]),
+    ("lpma_group_destroy", _int, [_vp]),
+    ("lpma_group_update", _int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_u64), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_u64), _vp]),
+    ("lpma_group_partitions", _u32, [_vp]),
+    ("lpma_group_get_stats", _int, [_vp, ctypes.POINTER(Stats)]),
+    ("lpma_group_set_profiling", _int, [_vp, _int]),
+    ("lpma_group_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_gen_uniform", _int, [_u32, _u64, _u64, _vp]),
     ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
     ("lpma_shuffle", _int, [_vp, _u64, _u64]),
@@ -395,6 +403,69 @@ class PMA:
     def profile(self):
         p = Profile()
         _check(lib().lpma_get_profile(self._h, ctypes.byref(p)), "profile")
+        return p
+
+
+class Group:
+    """Independent PMAs updated by one pipeline run per round (the batched
+    form; the reference has no counterpart).  Each member ends exactly as if
+    it had been updated alone with ``update_mixed``."""
+
+    def __init__(self, members, stream=None):
+        self.members = list(members)
+        n = len(self.members)
+        hs = (ctypes.c_void_p * n)(*[m._h for m in self.members])
+        h = ctypes.c_void_p()
+        _check(lib().lpma_group_create(ctypes.byref(h), hs, n), "group_create")
+        self._h = h
+        self.stream = stream
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lpma_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def partitions(self):
+        return int(lib().lpma_group_partitions(self._h))
+
+    def update(self, ins, dels):
+        """One round: ins[i] / dels[i] (int64 device tensors, numpy arrays or
+        None) for member i."""
+        n = len(self.members)
+        keep = []
+        pi = (ctypes.c_void_p * n)()
+        ni = (_u64 * n)()
+        pd = (ctypes.c_void_p * n)()
+        nd = (_u64 * n)()
+        for i, m in enumerate(self.members):
+            for arr, p_, n_ in ((ins[i] if ins is not None else None, pi, ni),
+                                (dels[i] if dels is not None else None, pd, nd)):
+                if arr is None or len(arr) == 0:
+                    continue
+                t = m._dev_keys(arr)
+                keep.append(t)
+                p_[i] = t.data_ptr()
+                n_[i] = t.numel()
+        _check(lib().lpma_group_update(self._h, pi, ni, pd, nd, _stream_ptr(self.stream)), "group_update")
+
+    def stats(self):
+        s = Stats()
+        _check(lib().lpma_group_get_stats(self._h, ctypes.byref(s)), "group_stats")
+        return s
+
+    def set_profiling(self, on=True):
+        _check(lib().lpma_group_set_profiling(self._h, 1 if on else 0), "group_set_profiling")
+
+    def profile(self):
+        p = Profile()
+        _check(lib().lpma_group_get_profile(self._h, ctypes.byref(p)), "group_profile")
         return p
 
 

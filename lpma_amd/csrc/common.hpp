@@ -13,6 +13,10 @@
 //                   segment whose separator is <= key.
 //   owner[2nseg]    u64 window claims of the current batch, tagged with the
 //                   batch epoch (epoch << 32 | ~run), so no reset is needed.
+//
+// A pipeline run updates a group of such arrays at once (one or many
+// independent PMAs): the kernels read each PMA's geometry from a device
+// table indexed by the PMA id carried in every batch element.
 //   row_off[V+1]    u32, row_off[r+1] = slot of row r's marker key (r<<32|~0u)
 //                   (GPMA convention, gpma.cuh:340-344), row_off[0] = 0.
 #pragma once
@@ -27,8 +31,9 @@ typedef int32_t i32;
 typedef int64_t i64;
 
 static constexpr u64 KEY_NONE = ~0ull;   // empty slot (rpma.cuh:38)
-static constexpr u32 SEG_NONE = 0xFFFFFFFFu;
-static constexpr u32 OP_BIT = 0x80000000u;
+// run key of a batch element: pma << 32 | segment, bit 63 the operation
+static constexpr u64 RK_NONE = ~0ull;
+static constexpr u64 RK_OP = 1ull << 63;
 
 // status flags raised on the device during one update
 static constexpr u32 FLAG_INVALID_KEY = 1u;   // src or dst >= V
@@ -70,7 +75,6 @@ enum StatSlot {
 static constexpr u32 ST_ROWS = 64;
 
 struct Status {
-    u32 pyr_ticket;   // last-workgroup-done counter of the pyramid kernel (0 between launches)
     u32 flags;
     u32 n_eff;        // effective updates after dedup / presence check
     i32 delta;        // inserts - deletes among them
@@ -78,6 +82,7 @@ struct Status {
 };
 
 struct Work {
+    u32 pid;  // PMA of the window
     u32 a;    // first segment of the window
     u32 L;    // level: the window holds 1 << L segments
     u32 u0;   // compacted update range [u0, u1)
@@ -97,13 +102,51 @@ struct DevState {
     u32 H;
     u32 V;
     u32 vbits;
-    u32 epoch;  // batch counter tagging window claims
 };
 
 struct Density {
     u64 lower[32];
     u64 upper[32];
     u64 root_shrink_below;  // root also fits below lower[H] when m > this
+};
+
+// One PMA of a pipeline run, as the kernels see it.
+struct PmaEntry {
+    DevState d;
+    Density dn;
+    u32 pyr_blk0;  // first workgroup of this PMA in the pyramid launch
+    u32 pad_;
+};
+
+// Per-PMA outcome of a run, read back by the host with the Status.
+struct PmaStat {
+    u32 flags;     // FLAG_ROOT / FLAG_RESIZE raised by its runs
+    u32 u0, u1;    // its effective updates: compacted range [u0, u1)
+    i32 pd0, pd1;  // the update prefix at u0 and u1 (delta = pd1 - pd0)
+};
+
+// One batch source: n keys of one operation for one PMA, placed at
+// [off, off + n) of the packed batch.
+struct BatchSeg {
+    const u64 *keys;
+    u64 n;
+    u64 off;
+    u32 pid;
+    u32 op;
+    u32 V;
+    u32 vbits;
+};
+
+// The PMAs one pipeline run updates.  Sort key of a batch element:
+// pid << pshift | (src << vbits | dst) << 1 | op, pshift = 2 * max vbits + 1.
+struct Group {
+    const PmaEntry *tab;
+    PmaStat *pstat;
+    u64 *stats;   // ST_ROWS x ST_NUM counters of this pipeline
+    u32 npma;
+    u32 pshift;
+    u32 epoch;    // tags this run's window claims
+    u32 log2S;    // segment size, the same for every PMA of the run
 };
 
 __host__ __device__ __forceinline__ u64 nseg_of(const DevState &d) { return (u64)1 << d.H; }

@@ -8,17 +8,16 @@ namespace pma {
 u64 scan_scratch_words(u64 n);
 hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
 u64 sort_scratch_words(u64 n);
-hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, u32 *flags, hipStream_t st);
 u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st);
-u64 *pack_and_sort(const u64 *a, u64 na, u32 op_a, const u64 *b, u64 nb, u32 op_b, u32 V, u32 vbits, u32 lo_bit,
-                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, u32 pshift, u32 lo_bit, u32 hi_bit, u64 *k0, u64 *k1,
+                   u32 *scratch, u32 *flags, hipStream_t st);
 
 // update.hip
 struct BatchBufs {
     u64 *ukey;     // n: decoded keys of the sorted batch
-    u32 *useg;     // n: segment | OP_BIT for effective updates, else SEG_NONE
-    u64 *ckey;     // n: compacted effective updates (sorted)
-    u32 *cseg;     // n: their segments
+    u64 *useg;     // n: run key pid << 32 | segment (| RK_OP for inserts) of effective updates, else RK_NONE
+    u64 *ckey;     // n: compacted effective updates (sorted by pid, key)
+    u64 *cseg;     // n: their run keys
     i32 *pd;       // n+1: exclusive prefix of +1 (insert) / -1 (delete)
     u32 *rlevel;   // n: chosen level per run start
     i32 *part;     // 2 * nblocks + 2: block partials (count, delta)
@@ -27,20 +26,28 @@ struct BatchBufs {
     u64 *lmbase;   // n: merged-scratch base per large window
     u32 *lhdr;     // 4: large windows, chunks
     Status *status;
-    u64 *stats;
+};
+
+// batch descriptors small enough to travel as kernel arguments
+static constexpr u32 INLINE_SEGS = 4;
+struct InlineSegs {
+    BatchSeg s[INLINE_SEGS];
+    u32 n;
 };
 
 u32 locate_blocks(u64 n);
-hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st);
-hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
-hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st);
-hipError_t elect_windows(const DevState &d, u64 n, const BatchBufs &b, hipStream_t st);
-hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
-hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 windows, hipStream_t st);
+hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, hipStream_t st);
+hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
+hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
+hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
+hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
+hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st);
+hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st);
+u32 pyramid_blocks(u32 H);
 
 // global (multi-workgroup) window path, in place, and the full rebuild
-hipError_t rebalance_large(const DevState &d, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st);
-hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
+hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st);
+hipError_t rebuild_push(const DevState &src, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
                         u32 *scan_scratch, hipStream_t st);
 hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <new>
@@ -124,6 +125,52 @@ struct DBuf {
 
 }  // namespace
 
+// ---------------------------------------------------------------- pipeline
+//
+// Batch scratch, run status and timing of one update pipeline.  A handle owns
+// one for its own updates (a group of one PMA); a group owns one per
+// partition of its members.
+
+enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR, PH_SEG, PH_NUM };
+
+struct Pipeline {
+    int device = 0;
+    hipStream_t cur_stream = nullptr;  // stream of the call in progress
+
+    DBuf<u64> pk0, pk1, ukey, useg, ckey, cseg, merged, lmbase;
+    DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr;
+    DBuf<i32> pd, part;
+    DBuf<Work> work[NUM_CLASSES];
+    DBuf<PmaEntry> tab;
+    DBuf<BatchSeg> segs;
+    DBuf<u64> out;  // Status, then one PmaStat per PMA
+
+    // pinned host mirrors
+    PmaEntry *h_tab = nullptr;
+    u32 h_tab_n = 0;
+    std::vector<PmaEntry> tab_shadow;  // what the device table holds
+    const PmaEntry *tab_shadow_at = nullptr;
+    BatchSeg *h_segs = nullptr;
+    u32 h_segs_n = 0;
+    unsigned char *h_out = nullptr;
+    u64 h_out_bytes = 0;
+
+    u64 *d_stats = nullptr;
+    u64 rounds = 0, effective = 0, rebuilds = 0, rebuild_slots = 0;
+
+    bool profiling = false;
+    struct Ev {
+        hipEvent_t a, b;
+        int phase;
+    };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> ev_pool;
+    double phase_ms[PH_NUM] = {0};
+    u64 launches_wave = 0, launches_block = 0, launches_seg = 0;
+};
+
+static constexpr u64 STATUS_BYTES = (sizeof(Status) + 15) / 16 * 16;
+
 // ------------------------------------------------------------------ handle
 
 struct lpma {
@@ -133,6 +180,7 @@ struct lpma {
     u32 V = 0, vbits = 1, log2S = 5;
     u64 live = 0;
     u64 version = 0;
+    u32 last_epoch = 0;  // epoch of the last run that updated this PMA
     DevState st;
     Density dn;
 
@@ -141,38 +189,29 @@ struct lpma {
     DBuf<u64> owner;
     DBuf<u64> sep;
 
-    // batch scratch
-    DBuf<u64> pk0, pk1, ukey, ckey, merged;
-    DBuf<u32> sort_scr, useg, cseg, rlevel, gpre, scan_scr, lchunk, lhdr;
-    DBuf<u64> lmbase;
+    Pipeline pl;  // plain updates, views and analytics scratch
     // analytics scratch
     DBuf<u32> aq0, aq1, alen, acnt;
     DBuf<float> af0, af1, af2, ared;
-    DBuf<i32> pd, part;
-    DBuf<Work> work[NUM_CLASSES];
-    Status *d_status = nullptr;
-    Status *h_status = nullptr;
-    u64 *d_stats = nullptr;
     u32 *d_err = nullptr;
 
     // host counters
     u64 calls = 0, effective = 0, rebuilds = 0, rebuild_slots = 0;
+};
 
-    // profiling
-    bool profiling = false;
-    struct Ev {
-        hipEvent_t a, b;
-        int phase;
-    };
-    std::vector<Ev> events;
-    std::vector<hipEvent_t> ev_pool;
-    double phase_ms[9] = {0};
-    u64 launches_wave = 0, launches_block = 0, launches_seg = 0;
+// A group of PMAs updated together: one pipeline run per partition (members
+// with the same segment size whose ids fit above their sort keys).
+struct lpma_group {
+    int device = 0;
+    std::vector<lpma *> members;
+    std::vector<std::vector<u32>> parts;
+    std::vector<Pipeline *> pls;
+    u64 calls = 0;
 };
 
 namespace {
 
-enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR, PH_SEG };
+std::atomic<u32> g_epoch{0};
 
 struct DevGuard {
     int prev = -1;
@@ -188,23 +227,23 @@ struct DevGuard {
 };
 
 template <typename T>
-int ensure(lpma *h, DBuf<T> &b, u64 n) {
+int ensure_on(int device, hipStream_t sync_st, DBuf<T> &b, u64 n) {
     if (b.n >= n && b.p) return LPMA_OK;
     u64 want = std::max<u64>(n, b.n + b.n / 2);
     if (want < 64) want = 64;
     void *p = nullptr;
-    int r = pool_alloc(h->device, &p, want * sizeof(T));
+    int r = pool_alloc(device, &p, want * sizeof(T));
     if (r != LPMA_OK) {
         // retry exactly
-        r = pool_alloc(h->device, &p, n * sizeof(T));
+        r = pool_alloc(device, &p, n * sizeof(T));
         if (r != LPMA_OK) return r;
         want = n;
     }
     if (b.p) {
         // queued kernels may still read the old block; it must not reach
         // another stream through the shared pool before they finish
-        (void)hipStreamSynchronize(h->cur_stream);
-        pool_free(h->device, b.p);
+        (void)hipStreamSynchronize(sync_st);
+        pool_free(device, b.p);
     }
     b.p = (T *)p;
     b.n = want;
@@ -212,16 +251,90 @@ int ensure(lpma *h, DBuf<T> &b, u64 n) {
 }
 
 template <typename T>
-void release(lpma *h, DBuf<T> &b) {
-    pool_free(h->device, b.p);
+int ensure(lpma *h, DBuf<T> &b, u64 n) { return ensure_on(h->device, h->cur_stream, b, n); }
+template <typename T>
+int ensure(Pipeline &pl, DBuf<T> &b, u64 n) { return ensure_on(pl.device, pl.cur_stream, b, n); }
+
+template <typename T>
+void release_on(int device, DBuf<T> &b) {
+    pool_free(device, b.p);
     b.p = nullptr;
     b.n = 0;
 }
+template <typename T>
+void release(lpma *h, DBuf<T> &b) { release_on(h->device, b); }
 
-hipEvent_t take_event(lpma *h) {
-    if (!h->ev_pool.empty()) {
-        hipEvent_t e = h->ev_pool.back();
-        h->ev_pool.pop_back();
+// pinned host buffer of at least n elements (contents not kept)
+template <typename T>
+int ensure_host(T *&p, u32 &cap, u64 n, hipStream_t sync_st) {
+    if (p && cap >= n) return LPMA_OK;
+    u64 want = std::max<u64>(n, 2ull * cap);
+    if (want < 8) want = 8;
+    if (p) {
+        (void)hipStreamSynchronize(sync_st);
+        (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    if (hipHostMalloc((void **)&p, want * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return LPMA_ERR_OUT_OF_MEMORY;
+    }
+    cap = (u32)want;
+    return LPMA_OK;
+}
+
+int pipeline_init(Pipeline &pl, int device, hipStream_t st) {
+    pl.device = device;
+    pl.cur_stream = st;
+    if (hipMalloc(&pl.d_stats, ST_ROWS * ST_NUM * sizeof(u64)) != hipSuccess) {
+        (void)hipGetLastError();
+        pl.d_stats = nullptr;
+        return LPMA_ERR_OUT_OF_MEMORY;
+    }
+    if (fill_u64(pl.d_stats, (u64)ST_ROWS * ST_NUM, 0, st) != hipSuccess) {
+        (void)hipGetLastError();
+        return LPMA_ERR_DEVICE;
+    }
+    return LPMA_OK;
+}
+
+void collect_events(Pipeline &pl) {
+    for (auto &e : pl.events) {
+        (void)hipEventSynchronize(e.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e.a, e.b);
+        pl.phase_ms[e.phase] += ms;
+        pl.ev_pool.push_back(e.a);
+        pl.ev_pool.push_back(e.b);
+    }
+    pl.events.clear();
+}
+
+void pipeline_free(Pipeline &pl) {
+    collect_events(pl);
+    for (auto e : pl.ev_pool) (void)hipEventDestroy(e);
+    pl.ev_pool.clear();
+    int d = pl.device;
+    release_on(d, pl.pk0); release_on(d, pl.pk1); release_on(d, pl.ukey); release_on(d, pl.useg);
+    release_on(d, pl.ckey); release_on(d, pl.cseg); release_on(d, pl.merged); release_on(d, pl.lmbase);
+    release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
+    release_on(d, pl.lchunk); release_on(d, pl.lhdr); release_on(d, pl.pd); release_on(d, pl.part);
+    for (int c = 0; c < NUM_CLASSES; ++c) release_on(d, pl.work[c]);
+    release_on(d, pl.tab); release_on(d, pl.segs); release_on(d, pl.out);
+    if (pl.h_tab) (void)hipHostFree(pl.h_tab);
+    if (pl.h_segs) (void)hipHostFree(pl.h_segs);
+    if (pl.h_out) (void)hipHostFree(pl.h_out);
+    pl.h_tab = nullptr; pl.h_segs = nullptr; pl.h_out = nullptr;
+    if (pl.d_stats) (void)hipFree(pl.d_stats);
+    pl.d_stats = nullptr;
+}
+
+hipEvent_t take_event(Pipeline &pl) {
+    if (!pl.ev_pool.empty()) {
+        hipEvent_t e = pl.ev_pool.back();
+        pl.ev_pool.pop_back();
         return e;
     }
     hipEvent_t e;
@@ -230,36 +343,24 @@ hipEvent_t take_event(lpma *h) {
 }
 
 struct PhaseTimer {
-    lpma *h;
+    Pipeline &pl;
     hipStream_t st;
     int phase;
     hipEvent_t a = nullptr;
-    PhaseTimer(lpma *h_, hipStream_t s, int p) : h(h_), st(s), phase(p) {
-        if (h->profiling) {
-            a = take_event(h);
+    PhaseTimer(Pipeline &pl_, hipStream_t s, int p) : pl(pl_), st(s), phase(p) {
+        if (pl.profiling) {
+            a = take_event(pl);
             (void)hipEventRecord(a, st);
         }
     }
     ~PhaseTimer() {
         if (a) {
-            hipEvent_t b = take_event(h);
+            hipEvent_t b = take_event(pl);
             (void)hipEventRecord(b, st);
-            h->events.push_back({a, b, phase});
+            pl.events.push_back({a, b, phase});
         }
     }
 };
-
-void collect_events(lpma *h) {
-    for (auto &e : h->events) {
-        (void)hipEventSynchronize(e.b);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e.a, e.b);
-        h->phase_ms[e.phase] += ms;
-        h->ev_pool.push_back(e.a);
-        h->ev_pool.push_back(e.b);
-    }
-    h->events.clear();
-}
 
 // Density table, rpma.cuh:136-168 (see lpma_config_t)
 void compute_density(lpma *h) {
@@ -331,35 +432,57 @@ DevState state_with(const lpma *h, u32 H, const Geometry &g) {
     return d;
 }
 
-BatchBufs batch_bufs(lpma *h) {
+Status *status_of(Pipeline &pl) { return (Status *)pl.out.p; }
+PmaStat *pstat_of(Pipeline &pl) { return (PmaStat *)((unsigned char *)pl.out.p + STATUS_BYTES); }
+
+BatchBufs batch_bufs(Pipeline &pl) {
     BatchBufs b;
-    b.ukey = h->ukey.p;
-    b.useg = h->useg.p;
-    b.ckey = h->ckey.p;
-    b.cseg = h->cseg.p;
-    b.pd = h->pd.p;
-    b.rlevel = h->rlevel.p;
-    b.part = h->part.p;
-    for (int c = 0; c < NUM_CLASSES; ++c) b.work[c] = h->work[c].p;
-    b.lchunk = h->lchunk.p;
-    b.lmbase = h->lmbase.p;
-    b.lhdr = h->lhdr.p;
-    b.status = h->d_status;
-    b.stats = h->d_stats;
+    b.ukey = pl.ukey.p;
+    b.useg = pl.useg.p;
+    b.ckey = pl.ckey.p;
+    b.cseg = pl.cseg.p;
+    b.pd = pl.pd.p;
+    b.rlevel = pl.rlevel.p;
+    b.part = pl.part.p;
+    for (int c = 0; c < NUM_CLASSES; ++c) b.work[c] = pl.work[c].p;
+    b.lchunk = pl.lchunk.p;
+    b.lmbase = pl.lmbase.p;
+    b.lhdr = pl.lhdr.p;
+    b.status = status_of(pl);
     return b;
 }
 
-int ensure_batch(lpma *h, u64 n) {
+int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
     int r;
     u64 nb = locate_blocks(n);
-    if ((r = ensure(h, h->pk0, n)) || (r = ensure(h, h->pk1, n)) || (r = ensure(h, h->sort_scr, sort_scratch_words(n))) ||
-        (r = ensure(h, h->ukey, n)) || (r = ensure(h, h->useg, n)) || (r = ensure(h, h->ckey, n)) ||
-        (r = ensure(h, h->cseg, n)) || (r = ensure(h, h->pd, n + 1)) || (r = ensure(h, h->rlevel, n)) ||
-        (r = ensure(h, h->part, 2 * nb + 2)))
+    u64 out_words = (STATUS_BYTES + (u64)npma * sizeof(PmaStat) + 7) / 8;
+    if ((r = ensure(pl, pl.pk0, n)) || (r = ensure(pl, pl.pk1, n)) ||
+        (r = ensure(pl, pl.sort_scr, sort_scratch_words(n))) || (r = ensure(pl, pl.ukey, n)) ||
+        (r = ensure(pl, pl.useg, n)) || (r = ensure(pl, pl.ckey, n)) || (r = ensure(pl, pl.cseg, n)) ||
+        (r = ensure(pl, pl.pd, n + 1)) || (r = ensure(pl, pl.rlevel, n)) || (r = ensure(pl, pl.part, 2 * nb + 2)) ||
+        (r = ensure(pl, pl.tab, npma)) || (r = ensure(pl, pl.segs, std::max<u32>(nsegs, INLINE_SEGS))) ||
+        (r = ensure(pl, pl.out, out_words)))
         return r;
-    u64 runs = std::min<u64>(n, 1ull << h->st.H);
     for (int c = 0; c < NUM_CLASSES; ++c)
-        if ((r = ensure(h, h->work[c], runs))) return r;
+        if ((r = ensure(pl, pl.work[c], runs))) return r;
+    if ((r = ensure_host(pl.h_tab, pl.h_tab_n, npma, pl.cur_stream)) ||
+        (r = ensure_host(pl.h_segs, pl.h_segs_n, nsegs, pl.cur_stream)))
+        return r;
+    if (!pl.h_out || pl.h_out_bytes < out_words * 8) {
+        if (pl.h_out) {
+            (void)hipStreamSynchronize(pl.cur_stream);
+            (void)hipHostFree(pl.h_out);
+            pl.h_out = nullptr;
+        }
+        u64 want = std::max<u64>(out_words * 8, 2 * pl.h_out_bytes);
+        if (hipHostMalloc((void **)&pl.h_out, want, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            pl.h_out = nullptr;
+            pl.h_out_bytes = 0;
+            return LPMA_ERR_OUT_OF_MEMORY;
+        }
+        pl.h_out_bytes = want;
+    }
     return LPMA_OK;
 }
 
@@ -380,11 +503,11 @@ int ensure_batch(lpma *h, u64 n) {
 
 // Spread m sorted keys from `merged` over a fresh geometry of height H and
 // install it (create and resize).
-int respread_into_new(lpma *h, u32 H, u64 m, hipStream_t st) {
+int respread_into_new(lpma *h, u32 H, const u64 *merged, u64 m, hipStream_t st) {
     Geometry g;
     CHECKED(alloc_geometry(h, H, g, st));
     DevState d2 = state_with(h, H, g);
-    HIPCHECK(rebuild_spread(d2, h->merged.p, m, st));
+    HIPCHECK(rebuild_spread(d2, merged, m, st));
     HIPCHECK(rebuild_pyramid(d2, true, st));
     // old arrays may still be read by queued kernels: wait before they
     // return to the pool
@@ -393,109 +516,284 @@ int respread_into_new(lpma *h, u32 H, u64 m, hipStream_t st) {
     return LPMA_OK;
 }
 
-int run_update(lpma *h, const u64 *a, u64 na, u32 opa, const u64 *b, u64 nb, u32 opb, hipStream_t st) {
-    const u64 n = na + nb;
-    h->cur_stream = st;
-    h->calls++;
+// One PMA's share of a pipeline run.
+struct Member {
+    lpma *h;
+    const u64 *ins;
+    u64 nins;
+    const u64 *del;
+    u64 ndel;
+};
+
+u32 bits_for(u64 n) {  // bits to hold the values 0 .. n-1
+    u32 b = 0;
+    while ((1ull << b) < n) ++b;
+    return b;
+}
+
+u32 next_epoch() {
+    u32 e = ++g_epoch;
+    if (e == 0) e = ++g_epoch;
+    return e;
+}
+
+// One update round over the members (all on pl's device, same segment size,
+// ids fitting above the sort keys): sort + dedup + locate + density check +
+// election over every member at once, one host synchronisation, then the
+// rebalance classes, the pyramid, and root rebuilds / resizes per member.
+int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
+    pl.cur_stream = st;
+    u64 n = 0, nseg_total = 0;
+    bool mixed = false;
+    u32 vmax = 1, nsegs = 0;
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        h->cur_stream = st;
+        h->calls++;
+        n += mem[p].nins + mem[p].ndel;
+        mixed |= mem[p].nins && mem[p].ndel;
+        vmax = std::max(vmax, h->vbits);
+        nseg_total += 1ull << h->st.H;
+        nsegs += (mem[p].nins ? 1 : 0) + (mem[p].ndel ? 1 : 0);
+    }
+    pl.rounds++;
     if (n == 0) {
-        h->version++;
+        for (u32 p = 0; p < npma; ++p) mem[p].h->version++;
         return LPMA_OK;
     }
     if (n > 0x7FFFFFF0ull) return LPMA_ERR_TOO_LARGE;
-    CHECKED(ensure_batch(h, n));
-    if (++h->st.epoch == 0) {
-        // claims are tagged with the epoch: after a wrap, clear the old tags
-        h->st.epoch = 1;
-        HIPCHECK(fill_u64(h->owner.p, 2 * (1ull << h->st.H), 0, st));
+    const u32 pshift = 2 * vmax + 1;
+    const u32 hi_bit = pshift + bits_for(npma);
+    if (hi_bit > 64) return LPMA_ERR_INVALID_ARGUMENT;
+    CHECKED(ensure_batch(pl, n, npma, nsegs, std::min<u64>(n, nseg_total)));
+
+    // claims are tagged with the epoch: after a wrap, clear a PMA's old tags
+    const u32 epoch = next_epoch();
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        if (epoch <= h->last_epoch) HIPCHECK(fill_u64(h->owner.p, 2 * (1ull << h->st.H), 0, st));
+        h->last_epoch = epoch;
     }
-    BatchBufs bb = batch_bufs(h);
-    // bit 0 of a packed key is the operation: sorted only when the batch mixes them
-    const u32 lo_bit = (na && nb && opa != opb) ? 0 : 1;
-    const u32 hi_bit = 2 * h->vbits + 1;
+
+    // device table of the members (uploaded only when it changed)
+    u32 pyr_blocks = 0;
+    for (u32 p = 0; p < npma; ++p) {
+        PmaEntry &e = pl.h_tab[p];
+        memset(&e, 0, sizeof(e));
+        e.d = mem[p].h->st;
+        e.dn = mem[p].h->dn;
+        e.pyr_blk0 = pyr_blocks;
+        pyr_blocks += pyramid_blocks(e.d.H);
+    }
+    if (pl.tab_shadow_at != pl.tab.p || pl.tab_shadow.size() != npma ||
+        memcmp(pl.tab_shadow.data(), pl.h_tab, npma * sizeof(PmaEntry)) != 0) {
+        HIPCHECK(hipMemcpyAsync(pl.tab.p, pl.h_tab, npma * sizeof(PmaEntry), hipMemcpyHostToDevice, st));
+        pl.tab_shadow.assign(pl.h_tab, pl.h_tab + npma);
+        pl.tab_shadow_at = pl.tab.p;
+    }
+
+    // batch sources: each member's deletes (op 0), then its inserts (op 1)
+    InlineSegs in;
+    memset(&in, 0, sizeof(in));
+    {
+        u64 off = 0;
+        u32 k = 0;
+        for (u32 p = 0; p < npma; ++p) {
+            lpma *h = mem[p].h;
+            for (int op = 0; op < 2; ++op) {
+                const u64 *keys = op ? mem[p].ins : mem[p].del;
+                u64 cnt = op ? mem[p].nins : mem[p].ndel;
+                if (!cnt) continue;
+                BatchSeg &b = pl.h_segs[k++];
+                b.keys = keys;
+                b.n = cnt;
+                b.off = off;
+                b.pid = p;
+                b.op = (u32)op;
+                b.V = h->V;
+                b.vbits = h->vbits;
+                off += cnt;
+            }
+        }
+        if (nsegs <= INLINE_SEGS) {
+            for (u32 k2 = 0; k2 < nsegs; ++k2) in.s[k2] = pl.h_segs[k2];
+            in.n = nsegs;
+        } else {
+            HIPCHECK(hipMemcpyAsync(pl.segs.p, pl.h_segs, nsegs * sizeof(BatchSeg), hipMemcpyHostToDevice, st));
+        }
+    }
+
+    Group g;
+    g.tab = pl.tab.p;
+    g.pstat = pstat_of(pl);
+    g.stats = pl.d_stats;
+    g.npma = npma;
+    g.pshift = pshift;
+    g.epoch = epoch;
+    g.log2S = mem[0].h->log2S;
+    BatchBufs bb = batch_bufs(pl);
+    // bit 0 of a packed key is the operation: sorted only when a member mixes them
+    const u32 lo_bit = mixed ? 0 : 1;
     u64 *pk;
     {
-        PhaseTimer t(h, st, PH_SORT);
-        HIPCHECK(batch_begin(h->d_status, h->st, st));
-        pk = pack_and_sort(a, na, opa, b, nb, opb, h->V, h->vbits, lo_bit, hi_bit, h->pk0.p, h->pk1.p,
-                           h->sort_scr.p, &h->d_status->flags, st);
+        PhaseTimer t(pl, st, PH_SORT);
+        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, st));
+        pk = pack_and_sort(pl.segs.p, nsegs, n, pshift, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p, pl.sort_scr.p,
+                           &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
     }
     {
-        PhaseTimer t(h, st, PH_LOCATE);
-        HIPCHECK(locate_and_compact(h->st, pk, n, bb, st));
-        HIPCHECK(density_check(h->st, h->dn, n, bb, st));
+        PhaseTimer t(pl, st, PH_LOCATE);
+        HIPCHECK(locate_and_compact(g, pk, n, bb, st));
+        HIPCHECK(density_check(g, n, bb, st));
     }
     {
         // window election before the single host synchronisation, so the
         // host learns the exact work of every rebalance class and sizes each
         // grid to it (idle workgroups still cost dispatch time)
-        PhaseTimer t(h, st, PH_ELECT);
-        HIPCHECK(elect_windows(h->st, n, bb, st));
+        PhaseTimer t(pl, st, PH_ELECT);
+        HIPCHECK(elect_windows(g, n, bb, st));
     }
-    HIPCHECK(hipMemcpyAsync(h->h_status, h->d_status, sizeof(Status), hipMemcpyDeviceToHost, st));
+    const u64 out_bytes = STATUS_BYTES + (u64)npma * sizeof(PmaStat);
+    HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    const Status s = *h->h_status;
+    const Status s = *(const Status *)pl.h_out;
+    const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
     if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
     const u64 n_eff = s.n_eff;
     if (n_eff == 0) {
-        h->version++;
+        for (u32 p = 0; p < npma; ++p) mem[p].h->version++;
         return LPMA_OK;
     }
-    const u64 M = (u64)((i64)h->live + s.delta);
 
-    if (s.flags & (FLAG_ROOT | FLAG_RESIZE)) {
-        // root rebalance or resize: every live key and every update is
-        // merged into scratch and spread again (rebuild_rpma, rpma.cuh:1417-1472)
-        u32 newH = (s.flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
-        if (!height_ok(h, newH)) return LPMA_ERR_TOO_LARGE;
-        u64 nseg = 1ull << h->st.H;
-        CHECKED(ensure(h, h->merged, M + 1));
-        CHECKED(ensure(h, h->gpre, nseg + 1));
-        CHECKED(ensure(h, h->scan_scr, scan_scratch_words(nseg) + 8));
-        PhaseTimer t(h, st, PH_REBUILD);
-        HIPCHECK(rebuild_push(h->st, n_eff, bb, h->merged.p, M, h->gpre.p, h->scan_scr.p, st));
-        if (newH != h->st.H) {
-            CHECKED(respread_into_new(h, newH, M, st));
-        } else {
-            HIPCHECK(rebuild_spread(h->st, h->merged.p, M, st));
-            HIPCHECK(rebuild_pyramid(h->st, true, st));
-        }
-        h->rebuilds++;
-        h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
-    } else {
-        if (s.nwork[CLASS_SEG]) {
-            PhaseTimer t(h, st, PH_SEG);
-            HIPCHECK(rebalance_seg(h->st, bb, s.nwork[CLASS_SEG], st));
-            h->launches_seg++;
-        }
-        if (s.nwork[CLASS_WAVE]) {
-            PhaseTimer t(h, st, PH_WAVE);
-            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
-            h->launches_wave++;
-        }
-        if (s.nwork[CLASS_BLOCK]) {
-            PhaseTimer t(h, st, PH_BLOCK);
-            HIPCHECK(rebalance_lds(h->st, n_eff, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
-            h->launches_block++;
-        }
-        if (s.nwork[CLASS_LARGE]) {
-            CHECKED(ensure(h, h->merged, M + 1));
-            CHECKED(ensure(h, h->lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
-            CHECKED(ensure(h, h->lmbase, s.nwork[CLASS_LARGE]));
-            CHECKED(ensure(h, h->lhdr, 4));
-            bb = batch_bufs(h);
-            PhaseTimer t(h, st, PH_LARGE);
-            HIPCHECK(rebalance_large(h->st, bb, h->merged.p, 1024, st));
-        }
-        {
-            PhaseTimer t(h, st, PH_PYR);
-            HIPCHECK(rebuild_pyramid(h->st, false, st));
+    // new live counts; every root rebuild / resize is checked before any
+    // kernel changes a PMA, so a failing round changes nothing
+    std::vector<u64> Mnew(npma);
+    u64 m_sum = 0, m_rebuild = 0;
+    bool any_rebuild = false;
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        Mnew[p] = (u64)((i64)h->live + (ps[p].pd1 - ps[p].pd0));
+        m_sum += Mnew[p];
+        if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) {
+            any_rebuild = true;
+            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, Mnew[p]) : h->st.H;
+            if (!height_ok(h, newH)) return LPMA_ERR_TOO_LARGE;
+            m_rebuild = std::max(m_rebuild, Mnew[p]);
         }
     }
-    h->live = M;
-    h->effective += n_eff;
-    h->version++;
+
+    if (s.nwork[CLASS_SEG]) {
+        PhaseTimer t(pl, st, PH_SEG);
+        HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st));
+        pl.launches_seg++;
+    }
+    if (s.nwork[CLASS_WAVE]) {
+        PhaseTimer t(pl, st, PH_WAVE);
+        HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
+        pl.launches_wave++;
+    }
+    if (s.nwork[CLASS_BLOCK]) {
+        PhaseTimer t(pl, st, PH_BLOCK);
+        HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
+        pl.launches_block++;
+    }
+    if (s.nwork[CLASS_LARGE]) {
+        // the windows' merged slices add up to at most the members' live keys
+        CHECKED(ensure(pl, pl.merged, m_sum + 1));
+        CHECKED(ensure(pl, pl.lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
+        CHECKED(ensure(pl, pl.lmbase, s.nwork[CLASS_LARGE]));
+        CHECKED(ensure(pl, pl.lhdr, 4));
+        bb = batch_bufs(pl);
+        PhaseTimer t(pl, st, PH_LARGE);
+        HIPCHECK(rebalance_large(g, bb, pl.merged.p, 1024, st));
+    }
+    {
+        PhaseTimer t(pl, st, PH_PYR);
+        HIPCHECK(group_pyramid(g, pyr_blocks, st));
+    }
+
+    if (any_rebuild) {
+        // root rebalance or resize: every live key and every update of the
+        // PMA is merged into scratch and spread again (rebuild_rpma,
+        // rpma.cuh:1417-1472)
+        PhaseTimer t(pl, st, PH_REBUILD);
+        CHECKED(ensure(pl, pl.merged, m_rebuild + 1));
+        for (u32 p = 0; p < npma; ++p) {
+            if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
+            lpma *h = mem[p].h;
+            const u64 M = Mnew[p];
+            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
+            u64 nseg = 1ull << h->st.H;
+            CHECKED(ensure(pl, pl.gpre, nseg + 1));
+            CHECKED(ensure(pl, pl.scan_scr, scan_scratch_words(nseg) + 8));
+            HIPCHECK(rebuild_push(h->st, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.scan_scr.p, st));
+            if (newH != h->st.H) {
+                CHECKED(respread_into_new(h, newH, pl.merged.p, M, st));
+            } else {
+                HIPCHECK(rebuild_spread(h->st, pl.merged.p, M, st));
+                HIPCHECK(rebuild_pyramid(h->st, true, st));
+            }
+            h->rebuilds++;
+            h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
+            pl.rebuilds++;
+            pl.rebuild_slots += (u64)1 << (h->st.H + h->log2S);
+        }
+    }
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        h->live = Mnew[p];
+        h->effective += ps[p].u1 - ps[p].u0;
+        h->version++;
+    }
+    pl.effective += n_eff;
     return LPMA_OK;
+}
+
+int run_update(lpma *h, const u64 *ins, u64 nins, const u64 *del, u64 ndel, hipStream_t st) {
+    Member m{h, ins, nins, del, ndel};
+    return run_pipeline(h->pl, &m, 1, st);
+}
+
+void fill_stats(const u64 *rows, u32 log2S, lpma_stats_t *out) {
+    u64 s[ST_NUM] = {0};
+    for (u32 r = 0; r < ST_ROWS; ++r)
+        for (u32 i = 0; i < ST_NUM; ++i) s[i] += rows[(size_t)r * ST_NUM + i];
+    for (int l = 0; l < 32; ++l) {
+        out->windows_per_level[l] += s[ST_WINDOWS + l];
+        out->slots_per_level[l] += s[ST_WINDOWS + l] << (l + log2S);
+    }
+    out->marker_moves += s[ST_MARKERS];
+    out->alg_bytes_wave += s[ST_ALG_BYTES_WAVE];
+    out->alg_bytes_block += s[ST_ALG_BYTES_BLOCK];
+    out->alg_bytes_large += s[ST_ALG_BYTES_LARGE];
+    out->alg_bytes_seg += s[ST_ALG_BYTES_SEG];
+}
+
+int read_stats(Pipeline &pl, u32 log2S, lpma_stats_t *out) {
+    std::vector<u64> rows((size_t)ST_ROWS * ST_NUM);
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(rows.data(), pl.d_stats, rows.size() * sizeof(u64), hipMemcpyDeviceToHost));
+    fill_stats(rows.data(), log2S, out);
+    return LPMA_OK;
+}
+
+void add_profile(Pipeline &pl, lpma_profile_t *out) {
+    collect_events(pl);
+    out->sort_ms += pl.phase_ms[PH_SORT];
+    out->locate_ms += pl.phase_ms[PH_LOCATE];
+    out->elect_ms += pl.phase_ms[PH_ELECT];
+    out->rebalance_wave_ms += pl.phase_ms[PH_WAVE];
+    out->rebalance_block_ms += pl.phase_ms[PH_BLOCK];
+    out->large_ms += pl.phase_ms[PH_LARGE];
+    out->rebuild_ms += pl.phase_ms[PH_REBUILD];
+    out->pyramid_ms += pl.phase_ms[PH_PYR];
+    out->rebalance_seg_ms += pl.phase_ms[PH_SEG];
+    out->launches_wave += pl.launches_wave;
+    out->launches_block += pl.launches_block;
+    out->launches_seg += pl.launches_seg;
+    for (double &v : pl.phase_ms) v = 0;
+    pl.launches_wave = pl.launches_block = pl.launches_seg = 0;
 }
 
 }  // namespace
@@ -568,18 +866,15 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
     h->st.vbits = h->vbits;
 
     int r = LPMA_OK;
-    if (hipMalloc(&h->d_status, sizeof(Status)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_status, sizeof(Status), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&h->d_stats, ST_ROWS * ST_NUM * sizeof(u64)) != hipSuccess || hipMalloc(&h->d_err, 16) != hipSuccess) {
+    if (hipMalloc(&h->d_err, 16) != hipSuccess) {
         (void)hipGetLastError();
+        h->d_err = nullptr;
         lpma_destroy(h);
         return LPMA_ERR_OUT_OF_MEMORY;
     }
-    if (hipMemsetAsync(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64), st) != hipSuccess ||
-        hipMemsetAsync(h->d_status, 0, sizeof(Status), st) != hipSuccess) {
-        (void)hipGetLastError();
+    if ((r = pipeline_init(h->pl, dev, st)) != LPMA_OK) {
         lpma_destroy(h);
-        return LPMA_ERR_DEVICE;
+        return r;
     }
     if ((r = ensure(h, h->row_off, (u64)num_vertices + 1)) != LPMA_OK) {
         lpma_destroy(h);
@@ -599,18 +894,18 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
         lpma_destroy(h);
         return LPMA_ERR_TOO_LARGE;
     }
-    if ((r = ensure(h, h->merged, (u64)num_vertices + 1)) != LPMA_OK) {
+    if ((r = ensure(h->pl, h->pl.merged, (u64)num_vertices + 1)) != LPMA_OK) {
         lpma_destroy(h);
         return r;
     }
-    if (make_markers(h->merged.p, num_vertices, st) != hipSuccess) {
+    if (make_markers(h->pl.merged.p, num_vertices, st) != hipSuccess) {
         (void)hipGetLastError();
         lpma_destroy(h);
         return LPMA_ERR_DEVICE;
     }
     h->st.H = H;
     compute_density(h);
-    if ((r = respread_into_new(h, H, num_vertices, st)) != LPMA_OK) {
+    if ((r = respread_into_new(h, H, h->pl.merged.p, num_vertices, st)) != LPMA_OK) {
         lpma_destroy(h);
         return r;
     }
@@ -628,20 +923,11 @@ int lpma_destroy(lpma_t *h) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     (void)hipDeviceSynchronize();
-    collect_events(h);
-    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+    pipeline_free(h->pl);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
-    release(h, h->pk0); release(h, h->pk1); release(h, h->ukey); release(h, h->ckey); release(h, h->merged);
-    release(h, h->sort_scr); release(h, h->useg); release(h, h->cseg); release(h, h->rlevel);
-    release(h, h->gpre); release(h, h->scan_scr); release(h, h->pd); release(h, h->part);
-    release(h, h->lchunk); release(h, h->lhdr); release(h, h->lmbase);
     release(h, h->aq0); release(h, h->aq1); release(h, h->alen); release(h, h->acnt);
     release(h, h->af0); release(h, h->af1); release(h, h->af2); release(h, h->ared);
-    for (int c = 0; c < NUM_CLASSES; ++c) release(h, h->work[c]);
-    if (h->d_status) (void)hipFree(h->d_status);
-    if (h->h_status) (void)hipHostFree(h->h_status);
-    if (h->d_stats) (void)hipFree(h->d_stats);
     if (h->d_err) (void)hipFree(h->d_err);
     delete h;
     return LPMA_OK;
@@ -650,14 +936,15 @@ int lpma_destroy(lpma_t *h) {
 int lpma_update(lpma_t *h, const uint64_t *d_keys, uint64_t n, int op, void *stream) {
     if (!h || (!d_keys && n) || (op != LPMA_OP_INSERT && op != LPMA_OP_DELETE)) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    return run_update(h, d_keys, n, (u32)op, nullptr, 0, 0, (hipStream_t)stream);
+    if (op == LPMA_OP_INSERT) return run_update(h, d_keys, n, nullptr, 0, (hipStream_t)stream);
+    return run_update(h, nullptr, 0, d_keys, n, (hipStream_t)stream);
 }
 
 int lpma_update_mixed(lpma_t *h, const uint64_t *d_ins, uint64_t n_ins, const uint64_t *d_del, uint64_t n_del,
                       void *stream) {
     if (!h || (!d_ins && n_ins) || (!d_del && n_del)) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    return run_update(h, d_del, n_del, 0, d_ins, n_ins, 1, (hipStream_t)stream);
+    return run_update(h, d_ins, n_ins, d_del, n_del, (hipStream_t)stream);
 }
 
 int lpma_bulk_load(lpma_t *h, const uint64_t *d_keys, uint64_t n, void *stream) {
@@ -697,10 +984,11 @@ static int export_common(lpma_t *h, u64 *edges, u32 *row_std, u64 *all, u32 *csr
     DevGuard guard(h->device);
     hipStream_t st = (hipStream_t)stream;
     h->cur_stream = st;
+    h->pl.cur_stream = st;
     u64 nseg = 1ull << h->st.H;
-    CHECKED(ensure(h, h->gpre, nseg + 1));
-    CHECKED(ensure(h, h->scan_scr, scan_scratch_words(nseg) + 8));
-    HIPCHECK(export_compacted(h->st, h->gpre.p, h->scan_scr.p, edges, row_std, all, csr_row, csr_col, st));
+    CHECKED(ensure(h->pl, h->pl.gpre, nseg + 1));
+    CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8));
+    HIPCHECK(export_compacted(h->st, h->pl.gpre.p, h->pl.scan_scr.p, edges, row_std, all, csr_row, csr_col, st));
     return LPMA_OK;
 }
 
@@ -763,7 +1051,8 @@ int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint6
     CHECKED(ensure(h, h->aq0, V + 1));
     CHECKED(ensure(h, h->aq1, V + 1));
     CHECKED(ensure(h, h->alen, V + 2));
-    CHECKED(ensure(h, h->scan_scr, scan_scratch_words(V + 1) + 8));
+    h->pl.cur_stream = st;
+    CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(V + 1) + 8));
     CHECKED(ensure(h, h->acnt, 4));
     HIPCHECK(bfs_init(d_levels, h->V, src, h->aq0.p, h->acnt.p, st));
     u32 *q = h->aq0.p, *qn_buf = h->aq1.p;
@@ -771,7 +1060,7 @@ int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint6
     u64 scanned = 0;
     u32 cnt[2];
     while (qn) {
-        HIPCHECK(bfs_level(h->st, q, qn, h->alen.p, h->scan_scr.p, d_levels, level, qn_buf, h->acnt.p, st));
+        HIPCHECK(bfs_level(h->st, q, qn, h->alen.p, h->pl.scan_scr.p, d_levels, level, qn_buf, h->acnt.p, st));
         HIPCHECK(hipMemcpyAsync(cnt, h->acnt.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(fill_u32(h->acnt.p, 2, 0, st));
@@ -793,14 +1082,15 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
     h->cur_stream = st;
     const u64 V = h->V;
     u64 nseg = 1ull << h->st.H;
-    CHECKED(ensure(h, h->gpre, nseg + 1));
-    CHECKED(ensure(h, h->scan_scr, scan_scratch_words(nseg) + 8));
+    h->pl.cur_stream = st;
+    CHECKED(ensure(h->pl, h->pl.gpre, nseg + 1));
+    CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8));
     CHECKED(ensure(h, h->aq0, V));   // out-degrees
     CHECKED(ensure(h, h->af0, V));   // contributions
     CHECKED(ensure(h, h->af1, V));   // accumulator
     CHECKED(ensure(h, h->af2, V));   // second rank vector
     CHECKED(ensure(h, h->ared, 8));
-    HIPCHECK(pr_outdeg(h->st, h->gpre.p, h->scan_scr.p, h->aq0.p, st));
+    HIPCHECK(pr_outdeg(h->st, h->pl.gpre.p, h->pl.scan_scr.p, h->aq0.p, st));
     float *red[2] = {h->ared.p, h->ared.p + 4};
     float *vec[2] = {d_rank, h->af2.p};
     HIPCHECK(pr_init(vec[0], h->V, red[0], st));
@@ -829,26 +1119,12 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
 int lpma_get_stats(lpma_t *h, lpma_stats_t *out) {
     if (!h || !out) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    std::vector<u64> rows((size_t)ST_ROWS * ST_NUM);
-    HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemcpy(rows.data(), h->d_stats, rows.size() * sizeof(u64), hipMemcpyDeviceToHost));
-    u64 s[ST_NUM] = {0};
-    for (u32 r = 0; r < ST_ROWS; ++r)
-        for (u32 i = 0; i < ST_NUM; ++i) s[i] += rows[(size_t)r * ST_NUM + i];
     memset(out, 0, sizeof(*out));
+    CHECKED(read_stats(h->pl, h->log2S, out));
     out->updates_calls = h->calls;
     out->effective_updates = h->effective;
-    for (int l = 0; l < 32; ++l) {
-        out->windows_per_level[l] = s[ST_WINDOWS + l];
-        out->slots_per_level[l] = s[ST_WINDOWS + l] << (l + h->log2S);
-    }
     out->rebuilds = h->rebuilds;
     out->rebuild_slots = h->rebuild_slots;
-    out->marker_moves = s[ST_MARKERS];
-    out->alg_bytes_wave = s[ST_ALG_BYTES_WAVE];
-    out->alg_bytes_block = s[ST_ALG_BYTES_BLOCK];
-    out->alg_bytes_large = s[ST_ALG_BYTES_LARGE];
-    out->alg_bytes_seg = s[ST_ALG_BYTES_SEG];
     return LPMA_OK;
 }
 
@@ -856,35 +1132,145 @@ int lpma_reset_stats(lpma_t *h) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipMemset(h->d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64)));
+    HIPCHECK(hipMemset(h->pl.d_stats, 0, ST_ROWS * ST_NUM * sizeof(u64)));
     h->calls = h->effective = h->rebuilds = h->rebuild_slots = 0;
     return LPMA_OK;
 }
 
 int lpma_set_profiling(lpma_t *h, int on) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
-    h->profiling = on != 0;
+    h->pl.profiling = on != 0;
     return LPMA_OK;
 }
 
 int lpma_get_profile(lpma_t *h, lpma_profile_t *out) {
     if (!h || !out) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    collect_events(h);
-    out->sort_ms = h->phase_ms[PH_SORT];
-    out->locate_ms = h->phase_ms[PH_LOCATE];
-    out->elect_ms = h->phase_ms[PH_ELECT];
-    out->rebalance_wave_ms = h->phase_ms[PH_WAVE];
-    out->rebalance_block_ms = h->phase_ms[PH_BLOCK];
-    out->large_ms = h->phase_ms[PH_LARGE];
-    out->rebuild_ms = h->phase_ms[PH_REBUILD];
-    out->pyramid_ms = h->phase_ms[PH_PYR];
-    out->rebalance_seg_ms = h->phase_ms[PH_SEG];
-    out->launches_wave = h->launches_wave;
-    out->launches_block = h->launches_block;
-    out->launches_seg = h->launches_seg;
-    for (double &v : h->phase_ms) v = 0;
-    h->launches_wave = h->launches_block = h->launches_seg = 0;
+    memset(out, 0, sizeof(*out));
+    add_profile(h->pl, out);
+    return LPMA_OK;
+}
+
+// ------------------------------------------------------------------ groups
+
+int lpma_group_create(lpma_group_t **out, lpma_t *const *members, uint32_t n) {
+    if (!out) return LPMA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!members || n == 0) return LPMA_ERR_INVALID_ARGUMENT;
+    for (u32 i = 0; i < n; ++i) {
+        if (!members[i] || members[i]->device != members[0]->device) return LPMA_ERR_INVALID_ARGUMENT;
+        for (u32 j = 0; j < i; ++j)
+            if (members[j] == members[i]) return LPMA_ERR_INVALID_ARGUMENT;
+    }
+    lpma_group *g = new (std::nothrow) lpma_group();
+    if (!g) return LPMA_ERR_OUT_OF_MEMORY;
+    g->device = members[0]->device;
+    g->members.assign(members, members + n);
+    // partitions: same segment size, and the ids must fit above the widest
+    // member's sort key (2 * vbits + 1 bits)
+    std::map<u32, std::vector<u32>> by_s;
+    for (u32 i = 0; i < n; ++i) by_s[members[i]->log2S].push_back(i);
+    for (auto &kv : by_s) {
+        std::vector<u32> cur;
+        u32 vmax = 1;
+        for (u32 i : kv.second) {
+            u32 v2 = std::max(vmax, members[i]->vbits);
+            if (!cur.empty() && 2 * v2 + 1 + bits_for(cur.size() + 1) > 64) {
+                g->parts.push_back(cur);
+                cur.clear();
+                v2 = members[i]->vbits;
+            }
+            cur.push_back(i);
+            vmax = v2;
+        }
+        if (!cur.empty()) g->parts.push_back(cur);
+    }
+    DevGuard guard(g->device);
+    for (size_t k = 0; k < g->parts.size(); ++k) {
+        Pipeline *pl = new (std::nothrow) Pipeline();
+        int r = pl ? pipeline_init(*pl, g->device, nullptr) : LPMA_ERR_OUT_OF_MEMORY;
+        if (pl) g->pls.push_back(pl);
+        if (r != LPMA_OK) {
+            lpma_group_destroy(g);
+            return r;
+        }
+    }
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipGetLastError();
+        lpma_group_destroy(g);
+        return LPMA_ERR_DEVICE;
+    }
+    *out = g;
+    return LPMA_OK;
+}
+
+int lpma_group_destroy(lpma_group_t *g) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(g->device);
+    (void)hipDeviceSynchronize();
+    for (Pipeline *pl : g->pls) {
+        pipeline_free(*pl);
+        delete pl;
+    }
+    delete g;
+    return LPMA_OK;
+}
+
+int lpma_group_update(lpma_group_t *g, const uint64_t *const *d_ins, const uint64_t *n_ins,
+                      const uint64_t *const *d_del, const uint64_t *n_del, void *stream) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    const u32 n = (u32)g->members.size();
+    for (u32 i = 0; i < n; ++i) {
+        u64 ni = n_ins ? n_ins[i] : 0, nd = n_del ? n_del[i] : 0;
+        if ((ni && (!d_ins || !d_ins[i])) || (nd && (!d_del || !d_del[i]))) return LPMA_ERR_INVALID_ARGUMENT;
+    }
+    DevGuard guard(g->device);
+    g->calls++;
+    std::vector<Member> mem;
+    for (size_t k = 0; k < g->parts.size(); ++k) {
+        mem.clear();
+        for (u32 i : g->parts[k]) {
+            Member m;
+            m.h = g->members[i];
+            m.nins = n_ins ? n_ins[i] : 0;
+            m.ndel = n_del ? n_del[i] : 0;
+            m.ins = m.nins ? d_ins[i] : nullptr;
+            m.del = m.ndel ? d_del[i] : nullptr;
+            mem.push_back(m);
+        }
+        CHECKED(run_pipeline(*g->pls[k], mem.data(), (u32)mem.size(), (hipStream_t)stream));
+    }
+    return LPMA_OK;
+}
+
+uint32_t lpma_group_partitions(const lpma_group_t *g) { return g ? (uint32_t)g->parts.size() : 0; }
+
+int lpma_group_get_stats(lpma_group_t *g, lpma_stats_t *out) {
+    if (!g || !out) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(g->device);
+    memset(out, 0, sizeof(*out));
+    for (size_t k = 0; k < g->pls.size(); ++k) {
+        Pipeline &pl = *g->pls[k];
+        CHECKED(read_stats(pl, g->members[g->parts[k][0]]->log2S, out));
+        out->effective_updates += pl.effective;
+        out->rebuilds += pl.rebuilds;
+        out->rebuild_slots += pl.rebuild_slots;
+    }
+    out->updates_calls = g->calls;
+    return LPMA_OK;
+}
+
+int lpma_group_set_profiling(lpma_group_t *g, int on) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    for (Pipeline *pl : g->pls) pl->profiling = on != 0;
+    return LPMA_OK;
+}
+
+int lpma_group_get_profile(lpma_group_t *g, lpma_profile_t *out) {
+    if (!g || !out) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(g->device);
+    memset(out, 0, sizeof(*out));
+    for (Pipeline *pl : g->pls) add_profile(*pl, out);
     return LPMA_OK;
 }
 

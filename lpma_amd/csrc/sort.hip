@@ -7,6 +7,7 @@
 // 2*vbits+1 bits are sorted (41 bits, 6 passes, for V = 2^20), and equal
 // keys end up with the delete before the insert; the dedup that follows
 // keeps the last of each run, which makes a batch "deletes, then inserts".
+// A group of PMAs sorts as one batch: the PMA id sits above the packed key.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -21,17 +22,54 @@ static constexpr int RS_SMALL_TPB = 1024;
 static constexpr int RS_SMALL_CAP = 8192;  // single-workgroup sort (2 x 64 KiB of LDS)
 static constexpr u32 RS_PREFIX_MAX_TILES = 512;  // scatter derives its offsets itself up to here
 
-__global__ void k_pack(const u64 *__restrict__ in, u64 n, u32 op, u32 V, u32 vbits,
-                       u64 *__restrict__ out, u32 *__restrict__ flags) {
-    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+// Batch element i comes from the source whose [off, off + n) holds it.
+// Sources are few and sorted by off; each thread finds its first source by
+// binary search and walks forward from there.
+__device__ __forceinline__ u32 source_of(const BatchSeg *__restrict__ segs, u32 nsegs, u64 i) {
+    u32 lo = 0, hi = nsegs;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (segs[mid].off <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Packs a group batch into sort keys pid << pshift | (src << vbits | dst) << 1 | op
+// (keys with src or dst >= V raise FLAG_INVALID_KEY) and, with HIST, counts
+// the first radix digit of every tile.
+template <bool HIST>
+__global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
+                                                        u32 pshift, u64 *__restrict__ out, u32 *__restrict__ flags,
+                                                        u32 shift, u32 items, u32 *__restrict__ hist, u32 nblocks,
+                                                        int block_major) {
+    __shared__ u32 h[RS_RADIX];
+    if (HIST) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    const u64 base = (u64)blockIdx.x * items * RS_TPB + threadIdx.x;
     bool bad = false;
-    for (; i < n; i += (u64)gridDim.x * blockDim.x) {
-        u64 k = in[i];
-        u32 src = (u32)(k >> 32), dst = (u32)k;
-        if (src >= V || dst >= V) { bad = true; src = 0; dst = 0; }
-        out[i] = (((((u64)src) << vbits) | dst) << 1) | op;
+    if (base < n) {
+        u32 s = source_of(segs, nsegs, base);
+        BatchSeg sg = segs[s];
+        for (u32 r = 0; r < items; ++r) {
+            u64 i = base + (u64)r * RS_TPB;
+            if (i >= n) break;
+            while (i >= sg.off + sg.n) sg = segs[++s];
+            u64 k = sg.keys[i - sg.off];
+            u32 src = (u32)(k >> 32), dst = (u32)k;
+            if (src >= sg.V || dst >= sg.V) { bad = true; src = 0; dst = 0; }
+            u64 p = ((u64)sg.pid << pshift) | (((((u64)src) << sg.vbits) | dst) << 1) | sg.op;
+            out[i] = p;
+            if (HIST) atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
+        }
     }
     if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
+    if (HIST) {
+        __syncthreads();
+        if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
+        else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    }
 }
 
 // ------------------------------------------------------------ stable ranking
@@ -81,36 +119,6 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s
         s.wcnt[w][digit] = 0;
     }
     __syncthreads();
-}
-
-// Packs the batch (a with op_a, then b with op_b) into sortable keys and
-// counts the first radix digit of every tile in the same pass.
-__global__ __launch_bounds__(RS_TPB) void k_pack_hist(const u64 *__restrict__ a, u64 na, u32 op_a,
-                                                       const u64 *__restrict__ b, u64 nb, u32 op_b, u32 V, u32 vbits,
-                                                       u64 *__restrict__ out, u32 *__restrict__ flags, u32 shift,
-                                                       u32 items, u32 *__restrict__ hist, u32 nblocks,
-                                                       int block_major) {
-    __shared__ u32 h[RS_RADIX];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const u64 n = na + nb;
-    u64 base = (u64)blockIdx.x * items * RS_TPB;
-    bool bad = false;
-    for (u32 r = 0; r < items; ++r) {
-        u64 i = base + (u64)r * RS_TPB + threadIdx.x;
-        if (i >= n) break;
-        u64 k = i < na ? a[i] : b[i - na];
-        u32 op = i < na ? op_a : op_b;
-        u32 src = (u32)(k >> 32), dst = (u32)k;
-        if (src >= V || dst >= V) { bad = true; src = 0; dst = 0; }
-        u64 p = (((((u64)src) << vbits) | dst) << 1) | op;
-        out[i] = p;
-        atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
-    }
-    if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
-    __syncthreads();
-    if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
-    else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
 // per-tile digit histogram; layout block-major (hist[b*RADIX + d]) when
@@ -174,7 +182,6 @@ __global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ key
     __shared__ u64 buf[2][RS_SMALL_CAP];
     __shared__ RankLds<RS_SMALL_TPB> s;
     __shared__ u32 h[RS_RADIX];
-    __shared__ u32 scan_tmp[RS_RADIX / 64 + 1];
     for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) buf[0][i] = keys[i];
     int cur = 0;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
@@ -295,12 +302,6 @@ u64 sort_scratch_words(u64 n) {
     return (u64)RS_RADIX * nb + 1 + scan_scratch_words((u64)RS_RADIX * nb);
 }
 
-hipError_t pack_batch(const u64 *in, u64 n, u32 op, u32 V, u32 vbits, u64 *out, u32 *flags, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    k_pack<<<grid_for(n, 256, 8192), 256, 0, st>>>(in, n, op, V, vbits, out, flags);
-    return hipGetLastError();
-}
-
 // Sorts keys[0..n) on bits [lo_bit, hi_bit).  Returns the buffer that holds
 // the result (keys or alt).
 u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st) {
@@ -325,15 +326,15 @@ u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t
     return src;
 }
 
-// Packs a batch (a: op_a, b: op_b) and sorts the packed keys on bits
-// [lo_bit, hi_bit): the first histogram is taken while packing.
-u64 *pack_and_sort(const u64 *a, u64 na, u32 op_a, const u64 *b, u64 nb, u32 op_b, u32 V, u32 vbits, u32 lo_bit,
-                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
-    const u64 n = na + nb;
+// Packs a group batch (n elements from nsegs sources) and sorts the packed
+// keys on bits [lo_bit, hi_bit): the first histogram is taken while packing.
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, u32 pshift, u32 lo_bit, u32 hi_bit, u64 *k0, u64 *k1,
+                   u32 *scratch, u32 *flags, hipStream_t st) {
     if (n == 0) return k0;
     if (n <= (u64)RS_SMALL_CAP) {
-        (void)pack_batch(a, na, op_a, V, vbits, k0, flags, st);
-        (void)pack_batch(b, nb, op_b, V, vbits, k0 + na, flags, st);
+        const u32 items = 4;
+        k_pack_group<false><<<(u32)((n + items * RS_TPB - 1) / (items * RS_TPB)), RS_TPB, 0, st>>>(
+            segs, nsegs, n, pshift, k0, flags, 0, items, nullptr, 0, 0);
         if (n > 1) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, lo_bit, hi_bit);
         return k0;
     }
@@ -346,8 +347,8 @@ u64 *pack_and_sort(const u64 *a, u64 na, u32 op_a, const u64 *b, u64 nb, u32 op_
     u64 *src = k0, *dst = k1;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
         if (shift == lo_bit)
-            k_pack_hist<<<nblk, RS_TPB, 0, st>>>(a, na, op_a, b, nb, op_b, V, vbits, k0, flags, shift, items, hist,
-                                                 nblk, block_major);
+            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, pshift, k0, flags, shift, items, hist, nblk,
+                                                       block_major);
         else
             k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, shift, items, hist, nblk, block_major);
         if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);

@@ -18,24 +18,29 @@
 //                                        the multi-workgroup global path
 //   resize / rebuild (:1301-1530)     -> rebuild_push + rebuild_spread
 //   row-offset fixup (:652-657, gpma.cuh:340-344) -> fused into the spread.
+//
+// Every kernel works on a Group: one or many independent PMAs whose batch
+// elements carry their PMA id, so a round over many PMAs costs one launch
+// sequence.  A run (the updates of one segment) is keyed pid << 32 | seg.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace pma {
 
 static constexpr int LOC_TPB = 256;
-static constexpr int LOC_ITEMS = 1;
-static constexpr int LOC_TILE = LOC_TPB * LOC_ITEMS;
+static constexpr int LOC_TILE = LOC_TPB;  // one batch element per thread
 
 u32 locate_blocks(u64 n) { return (u32)((n + LOC_TILE - 1) / LOC_TILE); }
 
-// Zeroes the batch status and finishes the count pyramid left by the
-// previous batch: the levels above the per-workgroup chunks of k_pyr_chunk
-// (nseg >> 11 entries and fewer), so no separate launch is needed for them.
-__global__ __launch_bounds__(1024) void k_batch_begin(Status *s, DevState d);
+// One workgroup per PMA: zeroes the run status and the PMA's outcome,
+// stores inline batch descriptors, and finishes the count pyramid left by
+// the previous batch: the levels above the per-workgroup chunks of
+// k_pyr_chunk (nseg >> 11 entries and fewer), so no separate launch is
+// needed for them.
+__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs);
 
-hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st) {
-    k_batch_begin<<<1, 1024, 0, st>>>(status, d);
+hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, hipStream_t st) {
+    k_batch_begin<<<g.npma, 1024, 0, st>>>(g, status, in, segs);
     return hipGetLastError();
 }
 
@@ -43,54 +48,34 @@ hipError_t batch_begin(Status *status, const DevState &d, hipStream_t st) {
 
 // Dedup (keep the last of equal keys: deletes sort before inserts), route
 // each key to its segment, and resolve it against the keys present there.
-__global__ __launch_bounds__(LOC_TPB) void k_locate(DevState d, const u64 *__restrict__ pk, u64 n,
-                                                    u64 *__restrict__ ukey, u32 *__restrict__ useg,
+__global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restrict__ pk, u64 n,
+                                                    u64 *__restrict__ ukey, u64 *__restrict__ useg,
                                                     i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
-    const u32 vmask = (u32)((1ull << d.vbits) - 1);
-    const u64 i0 = (u64)blockIdx.x * LOC_TILE + (u64)threadIdx.x * LOC_ITEMS;
-    u64 key[LOC_ITEMS];
-    u32 op[LOC_ITEMS], b[LOC_ITEMS];
-    bool act[LOC_ITEMS];
-#pragma unroll
-    for (int r = 0; r < LOC_ITEMS; ++r) {
-        u64 i = i0 + r;
-        act[r] = false;
-        key[r] = 0;
-        op[r] = 0;
-        b[r] = 0;
-        if (i < n) {
-            u64 p = pk[i];
-            u64 kk = p >> 1;
-            op[r] = (u32)(p & 1);
-            bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == kk);
-            act[r] = !dup;
-            key[r] = ((kk >> d.vbits) << 32) | (u64)((u32)kk & vmask);
-            if (dup) useg[i] = SEG_NONE;
-        }
-    }
-    for (u32 step = (1u << d.H) >> 1; step; step >>= 1) {
-        u64 v[LOC_ITEMS];
-#pragma unroll
-        for (int r = 0; r < LOC_ITEMS; ++r) v[r] = d.sep[b[r] + step];
-#pragma unroll
-        for (int r = 0; r < LOC_ITEMS; ++r)
-            if (v[r] <= key[r]) b[r] += step;
-    }
+    const u64 pmask = (1ull << g.pshift) - 1;
+    const u64 i = (u64)blockIdx.x * LOC_TILE + threadIdx.x;
     i32 eff = 0, dl = 0;
-#pragma unroll
-    for (int r = 0; r < LOC_ITEMS; ++r) {
-        u64 i = i0 + r;
-        if (!act[r]) continue;
-        u32 s = b[r];
-        u32 present = segment_holds(d, s, key[r]) ? 1u : 0u;
-        u32 sg = SEG_NONE;
-        if (op[r] != present) {
-            eff += 1;
-            dl += op[r] ? 1 : -1;
-            sg = s | (op[r] ? OP_BIT : 0u);
+    if (i < n) {
+        const u64 p = pk[i];
+        const bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == (p >> 1));
+        if (dup) {
+            useg[i] = RK_NONE;
+        } else {
+            const u32 pid = (u32)(p >> g.pshift);
+            const DevState d = g.tab[pid].d;
+            const u64 kk = (p & pmask) >> 1;
+            const u32 op = (u32)(p & 1);
+            const u64 key = ((kk >> d.vbits) << 32) | (kk & ((1ull << d.vbits) - 1));
+            const u32 s = locate_segment(d.sep, d.H, key);
+            const u32 present = segment_holds(d, s, key) ? 1u : 0u;
+            u64 rk = RK_NONE;
+            if (op != present) {
+                eff = 1;
+                dl = op ? 1 : -1;
+                rk = ((u64)pid << 32) | s | (op ? RK_OP : 0ull);
+            }
+            ukey[i] = key;
+            useg[i] = rk;
         }
-        ukey[i] = key[r];
-        useg[i] = sg;
     }
     __shared__ i32 tmp[2][LOC_TPB / 64];
     i32 ce = wave_sum(eff), cd = wave_sum(dl);
@@ -131,9 +116,9 @@ static constexpr u32 COMPACT_SELF_PREFIX_MAX = 2048;  // blocks
 // Stream compaction of the effective updates.  With few blocks every block
 // derives its prefix (and the totals) from the per-block partials itself;
 // otherwise k_scan_parts has already scanned them.
-__global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u32 *__restrict__ useg, u64 n,
+__global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u64 *__restrict__ useg, u64 n,
                                                      const i32 *__restrict__ pc, const i32 *__restrict__ pdl, u32 nb,
-                                                     int self_prefix, u64 *__restrict__ ckey, u32 *__restrict__ cseg,
+                                                     int self_prefix, u64 *__restrict__ ckey, u64 *__restrict__ cseg,
                                                      i32 *__restrict__ pd, Status *__restrict__ status) {
     __shared__ i32 tmp[LOC_TPB / 64 + 1];
     __shared__ i32 red[4][LOC_TPB / 64];
@@ -165,33 +150,24 @@ __global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ uke
         base_c = pc[blockIdx.x];
         base_d = pdl[blockIdx.x];
     }
-    // blocked: thread t owns entries i0 .. i0 + LOC_ITEMS - 1
-    const u64 i0 = (u64)blockIdx.x * LOC_TILE + (u64)threadIdx.x * LOC_ITEMS;
-    u32 sg[LOC_ITEMS];
+    const u64 i = (u64)blockIdx.x * LOC_TILE + threadIdx.x;
+    const u64 sg = i < n ? useg[i] : RK_NONE;
     i32 ce = 0, cd = 0;
-#pragma unroll
-    for (int r = 0; r < LOC_ITEMS; ++r) {
-        sg[r] = (i0 + r) < n ? useg[i0 + r] : SEG_NONE;
-        if (sg[r] != SEG_NONE) { ce += 1; cd += (sg[r] & OP_BIT) ? 1 : -1; }
-    }
+    if (sg != RK_NONE) { ce = 1; cd = (sg & RK_OP) ? 1 : -1; }
     i32 t1, t2;
     i32 ex = block_excl_scan<LOC_TPB, i32>(ce, t1, tmp) + base_c;
     i32 dx = block_excl_scan<LOC_TPB, i32>(cd, t2, tmp) + base_d;
-#pragma unroll
-    for (int r = 0; r < LOC_ITEMS; ++r) {
-        if (sg[r] == SEG_NONE) continue;
-        ckey[ex] = ukey[i0 + r];
-        cseg[ex] = sg[r] & ~OP_BIT;
+    if (sg != RK_NONE) {
+        ckey[ex] = ukey[i];
+        cseg[ex] = sg & ~RK_OP;
         pd[ex] = dx;
-        ex += 1;
-        dx += (sg[r] & OP_BIT) ? 1 : -1;
     }
 }
 
-hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st) {
+hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st) {
     u32 nb = locate_blocks(n);
     i32 *pc = b.part, *pdl = b.part + nb + 1;
-    k_locate<<<nb, LOC_TPB, 0, st>>>(d, pk, n, b.ukey, b.useg, pc, pdl);
+    k_locate<<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
     int self_prefix = nb <= COMPACT_SELF_PREFIX_MAX;
     if (!self_prefix) k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
     k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd, b.status);
@@ -201,7 +177,7 @@ hipError_t locate_and_compact(const DevState &d, const u64 *pk, u64 n, const Bat
 // ------------------------------------------------------------ density check
 
 // first index >= start with a[idx] >= target, given a[start-1] < target
-__device__ __forceinline__ u32 gallop_fwd(const u32 *__restrict__ a, u32 start, u32 n, u64 target) {
+__device__ __forceinline__ u32 gallop_fwd(const u64 *__restrict__ a, u32 start, u32 n, u64 target) {
     u32 lo = start;
     if (lo >= n || a[lo] >= target) return lo;
     u32 step = 1;
@@ -221,7 +197,7 @@ __device__ __forceinline__ u32 gallop_fwd(const u32 *__restrict__ a, u32 start, 
 }
 
 // smallest index <= start whose suffix [idx, start] is >= target, given a[start] >= target
-__device__ __forceinline__ u32 gallop_back(const u32 *__restrict__ a, u32 start, u64 target) {
+__device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start, u64 target) {
     if (start == 0 || a[start - 1] < target) return start;
     u32 hi = start - 1;  // a[hi] >= target
     u32 step = 1;
@@ -242,35 +218,53 @@ __device__ __forceinline__ u32 gallop_back(const u32 *__restrict__ a, u32 start,
     return l;
 }
 
-__global__ void k_density(DevState d, Density dn, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+// Every run leader (first update of a run) picks the lowest level whose
+// window fits after the batch, walking up the count pyramid; updates of the
+// window are counted by galloping over the sorted run keys.  The first and
+// last update of each PMA record its compacted range.
+__global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const Status *__restrict__ status_in, u32 *__restrict__ rlevel, Status *status) {
-    u32 n_eff = status_in->n_eff;
-    u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    const u32 n_eff = status_in->n_eff;
+    const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     u32 local_flags = 0;
     if (j < n_eff) {
-        u32 s = cseg[j];
-        if (j == 0 || cseg[j - 1] != s) {
+        const u64 rk = cseg[j];
+        const u32 pid = (u32)(rk >> 32);
+        const u64 prev = j > 0 ? cseg[j - 1] : RK_NONE;
+        if (j == 0 || (u32)(prev >> 32) != pid) {
+            g.pstat[pid].u0 = j;
+            g.pstat[pid].pd0 = pd[j];
+        }
+        if (j + 1 == n_eff || (u32)(cseg[j + 1] >> 32) != pid) {
+            g.pstat[pid].u1 = j + 1;
+            g.pstat[pid].pd1 = pd[j + 1];
+        }
+        if (j == 0 || prev != rk) {
+            const PmaEntry *e = g.tab + pid;
+            const DevState d = e->d;
+            const u64 pb = (u64)pid << 32;
+            const u32 s = (u32)rk;
             u32 u0 = j, u1 = j + 1;
             u32 l = 0;
             u64 C = 0;
             for (; l <= d.H; ++l) {
                 u64 w = (u64)s >> l;
                 u64 a = w << l;
-                u64 b = a + (1ull << l);
-                u0 = gallop_back(cseg, u0, a);
-                u1 = gallop_fwd(cseg, u1, n_eff, b);
+                u0 = gallop_back(cseg, u0, pb + a);
+                u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l));
                 C = *count_ptr(d, l, w);
                 i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
-                if (window_fits(dn, l, d.H, (u64)m)) break;
+                if (window_fits(e->dn, l, d.H, (u64)m)) break;
             }
             rlevel[j] = l;
             if (l >= d.H) {
                 local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
+                atomicOr(&g.pstat[pid].flags, local_flags);
             } else {
                 if (work_class(C + (u1 - u0), l, d.log2S) == CLASS_LARGE) local_flags = FLAG_LARGE;
                 // claim the window; the smallest run index wins (atomic max of ~j)
                 atomicMax((unsigned long long *)&d.owner[owner_idx(nseg_of(d), l, (u64)s >> l)],
-                          (unsigned long long)(((u64)d.epoch << 32) | (u64)(~j)));
+                          (unsigned long long)(((u64)g.epoch << 32) | (u64)(~j)));
             }
         }
     }
@@ -281,29 +275,33 @@ __global__ void k_density(DevState d, Density dn, const u32 *__restrict__ cseg, 
     if (lane_id() == 0 && f) atomicOr(&status->flags, f);
 }
 
-hipError_t density_check(const DevState &d, const Density &dn, u64 n, const BatchBufs &b, hipStream_t st) {
-    k_density<<<grid_for(n, 256), 256, 0, st>>>(d, dn, b.cseg, b.pd, b.status, b.rlevel, b.status);
+hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
+    k_density<<<grid_for(n, 256), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------- window election
 
-__global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+__global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
                           Status *status) {
-    // a root rebalance or a resize redoes everything: no windows needed
-    if (status->flags & (FLAG_ROOT | FLAG_RESIZE | FLAG_INVALID_KEY)) return;
+    if (status->flags & FLAG_INVALID_KEY) return;
     const u32 n_eff = status->n_eff;
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
     Work wk;
     if (j < n_eff) {
-        u32 s = cseg[j];
-        bool leader = (j == 0 || cseg[j - 1] != s);
+        const u64 rk = cseg[j];
+        const u32 pid = (u32)(rk >> 32);
+        bool leader = (j == 0 || cseg[j - 1] != rk);
+        // a root rebalance or a resize redoes the whole PMA: no windows needed
+        if (leader && (g.pstat[pid].flags & (FLAG_ROOT | FLAG_RESIZE))) leader = false;
+        const DevState d = g.tab[pid].d;
+        const u32 s = (u32)rk;
         u32 L = leader ? rlevel[j] : 0;
         u64 nseg = nseg_of(d);
         u64 w = (u64)s >> L;
-        const u64 tag = (u64)d.epoch << 32;
+        const u64 tag = (u64)g.epoch << 32;
         if (leader && d.owner[owner_idx(nseg, L, w)] != (tag | (u64)(~j))) leader = false;
         if (leader) {
             // inside a larger chosen window?  the loads are independent, so
@@ -311,14 +309,16 @@ __global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restr
             bool dom = false;
 #pragma unroll 8
             for (u32 l2 = L + 1; l2 < d.H; ++l2)
-                dom |= (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] >> 32) == (u64)d.epoch;
+                dom |= (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] >> 32) == (u64)g.epoch;
             if (dom) leader = false;
         }
         if (leader) {
+            const u64 pb = (u64)pid << 32;
             u64 a = w << L;
-            u32 u0 = gallop_back(cseg, j, a);
-            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, a + (1ull << L));
+            u32 u0 = gallop_back(cseg, j, pb + a);
+            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, pb + a + (1ull << L));
             u32 C = *count_ptr(d, L, w);
+            wk.pid = pid;
             wk.a = (u32)a;
             wk.L = L;
             wk.u0 = u0;
@@ -347,9 +347,9 @@ __global__ __launch_bounds__(1024) void k_leaders(DevState d, const u32 *__restr
 
 // n bounds the number of runs (the batch size); the kernel reads the exact
 // count from the status
-hipError_t elect_windows(const DevState &d, u64 n, const BatchBufs &b, hipStream_t st) {
-    u32 g = grid_for(n, 1024);
-    k_leaders<<<g, 1024, 0, st>>>(d, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2], b.work[3], b.status);
+hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
+    k_leaders<<<grid_for(n, 1024), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2], b.work[3],
+                                                   b.status);
     return hipGetLastError();
 }
 
@@ -469,8 +469,8 @@ __device__ void flush_stats(BlockStats &bs, u64 *stats_all, int cls_slot, u32 ti
 }
 
 template <int NT, u32 ECAP, u32 SEGCAP>
-__device__ void rebalance_loop(const DevState &d, const Work *__restrict__ work, const u32 *nwork,
-                               const u64 *__restrict__ ckey, const i32 *__restrict__ pd, u64 *stats, int cls_slot) {
+__device__ void rebalance_loop(const Group &g, const Work *__restrict__ work, const u32 *nwork,
+                               const u64 *__restrict__ ckey, const i32 *__restrict__ pd, int cls_slot) {
     __shared__ u64 A[ECAP];
     __shared__ u64 B[ECAP];
     __shared__ i32 P[ECAP + 1];
@@ -485,10 +485,11 @@ __device__ void rebalance_loop(const DevState &d, const Work *__restrict__ work,
     const u32 nw = *nwork;
     for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
         Work wk = work[i];
+        const DevState d = g.tab[wk.pid].d;
         rebalance_window<NT, ECAP, SEGCAP>(d, wk, ckey, pd, A, B, P, pre, stmp, bs);
     }
     __syncthreads();
-    flush_stats(bs, stats, cls_slot, threadIdx.x);
+    flush_stats(bs, g.stats, cls_slot, threadIdx.x);
 }
 
 // One wave64 per window (E <= WAVE_E_CAP, <= WAVE_SEG_CAP segments).  The
@@ -584,10 +585,9 @@ __device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_rebalance_wave(DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(64) void k_rebalance_wave(Group g, const Work *__restrict__ work,
                                                        const Status *__restrict__ status,
-                                                       const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
-                                                       u64 *stats) {
+                                                       const u64 *__restrict__ ckey, const i32 *__restrict__ pd) {
     __shared__ u64 A[WAVE_E_CAP];
     __shared__ u64 B[WAVE_E_CAP];
     __shared__ i32 P[WAVE_E_CAP + 1];
@@ -600,17 +600,17 @@ __global__ __launch_bounds__(64) void k_rebalance_wave(DevState d, const Work *_
     const u32 nw = status->nwork[CLASS_WAVE];
     for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
         Work wk = work[i];
+        const DevState d = g.tab[wk.pid].d;
         rebalance_window_wave(d, wk, ckey, pd, A, B, P, bs);
     }
     __syncthreads();
-    flush_stats(bs, stats, ST_ALG_BYTES_WAVE, threadIdx.x);
+    flush_stats(bs, g.stats, ST_ALG_BYTES_WAVE, threadIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(256) void k_rebalance_block(Group g, const Work *__restrict__ work,
                                                          const Status *__restrict__ status,
-                                                         const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
-                                                         u64 *stats) {
-    rebalance_loop<256, BLOCK_E_CAP, BLOCK_SEG_CAP>(d, work, &status->nwork[CLASS_BLOCK], ckey, pd, stats,
+                                                         const u64 *__restrict__ ckey, const i32 *__restrict__ pd) {
+    rebalance_loop<256, BLOCK_E_CAP, BLOCK_SEG_CAP>(g, work, &status->nwork[CLASS_BLOCK], ckey, pd,
                                                     ST_ALG_BYTES_BLOCK);
 }
 
@@ -623,13 +623,13 @@ __global__ __launch_bounds__(256) void k_rebalance_block(DevState d, const Work 
 // another lane has yet to read.
 static constexpr int SEG_TPB = 256;
 
-__global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *__restrict__ work,
                                                            const Status *__restrict__ status,
                                                            const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
-                                                           u64 *stats_all, u32 log2G) {
+                                                           u32 log2G) {
     extern __shared__ u64 lds[];
     const u32 G = 1u << log2G;
-    const u32 S = 1u << d.log2S;
+    const u32 S = 1u << g.log2S;
     const u32 SPL = S >> log2G;  // slots per lane
     const u32 lane_g = threadIdx.x & (G - 1);
     const u32 grp = threadIdx.x >> log2G;
@@ -640,9 +640,10 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Wor
     u64 bytes = 0, upd = 0, markers = 0, wins = 0;
     for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += gridDim.x * groups_per_block) {
         const Work wk = work[w];
+        const DevState d = g.tab[wk.pid].d;
         const u32 C = wk.C;
         const u32 k = wk.u1 - wk.u0;
-        const u64 base = (u64)wk.a << d.log2S;
+        const u64 base = (u64)wk.a << g.log2S;
         for (u32 r = 0; r < SPL; ++r) {
             u32 t = lane_g * SPL + r;
             if (t < C) buf[t] = d.keys[base + t];
@@ -698,7 +699,7 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Wor
     bytes = wave_sum(bytes);
     upd = wave_sum(upd);
     wins = wave_sum(wins);
-    u64 *row = stats_all + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    u64 *row = g.stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
     if (lane_id() == 0 && wins) {
         atomicAdd((unsigned long long *)&row[ST_WINDOWS + 0], (unsigned long long)wins);
         atomicAdd((unsigned long long *)&row[ST_UPDATES], (unsigned long long)upd);
@@ -707,22 +708,22 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(DevState d, const Wor
     }
 }
 
-hipError_t rebalance_seg(const DevState &d, const BatchBufs &b, u64 windows, hipStream_t st) {
-    u32 S = 1u << d.log2S;
-    u32 log2G = d.log2S >= 8 ? 6 : (d.log2S <= 2 ? 0 : d.log2S - 2);
+hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st) {
+    u32 S = 1u << g.log2S;
+    u32 log2G = g.log2S >= 8 ? 6 : (g.log2S <= 2 ? 0 : g.log2S - 2);
     u32 groups_per_block = SEG_TPB >> log2G;
     size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
     u64 blocks = (windows + groups_per_block - 1) / groups_per_block;
     u32 grid = (u32)(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks));
-    k_rebalance_seg<<<grid, SEG_TPB, lds, st>>>(d, b.work[CLASS_SEG], b.status, b.ckey, b.pd, b.stats, log2G);
+    k_rebalance_seg<<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G);
     return hipGetLastError();
 }
 
-hipError_t rebalance_lds(const DevState &d, u64 n_eff, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
+hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
     if (cls == CLASS_WAVE)
-        k_rebalance_wave<<<grid, 64, 0, st>>>(d, b.work[CLASS_WAVE], b.status, b.ckey, b.pd, b.stats);
+        k_rebalance_wave<<<grid, 64, 0, st>>>(g, b.work[CLASS_WAVE], b.status, b.ckey, b.pd);
     else
-        k_rebalance_block<<<grid, 256, 0, st>>>(d, b.work[CLASS_BLOCK], b.status, b.ckey, b.pd, b.stats);
+        k_rebalance_block<<<grid, 256, 0, st>>>(g, b.work[CLASS_BLOCK], b.status, b.ckey, b.pd);
     return hipGetLastError();
 }
 
@@ -754,13 +755,13 @@ __global__ void k_push_old(DevState s, u64 a, u32 L, const u32 *__restrict__ gpr
 }
 
 __global__ void k_push_ins(DevState s, u64 a, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
-                           const u32 *__restrict__ cseg, const i32 *__restrict__ pd, u32 u0, u32 u1,
+                           const u64 *__restrict__ cseg, const i32 *__restrict__ pd, u32 u0, u32 u1,
                            u64 *__restrict__ merged, u64 m) {
     const i32 pbase = pd[u0];
     for (u64 q = (u64)u0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; q < u1; q += (u64)gridDim.x * blockDim.x) {
         if (pd[q + 1] - pd[q] != 1) continue;
         u64 y = ckey[q];
-        u64 seg = cseg[q];
+        u64 seg = (u32)cseg[q];
         const u64 *sk = s.keys + (seg << s.log2S);
         u64 o = (u64)gpre[seg - a] + lower_bound_dev<u64>(sk, 0, s.cnt[seg], y) + (i64)(pd[q] - pbase);
         if (o < m) merged[o] = y;
@@ -817,8 +818,8 @@ __global__ void k_meta(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ 
 // window gets a private slice of the merged scratch (windows are disjoint,
 // so the slices add up to at most the live key count).
 
-__device__ __forceinline__ u32 chunk_level(const DevState &d, u32 L) {
-    u32 cl = d.log2S >= LARGE_CHUNK_LOG2_SLOTS ? 0 : LARGE_CHUNK_LOG2_SLOTS - d.log2S;
+__device__ __forceinline__ u32 chunk_level(u32 log2S, u32 L) {
+    u32 cl = log2S >= LARGE_CHUNK_LOG2_SLOTS ? 0 : LARGE_CHUNK_LOG2_SLOTS - log2S;
     return L < cl ? L : cl;
 }
 
@@ -834,7 +835,7 @@ __device__ __forceinline__ u64 rank_in_window(const DevState &d, u64 a, u32 L, u
     return off;
 }
 
-__global__ __launch_bounds__(1024) void k_large_prep(const DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(1024) void k_large_prep(const Group g, const Work *__restrict__ work,
                                                      const Status *__restrict__ status, u32 *__restrict__ chunk_base,
                                                      u64 *__restrict__ mbase, u32 *__restrict__ hdr) {
     __shared__ u32 tmp[1024 / 64 + 1];
@@ -848,7 +849,7 @@ __global__ __launch_bounds__(1024) void k_large_prep(const DevState d, const Wor
         u64 m = 0;
         if (i < n) {
             Work w = work[i];
-            nch = 1u << (w.L - chunk_level(d, w.L));
+            nch = 1u << (w.L - chunk_level(g.log2S, w.L));
             m = w.M;
         }
         u32 tot;
@@ -882,21 +883,23 @@ __device__ __forceinline__ u32 window_of_chunk(const u32 *chunk_base, u32 n, u32
 static constexpr int LG_TPB = 256;
 static constexpr u32 LG_SEG_MAX = 1u << LARGE_CHUNK_LOG2_SLOTS;  // segments per chunk at S = 1
 
-__global__ __launch_bounds__(LG_TPB) void k_large_push(DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__restrict__ work,
                                                         const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
                                                         const u32 *__restrict__ hdr, const u64 *__restrict__ ckey,
-                                                        const u32 *__restrict__ cseg, const i32 *__restrict__ pd,
+                                                        const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                                                         u64 *__restrict__ merged) {
     __shared__ u32 pre[LG_SEG_MAX + 1];
     __shared__ u32 stmp[LG_TPB / 64 + 1];
     __shared__ u32 qr[2];
     const u32 nwin = hdr[0], nch = hdr[1];
-    const u32 log2S = d.log2S;
+    const u32 log2S = g.log2S;
     const u32 Smask = (1u << log2S) - 1;
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
-        const u32 cl = chunk_level(d, wk.L);
+        const DevState d = g.tab[wk.pid].d;
+        const u64 pb = (u64)wk.pid << 32;
+        const u32 cl = chunk_level(log2S, wk.L);
         const u64 j = c - chunk_base[i];
         const u64 s0 = (u64)wk.a + (j << cl);
         const u32 nsc = 1u << cl;
@@ -905,8 +908,8 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(DevState d, const Work *_
         const i32 pbase = pd[wk.u0];
         for (u32 x = threadIdx.x; x < nsc; x += LG_TPB) pre[x] = d.cnt[s0 + x];
         if (threadIdx.x == 0) {
-            qr[0] = (u32)lower_bound_dev<u32>(cseg, wk.u0, wk.u1, (u32)s0);
-            qr[1] = (u32)lower_bound_dev<u32>(cseg, wk.u0, wk.u1, (u32)(s0 + nsc));
+            qr[0] = (u32)lower_bound_dev<u64>(cseg, wk.u0, wk.u1, pb + s0);
+            qr[1] = (u32)lower_bound_dev<u64>(cseg, wk.u0, wk.u1, pb + s0 + nsc);
         }
         __syncthreads();
         block_scan_array<LG_TPB>(pre, nsc, stmp);
@@ -925,7 +928,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(DevState d, const Work *_
         for (u32 q = qr[0] + threadIdx.x; q < qr[1]; q += LG_TPB) {
             if (pd[q + 1] - pd[q] != 1) continue;
             u64 y = ckey[q];
-            u64 sg = cseg[q];
+            u64 sg = (u32)cseg[q];
             u64 o = obase + pre[sg - s0] + lower_bound_dev<u64>(d.keys + (sg << log2S), 0, d.cnt[sg], y) +
                     (i64)(pd[q] - pbase);
             if (o < wk.M) merged[mb + o] = y;
@@ -934,12 +937,11 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(DevState d, const Work *_
     }
 }
 
-__global__ __launch_bounds__(LG_TPB) void k_large_pull(DevState d, const Work *__restrict__ work,
+__global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__restrict__ work,
                                                         const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
-                                                        const u32 *__restrict__ hdr, const u64 *__restrict__ merged,
-                                                        u64 *stats_all) {
+                                                        const u32 *__restrict__ hdr, const u64 *__restrict__ merged) {
     const u32 nwin = hdr[0], nch = hdr[1];
-    const u32 log2S = d.log2S;
+    const u32 log2S = g.log2S;
     const u32 Smask = (1u << log2S) - 1;
     u64 bytes = 0, markers = 0, upd = 0;
     __shared__ u32 windows[32];
@@ -949,7 +951,8 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(DevState d, const Work *_
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
-        const u32 cl = chunk_level(d, wk.L);
+        const DevState d = g.tab[wk.pid].d;
+        const u32 cl = chunk_level(log2S, wk.L);
         const u64 j = c - chunk_base[i];
         const u64 rel0 = j << cl;
         const u32 nsc = 1u << cl;
@@ -988,7 +991,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(DevState d, const Work *_
         }
     }
     markers = wave_sum(markers);
-    u64 *stats = stats_all + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    u64 *stats = g.stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
     if (lane_id() == 0 && markers) {
         atomicAdd((unsigned long long *)&stats[ST_MARKERS], (unsigned long long)markers);
         atomicAdd((unsigned long long *)&stats[ST_ALG_BYTES_LARGE], (unsigned long long)(4ull * markers));
@@ -1001,27 +1004,28 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(DevState d, const Work *_
     }
 }
 
-hipError_t rebalance_large(const DevState &d, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st) {
-    k_large_prep<<<1, 1024, 0, st>>>(d, b.work[CLASS_LARGE], b.status, b.lchunk, b.lmbase, b.lhdr);
-    k_large_push<<<grid, LG_TPB, 0, st>>>(d, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
+hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st) {
+    k_large_prep<<<1, 1024, 0, st>>>(g, b.work[CLASS_LARGE], b.status, b.lchunk, b.lmbase, b.lhdr);
+    k_large_push<<<grid, LG_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
                                           merged);
-    k_large_pull<<<grid, LG_TPB, 0, st>>>(d, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, merged, b.stats);
+    k_large_pull<<<grid, LG_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, merged);
     return hipGetLastError();
 }
 
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);
 
-// full push of all live keys and all effective updates (root / resize)
-hipError_t rebuild_push(const DevState &src, u64 n_eff, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
+// full push of all live keys of one PMA and its effective updates, the
+// compacted range [u0, u1) (root rebalance / resize)
+hipError_t rebuild_push(const DevState &src, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
                         u32 *scan_scratch, hipStream_t st) {
     u64 nseg = nseg_of(src);
     u64 W = nseg << src.log2S;
     (void)device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
-    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, 0, (u32)n_eff, merged, m);
-    if (n_eff)
-        k_push_ins<<<grid_for(n_eff, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, 0, (u32)n_eff,
-                                                                merged, m);
+    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, u0, u1, merged, m);
+    if (u1 > u0)
+        k_push_ins<<<grid_for(u1 - u0, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, u0, u1, merged,
+                                                                  m);
     return hipGetLastError();
 }
 
@@ -1049,13 +1053,13 @@ hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStre
 static constexpr u32 PYR_CHL = 11;
 static constexpr u32 PYR_CH = 1u << PYR_CHL;
 
-__global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) {
+__device__ void pyr_chunk(const DevState &d, u64 chunk) {
     __shared__ u32 v0[PYR_CH];
     __shared__ u32 v1[PYR_CH / 2];
     const u64 nseg = nseg_of(d);
     const u32 levels = d.H < PYR_CHL ? d.H : PYR_CHL;
     const u32 n = (u32)(nseg < PYR_CH ? nseg : PYR_CH);
-    const u64 c0 = (u64)blockIdx.x * n;
+    const u64 c0 = chunk * n;
     for (u32 i = threadIdx.x; i < n; i += 256) v0[i] = d.cnt[c0 + i];
     __syncthreads();
     u32 *in = v0, *tmp = v1;
@@ -1072,6 +1076,22 @@ __global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) { pyr_chunk(d, blockIdx.x); }
+
+// the chunks of every PMA of the run that took window rebalances
+__global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
+    u32 lo = 0, hi = g.npma;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (g.tab[mid].pyr_blk0 <= blockIdx.x) lo = mid; else hi = mid;
+    }
+    const PmaStat ps = g.pstat[lo];
+    if (ps.u1 == ps.u0 || (ps.flags & (FLAG_ROOT | FLAG_RESIZE))) return;
+    const DevState d = g.tab[lo].d;
+    if (d.H == 0) return;
+    pyr_chunk(d, blockIdx.x - g.tab[lo].pyr_blk0);
+}
+
 __device__ void pyr_top_levels(const DevState &d) {
     const u64 nseg = nseg_of(d);
     for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
@@ -1085,15 +1105,34 @@ __device__ void pyr_top_levels(const DevState &d) {
 
 __global__ __launch_bounds__(1024) void k_pyr_top(DevState d) { pyr_top_levels(d); }
 
-__global__ __launch_bounds__(1024) void k_batch_begin(Status *s, DevState d) {
-    if (threadIdx.x == 0) {
-        s->flags = 0;
-        s->n_eff = 0;
-        s->delta = 0;
-        for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
+__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            s->flags = 0;
+            s->n_eff = 0;
+            s->delta = 0;
+            for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
+        }
+        if (threadIdx.x < in.n) segs[threadIdx.x] = in.s[threadIdx.x];
     }
+    if (threadIdx.x == 0) {
+        PmaStat z;
+        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0;
+        g.pstat[blockIdx.x] = z;
+    }
+    const DevState d = g.tab[blockIdx.x].d;
     pyr_top_levels(d);
 }
+
+// chunk levels of the PMAs of a run (their top levels follow in the next
+// batch_begin); nblocks = the pyr_blk0 prefix total
+hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st) {
+    if (nblocks == 0) return hipSuccess;
+    k_pyr_chunk_group<<<nblocks, 256, 0, st>>>(g);
+    return hipGetLastError();
+}
+
+u32 pyramid_blocks(u32 H) { return H == 0 ? 0 : (H <= PYR_CHL ? 1 : 1u << (H - PYR_CHL)); }
 
 // full: also the levels above the chunks (otherwise the next batch_begin
 // computes them before anything reads them)

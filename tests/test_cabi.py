@@ -54,6 +54,11 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.lpma_edge_query(None, None, 0, None, None) == 1
     assert lib.lpma_get_stats(None, None) == 1
     assert lib.lpma_set_profiling(None, 1) == 1
+    assert lib.lpma_group_create(None, None, 0) == 1
+    assert lib.lpma_group_destroy(None) == 1
+    assert lib.lpma_group_update(None, None, None, None, None, None) == 1
+    assert lib.lpma_group_get_stats(None, None) == 1
+    assert lib.lpma_group_partitions(None) == 0
 
 
 def test_mt19937_64_known_answers():
