@@ -10,6 +10,7 @@ batches of B.  A step is one update batch; the stream is cyclic (deleted
 edges are re-inserted) when more steps are asked for than it holds.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py --workload batched [--steps K] [--warmup W]
 
 Prints one JSON line.  `value` is edge updates per second over the K timed
 batches (inputs resident in HBM, wall clock between device synchronisations).
@@ -18,6 +19,16 @@ launch / average launch time, from HIP events the library records around
 each launch on the update stream in a second, profiled pass).
 `cpu_baseline` times the CPU oracle (sorted merge + row offsets, pthreads)
 on a bounded sample of the same batches.
+
+`--workload batched` measures the batched form instead: 64 independent R-MAT
+graphs (edge counts log-uniform in [1e4, 1e7], vertices = edges/16 rounded
+to a power of two, seeds 3+i), 100 rounds in which every graph gets one
+batch (lognormal size, mu = ln 5000, sigma = 1.5, cap 1e6, 70% inserts /
+30% deletes).  A step is one round.  The same rounds run through one group
+update per round (the batched form) and through one update call per graph
+per round (the plain form), from identical bulk-loaded states; `value` is
+the batched form's edge updates/s, and both forms' round latencies are
+reported.
 """
 import argparse
 import json
@@ -160,11 +171,134 @@ def measured_traffic(kernel, batch):
         return None
 
 
+def _ptr_arrays(tensors_per_graph, r):
+    """ctypes (pointer, count) arrays of round r over all graphs."""
+    import ctypes
+    n = len(tensors_per_graph)
+    p = (ctypes.c_void_p * n)()
+    c = (ctypes.c_uint64 * n)()
+    for i in range(n):
+        t = tensors_per_graph[i][r]
+        c[i] = t.numel()
+        p[i] = t.data_ptr() if t.numel() else None
+    return p, c
+
+
+def build_fleet(w, seg, lower, upper):
+    import torch
+    import lpma_amd as L
+    fleet = []
+    for i in range(w.n):
+        p = L.PMA(w.V[i], segment_slots=seg, density_lower=lower, density_upper=upper)
+        p.bulk_load(L.to_device(w.initial[i]))
+        fleet.append(p)
+    torch.cuda.synchronize()
+    return fleet
+
+
+def run_rounds(w, fleet, group, ins_d, del_d, rounds, stream):
+    """Runs `rounds` (indices) through the group (batched form) or one
+    update call per graph (plain form); returns per-round event times (ms)
+    and the wall time."""
+    import ctypes
+    import torch
+    import lpma_amd as L
+    lib = L.lib()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    args = [(_ptr_arrays(ins_d, r), _ptr_arrays(del_d, r)) for r in rounds]
+    hs = [p._h for p in fleet]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in rounds]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j, r in enumerate(rounds):
+        (pi, ni), (pd, nd) = args[j]
+        ev[j][0].record(stream)
+        if group is not None:
+            rc = lib.lpma_group_update(group._h, pi, ni, pd, nd, sp)
+            if rc:
+                raise L.LpmaError(rc, "group_update")
+        else:
+            for i, h in enumerate(hs):
+                rc = lib.lpma_update_mixed(h, pi[i], ni[i], pd[i], nd[i], sp)
+                if rc:
+                    raise L.LpmaError(rc, "update_mixed")
+        ev[j][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return np.array([a.elapsed_time(b) for a, b in ev]), wall
+
+
+def run_batched(args):
+    import torch
+    import lpma_amd as L
+    from lpma_amd.workloads import BatchedWorkload
+    L.lib()
+    total = args.warmup + args.steps
+    t = time.time()
+    w = BatchedWorkload(rounds=total)
+    log(f"batched workload: {w.n} graphs, {sum(w.edges)} initial edges, {w.total_updates} updates in {total} "
+        f"rounds, generated in {time.time() - t:.1f}s")
+    dev = torch.device("cuda", 0)
+    ins_d = [[L.to_device(b, dev) for b in w.ins[i]] for i in range(w.n)]
+    del_d = [[L.to_device(b, dev) for b in w.dels[i]] for i in range(w.n)]
+    stream = torch.cuda.current_stream()
+    seg, lower, upper = args.segment_slots, 0.42, 1.0
+    warm = list(range(args.warmup))
+    timed = list(range(args.warmup, total))
+    n_timed = int(w.sizes[args.warmup:].sum())
+
+    res = {}
+    finals = {}
+    for form in ("plain", "batched"):
+        t = time.time()
+        fleet = build_fleet(w, seg, lower, upper)
+        log(f"{form}: fleet bulk-loaded in {time.time() - t:.1f}s")
+        group = L.Group(fleet) if form == "batched" else None
+        run_rounds(w, fleet, group, ins_d, del_d, warm, stream)
+        lat, wall = run_rounds(w, fleet, group, ins_d, del_d, timed, stream)
+        res[form] = dict(value=n_timed / wall, wall=wall, p50=float(np.percentile(lat, 50)),
+                         p99=float(np.percentile(lat, 99)), lat=lat)
+        finals[form] = [(int(p.view().live_keys), int(p.view().tree_height)) for p in fleet]
+        assert all(p.check() == 0 for p in fleet)
+        log(f"{form}: {res[form]['value']:.3e} updates/s, round p50 {res[form]['p50']:.3f} ms "
+            f"p99 {res[form]['p99']:.3f} ms")
+        if group is not None:
+            group.close()
+        for p in fleet:
+            p.close()
+        torch.cuda.synchronize()
+    assert finals["plain"] == finals["batched"], "batched and plain forms ended in different states"
+    b, p = res["batched"], res["plain"]
+    out = {
+        "metric": "edge updates/sec, batched form (64 PMAs, one group update per round) vs plain form",
+        "value": b["value"],
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded R-MAT generators)",
+        "config": {"workload": "batched_64_rmat_graphs_lognormal_batches", "graphs": w.n,
+                   "initial_edges": int(sum(w.edges)), "updates_timed": n_timed, "segment_slots": seg,
+                   "density": f"lower {lower} / upper {upper}", "parallelism": "single GPU"},
+        "plain_value": p["value"],
+        "speedup_vs_plain": b["value"] / p["value"],
+        "round_p50_ms": b["p50"], "round_p99_ms": b["p99"],
+        "plain_round_p50_ms": p["p50"], "plain_round_p99_ms": p["p99"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=160)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", choices=("headline", "batched"), default="headline")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--batch", type=int, default=100000)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edges", type=int, default=1 << 24)
@@ -179,6 +313,12 @@ def main():
               "sharded by key range across GPUs only by design (no multi-GPU run is part of this build).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        args.steps = 95 if args.steps is None else args.steps
+        args.warmup = 5 if args.warmup is None else args.warmup
+        return run_batched(args)
+    args.steps = 160 if args.steps is None else args.steps
+    args.warmup = 4 if args.warmup is None else args.warmup
 
     import torch
     import lpma_amd as L

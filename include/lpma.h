@@ -241,6 +241,10 @@ int lpma_gen_uniform(uint32_t V, uint64_t n, uint64_t seed, uint64_t *out);
  * probabilities (a, b, c, 1-a-b-c), no self loops, generation order. */
 int lpma_gen_rmat(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *out);
 
+/* R-MAT samples with replacement (repeats kept, self loops rejected): the
+ * insert stream of the batched and served workloads. */
+int lpma_gen_rmat_samples(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *out);
+
 /* Seeded Fisher-Yates shuffle of n keys in place. */
 int lpma_shuffle(uint64_t *keys, uint64_t n, uint64_t seed);
 

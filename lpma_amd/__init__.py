@@ -123,6 +123,7 @@ SIGNATURES = [
     ("lpma_group_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_gen_uniform", _int, [_u32, _u64, _u64, _vp]),
     ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
+    ("lpma_gen_rmat_samples", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
     ("lpma_shuffle", _int, [_vp, _u64, _u64]),
     ("lpma_gen_absent", _int, [_u32, _vp, _u64, _u64, _u64, _vp]),
     ("lpma_read_edgelist", _int, [ctypes.c_char_p, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _u64, _vp]),
@@ -167,6 +168,13 @@ def gen_uniform(V, n, seed):
 def gen_rmat(scale, n, a=0.57, b=0.19, c=0.19, seed=2):
     out = _np_u64(n)
     _check(lib().lpma_gen_rmat(scale, n, a, b, c, seed, out.ctypes.data), "gen_rmat")
+    return out
+
+
+def gen_rmat_samples(scale, n, a=0.57, b=0.19, c=0.19, seed=2):
+    """R-MAT edges drawn with replacement (repeats kept, no self loops)."""
+    out = _np_u64(n)
+    _check(lib().lpma_gen_rmat_samples(scale, n, a, b, c, seed, out.ctypes.data), "gen_rmat_samples")
     return out
 
 

@@ -99,6 +99,23 @@ struct KeySet {
     }
 };
 
+// one R-MAT edge (Chakrabarti, Zhan & Faloutsos 2004): per level, the
+// quadrant (a | b / c | d) picks one bit of src and of dst
+u64 rmat_draw(Mt64 &g, u32 scale, double a, double ab, double abc) {
+    u64 src = 0, dst = 0;
+    for (u32 l = 0; l < scale; ++l) {
+        double u = g.unit();
+        u64 bs, bd;
+        if (u < a) { bs = 0; bd = 0; }
+        else if (u < ab) { bs = 0; bd = 1; }
+        else if (u < abc) { bs = 1; bd = 0; }
+        else { bs = 1; bd = 1; }
+        src = (src << 1) | bs;
+        dst = (dst << 1) | bd;
+    }
+    return (src << 32) | dst;
+}
+
 bool sorted_contains(const u64 *a, u64 n, u64 x) {
     u64 lo = 0, hi = n;
     while (lo < hi) {
@@ -143,20 +160,24 @@ extern "C" int lpma_gen_rmat(uint32_t scale, uint64_t n, double a, double b, dou
     const double ab = a + b, abc = a + b + c;
     u64 k = 0;
     while (k < n) {
-        u64 src = 0, dst = 0;
-        for (u32 l = 0; l < scale; ++l) {
-            double u = g.unit();
-            u64 bs, bd;
-            if (u < a) { bs = 0; bd = 0; }
-            else if (u < ab) { bs = 0; bd = 1; }
-            else if (u < abc) { bs = 1; bd = 0; }
-            else { bs = 1; bd = 1; }
-            src = (src << 1) | bs;
-            dst = (dst << 1) | bd;
-        }
-        if (src == dst) continue;
-        u64 key = (src << 32) | dst;
+        u64 key = rmat_draw(g, scale, a, ab, abc);
+        if ((key >> 32) == (key & 0xFFFFFFFFull)) continue;
         if (set.insert(key)) out[k++] = key;
+    }
+    return LPMA_OK;
+}
+
+extern "C" int lpma_gen_rmat_samples(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed,
+                                     uint64_t *out) {
+    if ((!out && n) || scale < 1 || scale > 31) return LPMA_ERR_INVALID_ARGUMENT;
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return LPMA_ERR_INVALID_ARGUMENT;
+    Mt64 g(seed);
+    const double ab = a + b, abc = a + b + c;
+    u64 k = 0;
+    while (k < n) {
+        u64 key = rmat_draw(g, scale, a, ab, abc);
+        if ((key >> 32) == (key & 0xFFFFFFFFull)) continue;
+        out[k++] = key;
     }
     return LPMA_OK;
 }

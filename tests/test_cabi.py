@@ -97,6 +97,27 @@ def test_rmat_generator_quadrants():
     assert np.array_equal(a, L.gen_rmat(scale, 100000, 0.57, 0.19, 0.19, 2))
 
 
+def test_rmat_samples_are_the_draws_before_dedup():
+    """gen_rmat keeps the first occurrence of each sampled edge, so it must
+    equal the first-occurrence order of gen_rmat_samples from the same seed;
+    samples keep repeats and follow the quadrant probabilities."""
+    scale = 8
+    s = L.gen_rmat_samples(scale, 60000, 0.57, 0.19, 0.19, 5)
+    assert np.array_equal(s, L.gen_rmat_samples(scale, 60000, 0.57, 0.19, 0.19, 5))
+    src, dst = s >> U32, s & np.uint64(0xFFFFFFFF)
+    assert (src < (1 << scale)).all() and (dst < (1 << scale)).all() and (src != dst).all()
+    uniq, first = np.unique(s, return_index=True)
+    assert uniq.size < s.size  # repeats kept
+    order = s[np.sort(first)]
+    d = L.gen_rmat(scale, order.size, 0.57, 0.19, 0.19, 5)
+    assert np.array_equal(d, order)
+    top_s = (src >> np.uint64(scale - 1)) & np.uint64(1)
+    top_d = (dst >> np.uint64(scale - 1)) & np.uint64(1)
+    frac = [np.mean((top_s == i) & (top_d == j)) for i in (0, 1) for j in (0, 1)]
+    for f, p in zip(frac, (0.57, 0.19, 0.19, 0.05)):
+        assert abs(f - p) < 0.03, frac
+
+
 def test_shuffle_and_absent():
     a = L.gen_uniform(1 << 8, 5000, 3)
     b = L.shuffle(a.copy(), 9)
