@@ -249,11 +249,16 @@ def run_batched(args):
 
     res = {}
     finals = {}
-    for form in ("plain", "batched"):
+    forms = ("plain", "batched", "batched_graph") if args.forms == "both" else (args.forms,)
+    for form in forms:
         t = time.time()
         fleet = build_fleet(w, seg, lower, upper)
         log(f"{form}: fleet bulk-loaded in {time.time() - t:.1f}s")
-        group = L.Group(fleet) if form == "batched" else None
+        group = L.Group(fleet) if form != "plain" else None
+        if group is not None:
+            group.set_graph(form == "batched_graph" or (args.forms == "batched" and bool(args.graph)))
+            # size the buffers (and capture the round graphs) before timing
+            group.reserve(int(w.sizes.sum(axis=1).max()))
         run_rounds(w, fleet, group, ins_d, del_d, warm, stream)
         lat, wall = run_rounds(w, fleet, group, ins_d, del_d, timed, stream)
         res[form] = dict(value=n_timed / wall, wall=wall, p50=float(np.percentile(lat, 50)),
@@ -267,8 +272,13 @@ def run_batched(args):
         for p in fleet:
             p.close()
         torch.cuda.synchronize()
-    assert finals["plain"] == finals["batched"], "batched and plain forms ended in different states"
-    b, p = res["batched"], res["plain"]
+    if len(forms) == 1:
+        r = res[forms[0]]
+        print(json.dumps({"form": forms[0], "graph": args.graph, "value": r["value"], "p50": r["p50"],
+                          "p99": r["p99"]}))
+        return
+    assert finals["plain"] == finals["batched"] == finals["batched_graph"], "the forms ended in different states"
+    b, p, bg = res["batched"], res["plain"], res["batched_graph"]
     out = {
         "metric": "edge updates/sec, batched form (64 PMAs, one group update per round) vs plain form",
         "value": b["value"],
@@ -284,11 +294,13 @@ def run_batched(args):
         "data": "synthetic (seeded R-MAT generators)",
         "config": {"workload": "batched_64_rmat_graphs_lognormal_batches", "graphs": w.n,
                    "initial_edges": int(sum(w.edges)), "updates_timed": n_timed, "segment_slots": seg,
-                   "density": f"lower {lower} / upper {upper}", "parallelism": "single GPU"},
+                   "density": f"lower {lower} / upper {upper}", "rounds": "stream-ordered, one sync per round",
+                   "parallelism": "single GPU"},
         "plain_value": p["value"],
         "speedup_vs_plain": b["value"] / p["value"],
         "round_p50_ms": b["p50"], "round_p99_ms": b["p99"],
         "plain_round_p50_ms": p["p50"], "plain_round_p99_ms": p["p99"],
+        "graph_rounds_value": bg["value"], "graph_round_p50_ms": bg["p50"], "graph_round_p99_ms": bg["p99"],
     }
     print(json.dumps(out))
 
@@ -297,6 +309,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=("headline", "batched"), default="headline")
+    ap.add_argument("--graph", type=int, default=0, help="--forms batched: replay rounds as hipGraphs")
+    ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--batch", type=int, default=100000)

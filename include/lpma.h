@@ -102,6 +102,9 @@ typedef struct lpma_profile {
     uint64_t launches_wave;
     uint64_t launches_block;
     uint64_t launches_seg;
+    double graph_ms;          /* whole rounds replayed as one graph */
+    uint64_t graph_launches;
+    uint64_t graph_captures;
 } lpma_profile_t;
 
 /* Fills the reference defaults (32 slots, 0.42 / 1.0). */
@@ -190,6 +193,19 @@ int lpma_reset_stats(lpma_t *h);
 int lpma_set_profiling(lpma_t *h, int on);
 int lpma_get_profile(lpma_t *h, lpma_profile_t *out);
 
+/* Graph replay of update rounds (off by default for a handle): each round
+ * is one captured hipGraph with grids sized for a batch capacity and a
+ * single host synchronisation at its end, instead of the stream-ordered
+ * kernel sequence that synchronises once after the window election.  The
+ * graph is captured again when the batch capacity, a buffer or a member's
+ * geometry changes.  Results are identical either way. */
+int lpma_set_graph(lpma_t *h, int on);
+
+/* Sizes the update buffers for batches of up to max_batch keys and, in graph
+ * mode, captures the round graphs of every batch capacity up to it now, so
+ * later rounds of that size capture nothing. */
+int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream);
+
 /* Device pool shared by all handles on a device: cap on bytes held from
  * hipMalloc (0 = unlimited), and release of cached free blocks. */
 int lpma_set_pool_budget(uint64_t bytes);
@@ -226,6 +242,11 @@ uint32_t lpma_group_partitions(const lpma_group_t *g);
  * own counters cover only their lpma_update calls). */
 int lpma_group_get_stats(lpma_group_t *g, lpma_stats_t *out);
 int lpma_group_set_profiling(lpma_group_t *g, int on);
+/* Graph replay of the group's rounds (see lpma_set_graph); off by default. */
+int lpma_group_set_graph(lpma_group_t *g, int on);
+/* lpma_reserve for every partition of the group (max_batch = the largest
+ * total batch of one round). */
+int lpma_group_reserve(lpma_group_t *g, uint64_t max_batch, void *stream);
 int lpma_group_get_profile(lpma_group_t *g, lpma_profile_t *out);
 
 /* ---------------------------------------------------------------------------

@@ -77,7 +77,8 @@ class Profile(ctypes.Structure):
                 ("rebalance_wave_ms", ctypes.c_double), ("rebalance_block_ms", ctypes.c_double),
                 ("large_ms", ctypes.c_double), ("rebuild_ms", ctypes.c_double), ("pyramid_ms", ctypes.c_double),
                 ("rebalance_seg_ms", ctypes.c_double), ("launches_wave", ctypes.c_uint64),
-                ("launches_block", ctypes.c_uint64), ("launches_seg", ctypes.c_uint64)]
+                ("launches_block", ctypes.c_uint64), ("launches_seg", ctypes.c_uint64), ("graph_ms", ctypes.c_double),
+                ("graph_launches", ctypes.c_uint64), ("graph_captures", ctypes.c_uint64)]
 
 
 _vp = ctypes.c_void_p
@@ -109,6 +110,8 @@ SIGNATURES = [
     ("lpma_reset_stats", _int, [_vp]),
     ("lpma_set_profiling", _int, [_vp, _int]),
     ("lpma_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
+    ("lpma_set_graph", _int, [_vp, _int]),
+    ("lpma_reserve", _int, [_vp, _u64, _vp]),
     ("lpma_set_pool_budget", _int, [_u64]),
     ("lpma_pool_trim", _int, []),
     ("lpma_pool_reserved_bytes", _u64, []),
@@ -120,6 +123,8 @@ SIGNATURES = [
     ("lpma_group_partitions", _u32, [_vp]),
     ("lpma_group_get_stats", _int, [_vp, ctypes.POINTER(Stats)]),
     ("lpma_group_set_profiling", _int, [_vp, _int]),
+    ("lpma_group_set_graph", _int, [_vp, _int]),
+    ("lpma_group_reserve", _int, [_vp, _u64, _vp]),
     ("lpma_group_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_gen_uniform", _int, [_u32, _u64, _u64, _vp]),
     ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
@@ -413,6 +418,12 @@ class PMA:
         _check(lib().lpma_get_profile(self._h, ctypes.byref(p)), "profile")
         return p
 
+    def set_graph(self, on=True):
+        _check(lib().lpma_set_graph(self._h, 1 if on else 0), "set_graph")
+
+    def reserve(self, max_batch):
+        _check(lib().lpma_reserve(self._h, int(max_batch), _stream_ptr(self.stream)), "reserve")
+
 
 class Group:
     """Independent PMAs updated by one pipeline run per round (the batched
@@ -470,6 +481,12 @@ class Group:
 
     def set_profiling(self, on=True):
         _check(lib().lpma_group_set_profiling(self._h, 1 if on else 0), "group_set_profiling")
+
+    def set_graph(self, on=True):
+        _check(lib().lpma_group_set_graph(self._h, 1 if on else 0), "group_set_graph")
+
+    def reserve(self, max_batch):
+        _check(lib().lpma_group_reserve(self._h, int(max_batch), _stream_ptr(self.stream)), "group_reserve")
 
     def profile(self):
         p = Profile()

@@ -39,7 +39,6 @@ static constexpr u64 RK_OP = 1ull << 63;
 static constexpr u32 FLAG_INVALID_KEY = 1u;   // src or dst >= V
 static constexpr u32 FLAG_ROOT = 2u;          // a run chose the root window
 static constexpr u32 FLAG_RESIZE = 4u;        // the root window does not fit
-static constexpr u32 FLAG_LARGE = 8u;         // a chosen window exceeds the LDS path
 
 // rebalance work classes
 static constexpr int CLASS_WAVE = 0;   // one wave64 per window
@@ -114,8 +113,8 @@ struct Density {
 struct PmaEntry {
     DevState d;
     Density dn;
-    u32 pyr_blk0;  // first workgroup of this PMA in the pyramid launch
-    u32 pad_;
+    u32 pyr_blk0;  // first pyramid chunk of this PMA in the group's chunk list
+    u32 pyr_nblk;  // its chunks
 };
 
 // Per-PMA outcome of a run, read back by the host with the Status.
@@ -123,6 +122,7 @@ struct PmaStat {
     u32 flags;     // FLAG_ROOT / FLAG_RESIZE raised by its runs
     u32 u0, u1;    // its effective updates: compacted range [u0, u1)
     i32 pd0, pd1;  // the update prefix at u0 and u1 (delta = pd1 - pd0)
+    u32 levels;    // bit l: some run claimed a window of level l >= 1
 };
 
 // One batch source: n keys of one operation for one PMA, placed at
@@ -137,6 +137,15 @@ struct BatchSeg {
     u32 vbits;
 };
 
+// Per-round values a graph replay reads from device memory (the host
+// rewrites them before every launch).
+struct RoundHdr {
+    u64 n;        // batch elements
+    u64 *merged;  // scratch of the multi-workgroup windows
+    u32 epoch;
+    u32 pad_;
+};
+
 // The PMAs one pipeline run updates.  Sort key of a batch element:
 // pid << pshift | (src << vbits | dst) << 1 | op, pshift = 2 * max vbits + 1.
 struct Group {
@@ -147,7 +156,10 @@ struct Group {
     u32 pshift;
     u32 epoch;    // tags this run's window claims
     u32 log2S;    // segment size, the same for every PMA of the run
+    const RoundHdr *rh;  // graph replays: n and epoch come from here
 };
+
+__device__ __forceinline__ u32 group_epoch(const Group &g) { return g.rh ? g.rh->epoch : g.epoch; }
 
 __host__ __device__ __forceinline__ u64 nseg_of(const DevState &d) { return (u64)1 << d.H; }
 

@@ -8,9 +8,8 @@ namespace pma {
 u64 scan_scratch_words(u64 n);
 hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
 u64 sort_scratch_words(u64 n);
-u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st);
-u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, u32 pshift, u32 lo_bit, u32 hi_bit, u64 *k0, u64 *k1,
-                   u32 *scratch, u32 *flags, hipStream_t st);
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 lo_bit, u32 hi_bit,
+                   u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
 
 // update.hip
 struct BatchBufs {
@@ -38,6 +37,8 @@ struct InlineSegs {
 u32 locate_blocks(u64 n);
 hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, hipStream_t st);
 hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
+// fixed grids for graph replays (work counts read on the device)
+hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st);
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st);

@@ -131,7 +131,7 @@ struct DBuf {
 // one for its own updates (a group of one PMA); a group owns one per
 // partition of its members.
 
-enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR, PH_SEG, PH_NUM };
+enum Phase { PH_SORT, PH_LOCATE, PH_ELECT, PH_WAVE, PH_BLOCK, PH_LARGE, PH_REBUILD, PH_PYR, PH_SEG, PH_GRAPH, PH_NUM };
 
 struct Pipeline {
     int device = 0;
@@ -157,6 +157,18 @@ struct Pipeline {
 
     u64 *d_stats = nullptr;
     u64 rounds = 0, effective = 0, rebuilds = 0, rebuild_slots = 0;
+
+    // graph replays: one captured graph per round shape
+    bool graph_on = false;
+    hipStream_t cap_stream = nullptr;
+    struct GraphEntry {
+        hipGraphExec_t exec;
+        std::vector<u64> key;  // everything the captured graph has baked in
+    };
+    std::map<u64, GraphEntry> graphs;  // by batch capacity
+    DBuf<RoundHdr> rhdr;
+    RoundHdr *h_rhdr = nullptr;
+    u64 graph_captures = 0, graph_launches = 0;
 
     bool profiling = false;
     struct Ev {
@@ -322,7 +334,13 @@ void pipeline_free(Pipeline &pl) {
     release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
     release_on(d, pl.lchunk); release_on(d, pl.lhdr); release_on(d, pl.pd); release_on(d, pl.part);
     for (int c = 0; c < NUM_CLASSES; ++c) release_on(d, pl.work[c]);
-    release_on(d, pl.tab); release_on(d, pl.segs); release_on(d, pl.out);
+    release_on(d, pl.tab); release_on(d, pl.segs); release_on(d, pl.out); release_on(d, pl.rhdr);
+    for (auto &kv : pl.graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    pl.graphs.clear();
+    if (pl.cap_stream) (void)hipStreamDestroy(pl.cap_stream);
+    pl.cap_stream = nullptr;
+    if (pl.h_rhdr) (void)hipHostFree(pl.h_rhdr);
+    pl.h_rhdr = nullptr;
     if (pl.h_tab) (void)hipHostFree(pl.h_tab);
     if (pl.h_segs) (void)hipHostFree(pl.h_segs);
     if (pl.h_out) (void)hipHostFree(pl.h_out);
@@ -537,6 +555,144 @@ u32 next_epoch() {
     return e;
 }
 
+// Epoch of a new round; clears a member's claims after the counter wrapped.
+int begin_epoch(const Member *mem, u32 npma, u32 &epoch, hipStream_t st) {
+    epoch = next_epoch();
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        if (epoch <= h->last_epoch) HIPCHECK(fill_u64(h->owner.p, 2 * (1ull << h->st.H), 0, st));
+        h->last_epoch = epoch;
+    }
+    return LPMA_OK;
+}
+
+// Device table of the members (uploaded only when it changed); returns the
+// workgroups of the group pyramid launch.
+int upload_table(Pipeline &pl, const Member *mem, u32 npma, u32 &pyr_blocks, hipStream_t st) {
+    pyr_blocks = 0;
+    for (u32 p = 0; p < npma; ++p) {
+        PmaEntry &e = pl.h_tab[p];
+        memset(&e, 0, sizeof(e));
+        e.d = mem[p].h->st;
+        e.dn = mem[p].h->dn;
+        e.pyr_blk0 = pyr_blocks;
+        e.pyr_nblk = pyramid_blocks(e.d.H);
+        pyr_blocks += e.pyr_nblk;
+    }
+    if (pl.tab_shadow_at != pl.tab.p || pl.tab_shadow.size() != npma ||
+        memcmp(pl.tab_shadow.data(), pl.h_tab, npma * sizeof(PmaEntry)) != 0) {
+        HIPCHECK(hipMemcpyAsync(pl.tab.p, pl.h_tab, npma * sizeof(PmaEntry), hipMemcpyHostToDevice, st));
+        pl.tab_shadow.assign(pl.h_tab, pl.h_tab + npma);
+        pl.tab_shadow_at = pl.tab.p;
+    }
+    return LPMA_OK;
+}
+
+// After the round's status is on the host: root rebuilds / resizes of the
+// flagged members (checked before anything changes), then the bookkeeping.
+// `windows_done` tells whether the window rebalances and the pyramid have
+// already been queued (graph replays) or are still to be launched here.
+int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 pyr_blocks, bool windows_done,
+                 hipStream_t st) {
+    const Status s = *(const Status *)pl.h_out;
+    const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
+    if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
+    const u64 n_eff = s.n_eff;
+    if (n_eff == 0) {
+        for (u32 p = 0; p < npma; ++p) mem[p].h->version++;
+        return LPMA_OK;
+    }
+
+    // new live counts; every root rebuild / resize is checked before any
+    // kernel changes a PMA, so a failing round changes nothing
+    std::vector<u64> Mnew(npma);
+    u64 m_sum = 0, m_rebuild = 0;
+    bool any_rebuild = false;
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        Mnew[p] = (u64)((i64)h->live + (ps[p].pd1 - ps[p].pd0));
+        m_sum += Mnew[p];
+        if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) {
+            any_rebuild = true;
+            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, Mnew[p]) : h->st.H;
+            if (!height_ok(h, newH)) return LPMA_ERR_TOO_LARGE;
+            m_rebuild = std::max(m_rebuild, Mnew[p]);
+        }
+    }
+    BatchBufs bb = batch_bufs(pl);
+
+    if (!windows_done) {
+        if (s.nwork[CLASS_SEG]) {
+            PhaseTimer t(pl, st, PH_SEG);
+            HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st));
+            pl.launches_seg++;
+        }
+        if (s.nwork[CLASS_WAVE]) {
+            PhaseTimer t(pl, st, PH_WAVE);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
+            pl.launches_wave++;
+        }
+        if (s.nwork[CLASS_BLOCK]) {
+            PhaseTimer t(pl, st, PH_BLOCK);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
+            pl.launches_block++;
+        }
+        if (s.nwork[CLASS_LARGE]) {
+            // the windows' merged slices add up to at most the members' live keys
+            CHECKED(ensure(pl, pl.merged, m_sum + 1));
+            CHECKED(ensure(pl, pl.lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
+            CHECKED(ensure(pl, pl.lmbase, s.nwork[CLASS_LARGE]));
+            CHECKED(ensure(pl, pl.lhdr, 4));
+            bb = batch_bufs(pl);
+            PhaseTimer t(pl, st, PH_LARGE);
+            HIPCHECK(rebalance_large(g, bb, pl.merged.p, 1024, st));
+        }
+        {
+            PhaseTimer t(pl, st, PH_PYR);
+            HIPCHECK(group_pyramid(g, pyr_blocks, st));
+        }
+    }
+
+    if (any_rebuild) {
+        // root rebalance or resize: every live key and every update of the
+        // PMA is merged into scratch and spread again (rebuild_rpma,
+        // rpma.cuh:1417-1472)
+        PhaseTimer t(pl, st, PH_REBUILD);
+        CHECKED(ensure(pl, pl.merged, m_rebuild + 1));
+        bb = batch_bufs(pl);
+        for (u32 p = 0; p < npma; ++p) {
+            if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
+            lpma *h = mem[p].h;
+            const u64 M = Mnew[p];
+            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
+            u64 nseg = 1ull << h->st.H;
+            CHECKED(ensure(pl, pl.gpre, nseg + 1));
+            CHECKED(ensure(pl, pl.scan_scr, scan_scratch_words(nseg) + 8));
+            HIPCHECK(rebuild_push(h->st, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.scan_scr.p, st));
+            if (newH != h->st.H) {
+                CHECKED(respread_into_new(h, newH, pl.merged.p, M, st));
+            } else {
+                HIPCHECK(rebuild_spread(h->st, pl.merged.p, M, st));
+                HIPCHECK(rebuild_pyramid(h->st, true, st));
+            }
+            h->rebuilds++;
+            h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
+            pl.rebuilds++;
+            pl.rebuild_slots += (u64)1 << (h->st.H + h->log2S);
+        }
+    }
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        h->live = Mnew[p];
+        h->effective += ps[p].u1 - ps[p].u0;
+        h->version++;
+    }
+    pl.effective += n_eff;
+    return LPMA_OK;
+}
+
+int run_pipeline_graph(Pipeline &pl, const Member *mem, u32 npma, u64 n, u32 pshift, u32 hi_bit, hipStream_t st);
+
 // One update round over the members (all on pl's device, same segment size,
 // ids fitting above the sort keys): sort + dedup + locate + density check +
 // election over every member at once, one host synchronisation, then the
@@ -565,32 +721,12 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     const u32 pshift = 2 * vmax + 1;
     const u32 hi_bit = pshift + bits_for(npma);
     if (hi_bit > 64) return LPMA_ERR_INVALID_ARGUMENT;
+    if (pl.graph_on) return run_pipeline_graph(pl, mem, npma, n, pshift, hi_bit, st);
     CHECKED(ensure_batch(pl, n, npma, nsegs, std::min<u64>(n, nseg_total)));
 
-    // claims are tagged with the epoch: after a wrap, clear a PMA's old tags
-    const u32 epoch = next_epoch();
-    for (u32 p = 0; p < npma; ++p) {
-        lpma *h = mem[p].h;
-        if (epoch <= h->last_epoch) HIPCHECK(fill_u64(h->owner.p, 2 * (1ull << h->st.H), 0, st));
-        h->last_epoch = epoch;
-    }
-
-    // device table of the members (uploaded only when it changed)
-    u32 pyr_blocks = 0;
-    for (u32 p = 0; p < npma; ++p) {
-        PmaEntry &e = pl.h_tab[p];
-        memset(&e, 0, sizeof(e));
-        e.d = mem[p].h->st;
-        e.dn = mem[p].h->dn;
-        e.pyr_blk0 = pyr_blocks;
-        pyr_blocks += pyramid_blocks(e.d.H);
-    }
-    if (pl.tab_shadow_at != pl.tab.p || pl.tab_shadow.size() != npma ||
-        memcmp(pl.tab_shadow.data(), pl.h_tab, npma * sizeof(PmaEntry)) != 0) {
-        HIPCHECK(hipMemcpyAsync(pl.tab.p, pl.h_tab, npma * sizeof(PmaEntry), hipMemcpyHostToDevice, st));
-        pl.tab_shadow.assign(pl.h_tab, pl.h_tab + npma);
-        pl.tab_shadow_at = pl.tab.p;
-    }
+    u32 epoch, pyr_blocks;
+    CHECKED(begin_epoch(mem, npma, epoch, st));
+    CHECKED(upload_table(pl, mem, npma, pyr_blocks, st));
 
     // batch sources: each member's deletes (op 0), then its inserts (op 1)
     InlineSegs in;
@@ -631,6 +767,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.pshift = pshift;
     g.epoch = epoch;
     g.log2S = mem[0].h->log2S;
+    g.rh = nullptr;
     BatchBufs bb = batch_bufs(pl);
     // bit 0 of a packed key is the operation: sorted only when a member mixes them
     const u32 lo_bit = mixed ? 0 : 1;
@@ -638,7 +775,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     {
         PhaseTimer t(pl, st, PH_SORT);
         HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, st));
-        pk = pack_and_sort(pl.segs.p, nsegs, n, pshift, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p, pl.sort_scr.p,
+        pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p, pl.sort_scr.p,
                            &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
     }
@@ -657,96 +794,195 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     const u64 out_bytes = STATUS_BYTES + (u64)npma * sizeof(PmaStat);
     HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    const Status s = *(const Status *)pl.h_out;
-    const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
-    if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
-    const u64 n_eff = s.n_eff;
-    if (n_eff == 0) {
-        for (u32 p = 0; p < npma; ++p) mem[p].h->version++;
-        return LPMA_OK;
-    }
+    return finish_round(pl, mem, npma, g, pyr_blocks, false, st);
+}
 
-    // new live counts; every root rebuild / resize is checked before any
-    // kernel changes a PMA, so a failing round changes nothing
-    std::vector<u64> Mnew(npma);
-    u64 m_sum = 0, m_rebuild = 0;
-    bool any_rebuild = false;
-    for (u32 p = 0; p < npma; ++p) {
-        lpma *h = mem[p].h;
-        Mnew[p] = (u64)((i64)h->live + (ps[p].pd1 - ps[p].pd0));
-        m_sum += Mnew[p];
-        if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) {
-            any_rebuild = true;
-            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, Mnew[p]) : h->st.H;
-            if (!height_ok(h, newH)) return LPMA_ERR_TOO_LARGE;
-            m_rebuild = std::max(m_rebuild, Mnew[p]);
+// Graph replay of a round: the host writes the batch descriptors and the
+// round header into pinned memory and launches one captured graph (copies
+// in, sort, locate, density check, election, every rebalance class and the
+// pyramid with fixed grids, status copy out), then synchronises once at the
+// end.  Grids are sized for a batch capacity; the kernels read the batch
+// size, epoch and scratch pointer from the copied header and the members'
+// geometry from the device table, so a graph stays valid across rounds
+// until the capacity or a batch buffer changes.
+
+// batch capacity of n elements: the next of 4096 * {1, 1.25, 1.5, 1.75} * 2^k,
+// so the grids exceed the batch by at most a quarter
+u64 graph_capacity(u64 n) {
+    for (u64 b = 4096;; b <<= 1)
+        for (u64 q = 4; q < 8; ++q)
+            if (b * q / 4 >= n) return b * q / 4;
+}
+
+static constexpr u32 GRAPH_PYR_GRID = 2048;
+
+int prepare_graph_buffers(Pipeline &pl, u64 cap, u32 npma, u64 merged_keys) {
+    CHECKED(ensure_batch(pl, cap, npma, 2 * npma, cap));
+    // merged scratch of the large windows: at most every live key of the
+    // group after the round
+    CHECKED(ensure(pl, pl.merged, merged_keys + 1));
+    CHECKED(ensure(pl, pl.lchunk, cap + 1));
+    CHECKED(ensure(pl, pl.lmbase, cap));
+    CHECKED(ensure(pl, pl.lhdr, 4));
+    CHECKED(ensure(pl, pl.rhdr, 1));
+    if (!pl.h_rhdr) {
+        if (hipHostMalloc((void **)&pl.h_rhdr, sizeof(RoundHdr), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            pl.h_rhdr = nullptr;
+            return LPMA_ERR_OUT_OF_MEMORY;
         }
     }
+    if (!pl.cap_stream) HIPCHECK(hipStreamCreateWithFlags(&pl.cap_stream, hipStreamNonBlocking));
+    return LPMA_OK;
+}
 
-    if (s.nwork[CLASS_SEG]) {
-        PhaseTimer t(pl, st, PH_SEG);
-        HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st));
-        pl.launches_seg++;
+Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
+    Group g;
+    g.tab = pl.tab.p;
+    g.pstat = pstat_of(pl);
+    g.stats = pl.d_stats;
+    g.npma = npma;
+    g.pshift = pshift;
+    g.epoch = 0;
+    g.log2S = log2S;
+    g.rh = pl.rhdr.p;
+    return g;
+}
+
+// The graph of capacity `cap`, captured now unless a valid one is cached.
+int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S, hipGraphExec_t *out,
+              hipStream_t st) {
+    const u32 nsegs = 2 * npma;
+    std::vector<u64> key = {cap, npma, pshift, hi_bit, log2S,
+                            (u64)pl.pk0.p, (u64)pl.pk1.p, (u64)pl.sort_scr.p, (u64)pl.ukey.p, (u64)pl.useg.p,
+                            (u64)pl.ckey.p, (u64)pl.cseg.p, (u64)pl.pd.p, (u64)pl.rlevel.p, (u64)pl.part.p,
+                            (u64)pl.lchunk.p, (u64)pl.lmbase.p, (u64)pl.lhdr.p, (u64)pl.out.p, (u64)pl.tab.p,
+                            (u64)pl.segs.p, (u64)pl.rhdr.p, (u64)pl.d_stats, (u64)pl.h_out, (u64)pl.h_segs,
+                            (u64)pl.h_rhdr};
+    for (int c = 0; c < NUM_CLASSES; ++c) key.push_back((u64)pl.work[c].p);
+    auto it = pl.graphs.find(cap);
+    if (it != pl.graphs.end() && it->second.key != key) {
+        HIPCHECK(hipGraphExecDestroy(it->second.exec));
+        pl.graphs.erase(it);
+        it = pl.graphs.end();
     }
-    if (s.nwork[CLASS_WAVE]) {
-        PhaseTimer t(pl, st, PH_WAVE);
-        HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
-        pl.launches_wave++;
+    if (it == pl.graphs.end()) {
+        const Group g = graph_group(pl, npma, pshift, log2S);
+        const u64 out_bytes = STATUS_BYTES + (u64)npma * sizeof(PmaStat);
+        // the capture stream must not start before the work queued on st
+        HIPCHECK(hipStreamSynchronize(st));
+        hipStream_t cs = pl.cap_stream;
+        HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+        BatchBufs bb = batch_bufs(pl);
+        InlineSegs none;
+        memset(&none, 0, sizeof(none));
+        bool ok = hipMemcpyAsync(pl.rhdr.p, pl.h_rhdr, sizeof(RoundHdr), hipMemcpyHostToDevice, cs) == hipSuccess &&
+                  hipMemcpyAsync(pl.segs.p, pl.h_segs, nsegs * sizeof(BatchSeg), hipMemcpyHostToDevice, cs) ==
+                      hipSuccess &&
+                  batch_begin(g, bb.status, none, pl.segs.p, cs) == hipSuccess;
+        if (ok) {
+            u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, hi_bit, pl.pk0.p, pl.pk1.p,
+                                    pl.sort_scr.p, &bb.status->flags, cs);
+            ok = hipGetLastError() == hipSuccess && locate_and_compact(g, pk, cap, bb, cs) == hipSuccess &&
+                 density_check(g, cap, bb, cs) == hipSuccess && elect_windows(g, cap, bb, cs) == hipSuccess &&
+                 rebalance_fixed(g, bb, (u32)cap, nullptr, cs) == hipSuccess &&
+                 group_pyramid(g, GRAPH_PYR_GRID, cs) == hipSuccess &&
+                 hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, cs) == hipSuccess;
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (!ok || e != hipSuccess || !graph) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            return LPMA_ERR_DEVICE;
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return LPMA_ERR_DEVICE;
+        }
+        it = pl.graphs.emplace(cap, Pipeline::GraphEntry{exec, key}).first;
+        pl.graph_captures++;
     }
-    if (s.nwork[CLASS_BLOCK]) {
-        PhaseTimer t(pl, st, PH_BLOCK);
-        HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
-        pl.launches_block++;
+    *out = it->second.exec;
+    return LPMA_OK;
+}
+
+int run_pipeline_graph(Pipeline &pl, const Member *mem, u32 npma, u64 n, u32 pshift, u32 hi_bit, hipStream_t st) {
+    const u64 cap = graph_capacity(n);
+    u64 live_sum = 0;
+    for (u32 p = 0; p < npma; ++p) live_sum += mem[p].h->live;
+    CHECKED(prepare_graph_buffers(pl, cap, npma, live_sum + n));
+
+    u32 epoch, pyr_blocks;
+    CHECKED(begin_epoch(mem, npma, epoch, st));
+    CHECKED(upload_table(pl, mem, npma, pyr_blocks, st));
+
+    // fixed descriptor layout: member p's deletes at 2p, inserts at 2p+1
+    u64 off = 0;
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        for (int op = 0; op < 2; ++op) {
+            BatchSeg &b = pl.h_segs[2 * p + op];
+            b.keys = op ? mem[p].ins : mem[p].del;
+            b.n = op ? mem[p].nins : mem[p].ndel;
+            b.off = off;
+            b.pid = p;
+            b.op = (u32)op;
+            b.V = h->V;
+            b.vbits = h->vbits;
+            off += b.n;
+        }
     }
-    if (s.nwork[CLASS_LARGE]) {
-        // the windows' merged slices add up to at most the members' live keys
-        CHECKED(ensure(pl, pl.merged, m_sum + 1));
-        CHECKED(ensure(pl, pl.lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
-        CHECKED(ensure(pl, pl.lmbase, s.nwork[CLASS_LARGE]));
-        CHECKED(ensure(pl, pl.lhdr, 4));
-        bb = batch_bufs(pl);
-        PhaseTimer t(pl, st, PH_LARGE);
-        HIPCHECK(rebalance_large(g, bb, pl.merged.p, 1024, st));
-    }
+    const u32 log2S = mem[0].h->log2S;
+    hipGraphExec_t exec;
+    CHECKED(graph_for(pl, cap, npma, pshift, hi_bit, log2S, &exec, st));
+    pl.h_rhdr->n = n;
+    pl.h_rhdr->merged = pl.merged.p;
+    pl.h_rhdr->epoch = epoch;
+    pl.h_rhdr->pad_ = 0;
     {
-        PhaseTimer t(pl, st, PH_PYR);
-        HIPCHECK(group_pyramid(g, pyr_blocks, st));
+        PhaseTimer t(pl, st, PH_GRAPH);
+        HIPCHECK(hipGraphLaunch(exec, st));
     }
+    pl.graph_launches++;
+    HIPCHECK(hipStreamSynchronize(st));
+    const Group g = graph_group(pl, npma, pshift, log2S);
+    return finish_round(pl, mem, npma, g, pyr_blocks, true, st);
+}
 
-    if (any_rebuild) {
-        // root rebalance or resize: every live key and every update of the
-        // PMA is merged into scratch and spread again (rebuild_rpma,
-        // rpma.cuh:1417-1472)
-        PhaseTimer t(pl, st, PH_REBUILD);
-        CHECKED(ensure(pl, pl.merged, m_rebuild + 1));
-        for (u32 p = 0; p < npma; ++p) {
-            if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
-            lpma *h = mem[p].h;
-            const u64 M = Mnew[p];
-            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
-            u64 nseg = 1ull << h->st.H;
-            CHECKED(ensure(pl, pl.gpre, nseg + 1));
-            CHECKED(ensure(pl, pl.scan_scr, scan_scratch_words(nseg) + 8));
-            HIPCHECK(rebuild_push(h->st, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.scan_scr.p, st));
-            if (newH != h->st.H) {
-                CHECKED(respread_into_new(h, newH, pl.merged.p, M, st));
-            } else {
-                HIPCHECK(rebuild_spread(h->st, pl.merged.p, M, st));
-                HIPCHECK(rebuild_pyramid(h->st, true, st));
-            }
-            h->rebuilds++;
-            h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
-            pl.rebuilds++;
-            pl.rebuild_slots += (u64)1 << (h->st.H + h->log2S);
-        }
-    }
+// Sizes the batch buffers for rounds of up to max_batch elements and, in
+// graph mode, captures the graph of every capacity up to it now.
+int reserve_pipeline(Pipeline &pl, lpma *const *members, u32 npma, u64 max_batch, hipStream_t st) {
+    pl.cur_stream = st;
+    u32 vmax = 1;
+    u64 live_sum = 0, nseg_total = 0;
     for (u32 p = 0; p < npma; ++p) {
-        lpma *h = mem[p].h;
-        h->live = Mnew[p];
-        h->effective += ps[p].u1 - ps[p].u0;
-        h->version++;
+        vmax = std::max(vmax, members[p]->vbits);
+        live_sum += members[p]->live;
+        nseg_total += 1ull << members[p]->st.H;
     }
-    pl.effective += n_eff;
+    if (max_batch == 0) return LPMA_OK;
+    if (max_batch > 0x7FFFFFF0ull) return LPMA_ERR_TOO_LARGE;
+    const u32 pshift = 2 * vmax + 1;
+    const u32 hi_bit = pshift + bits_for(npma);
+    if (hi_bit > 64) return LPMA_ERR_INVALID_ARGUMENT;
+    if (!pl.graph_on) return ensure_batch(pl, max_batch, npma, 2 * npma, std::min<u64>(max_batch, nseg_total));
+    const u64 top = graph_capacity(max_batch);
+    // headroom for the live keys to grow before the scratch is reallocated
+    CHECKED(prepare_graph_buffers(pl, top, npma, live_sum + live_sum / 4 + max_batch));
+    u32 pyr_blocks;
+    Member m0{members[0], nullptr, 0, nullptr, 0};
+    std::vector<Member> mem(npma, m0);
+    for (u32 p = 0; p < npma; ++p) mem[p].h = members[p];
+    CHECKED(upload_table(pl, mem.data(), npma, pyr_blocks, st));
+    for (u64 c = 4096; c <= top; c = graph_capacity(c + 1)) {
+        hipGraphExec_t exec;
+        CHECKED(graph_for(pl, c, npma, pshift, hi_bit, members[0]->log2S, &exec, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
     return LPMA_OK;
 }
 
@@ -792,8 +1028,12 @@ void add_profile(Pipeline &pl, lpma_profile_t *out) {
     out->launches_wave += pl.launches_wave;
     out->launches_block += pl.launches_block;
     out->launches_seg += pl.launches_seg;
+    out->graph_ms += pl.phase_ms[PH_GRAPH];
+    out->graph_launches += pl.graph_launches;
+    out->graph_captures += pl.graph_captures;
     for (double &v : pl.phase_ms) v = 0;
     pl.launches_wave = pl.launches_block = pl.launches_seg = 0;
+    pl.graph_launches = pl.graph_captures = 0;
 }
 
 }  // namespace
@@ -1143,6 +1383,20 @@ int lpma_set_profiling(lpma_t *h, int on) {
     return LPMA_OK;
 }
 
+int lpma_set_graph(lpma_t *h, int on) {
+    if (!h) return LPMA_ERR_INVALID_ARGUMENT;
+    h->pl.graph_on = on != 0;
+    return LPMA_OK;
+}
+
+int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream) {
+    if (!h) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(h->device);
+    h->cur_stream = (hipStream_t)stream;
+    lpma *one[1] = {h};
+    return reserve_pipeline(h->pl, one, 1, max_batch, (hipStream_t)stream);
+}
+
 int lpma_get_profile(lpma_t *h, lpma_profile_t *out) {
     if (!h || !out) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
@@ -1263,6 +1517,24 @@ int lpma_group_get_stats(lpma_group_t *g, lpma_stats_t *out) {
 int lpma_group_set_profiling(lpma_group_t *g, int on) {
     if (!g) return LPMA_ERR_INVALID_ARGUMENT;
     for (Pipeline *pl : g->pls) pl->profiling = on != 0;
+    return LPMA_OK;
+}
+
+int lpma_group_set_graph(lpma_group_t *g, int on) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    for (Pipeline *pl : g->pls) pl->graph_on = on != 0;
+    return LPMA_OK;
+}
+
+int lpma_group_reserve(lpma_group_t *g, uint64_t max_batch, void *stream) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(g->device);
+    std::vector<lpma *> ms;
+    for (size_t k = 0; k < g->parts.size(); ++k) {
+        ms.clear();
+        for (u32 i : g->parts[k]) ms.push_back(g->members[i]);
+        CHECKED(reserve_pipeline(*g->pls[k], ms.data(), (u32)ms.size(), max_batch, (hipStream_t)stream));
+    }
     return LPMA_OK;
 }
 

@@ -39,10 +39,11 @@ __device__ __forceinline__ u32 source_of(const BatchSeg *__restrict__ segs, u32 
 // the first radix digit of every tile.
 template <bool HIST>
 __global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
-                                                        u32 pshift, u64 *__restrict__ out, u32 *__restrict__ flags,
-                                                        u32 shift, u32 items, u32 *__restrict__ hist, u32 nblocks,
-                                                        int block_major) {
+                                                        const u64 *n_dev, u32 pshift, u64 *__restrict__ out,
+                                                        u32 *__restrict__ flags, u32 shift, u32 items,
+                                                        u32 *__restrict__ hist, u32 nblocks, int block_major) {
     __shared__ u32 h[RS_RADIX];
+    if (n_dev) n = *n_dev;
     if (HIST) {
         h[threadIdx.x] = 0;
         __syncthreads();
@@ -123,9 +124,10 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s
 
 // per-tile digit histogram; layout block-major (hist[b*RADIX + d]) when
 // `block_major`, digit-major (hist[d*nb + b]) otherwise
-__global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys, u64 n, u32 shift, u32 items,
-                                                     u32 *__restrict__ hist, u32 nblocks, int block_major) {
+__global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys, u64 n, const u64 *n_dev, u32 shift,
+                                                     u32 items, u32 *__restrict__ hist, u32 nblocks, int block_major) {
     __shared__ u32 h[RS_RADIX];
+    if (n_dev) n = *n_dev;
     h[threadIdx.x] = 0;
     __syncthreads();
     u64 base = (u64)blockIdx.x * items * RS_TPB;
@@ -143,11 +145,12 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys
 // over earlier tiles from the histogram rows, then a scan over the digits;
 // otherwise hist_scan holds the device-wide exclusive scan (digit-major).
 __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ in, u64 *__restrict__ out, u64 n,
-                                                        u32 shift, u32 items, const u32 *__restrict__ hist,
-                                                        u32 nblocks, int block_major) {
+                                                        const u64 *n_dev, u32 shift, u32 items,
+                                                        const u32 *__restrict__ hist, u32 nblocks, int block_major) {
     __shared__ RankLds<RS_TPB> s;
     __shared__ u32 gofs[RS_RADIX];
     __shared__ u32 stmp[RS_TPB / 64 + 1];
+    if (n_dev) n = *n_dev;
     const u32 d = threadIdx.x;
     if (block_major) {
         u32 tot = 0, pre = 0;
@@ -178,10 +181,12 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ i
 }
 
 // Whole sort inside one workgroup's LDS for n <= RS_SMALL_CAP.
-__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, u32 lo_bit, u32 hi_bit) {
+__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, const u64 *n_dev,
+                                                             u32 lo_bit, u32 hi_bit) {
     __shared__ u64 buf[2][RS_SMALL_CAP];
     __shared__ RankLds<RS_SMALL_TPB> s;
     __shared__ u32 h[RS_RADIX];
+    if (n_dev) n = *n_dev;
     for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) buf[0][i] = keys[i];
     int cur = 0;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
@@ -295,47 +300,26 @@ hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStr
 
 // ------------------------------------------------------------ radix driver
 
-static u32 rs_items(u64 n) { return n <= (1ull << 20) ? RS_ITEMS_SMALL : RS_ITEMS_BIG; }
+// small tiles keep >= ~2000 workgroups in flight up to 8M keys
+static u32 rs_items(u64 n) { return n <= (1ull << 23) ? RS_ITEMS_SMALL : RS_ITEMS_BIG; }
 
 u64 sort_scratch_words(u64 n) {
     u64 nb = (n + (u64)rs_items(n) * RS_TPB - 1) / ((u64)rs_items(n) * RS_TPB);
     return (u64)RS_RADIX * nb + 1 + scan_scratch_words((u64)RS_RADIX * nb);
 }
 
-// Sorts keys[0..n) on bits [lo_bit, hi_bit).  Returns the buffer that holds
-// the result (keys or alt).
-u64 *radix_sort(u64 *keys, u64 *alt, u64 n, u32 nbits, u32 *scratch, hipStream_t st) {
-    if (n <= 1) return keys;
-    if (n <= (u64)RS_SMALL_CAP) {
-        k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(keys, n, 0, nbits);
-        return keys;
-    }
-    const u32 items = rs_items(n);
-    const u64 tile = (u64)items * RS_TPB;
-    const u32 nb = (u32)((n + tile - 1) / tile);
-    const int block_major = nb <= RS_PREFIX_MAX_TILES;
-    u32 *hist = scratch;
-    u32 *sscr = scratch + (u64)RS_RADIX * nb + 1;
-    u64 *src = keys, *dst = alt;
-    for (u32 shift = 0; shift < nbits; shift += RS_BITS) {
-        k_rs_hist<<<nb, RS_TPB, 0, st>>>(src, n, shift, items, hist, nb, block_major);
-        if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nb, hist, sscr, st);
-        k_rs_scatter<<<nb, RS_TPB, 0, st>>>(src, dst, n, shift, items, hist, nb, block_major);
-        u64 *t = src; src = dst; dst = t;
-    }
-    return src;
-}
-
 // Packs a group batch (n elements from nsegs sources) and sorts the packed
 // keys on bits [lo_bit, hi_bit): the first histogram is taken while packing.
-u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, u32 pshift, u32 lo_bit, u32 hi_bit, u64 *k0, u64 *k1,
-                   u32 *scratch, u32 *flags, hipStream_t st) {
+// With n_dev the kernels read the element count from device memory and n
+// is only a capacity that sizes the grids (graph replays).
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 lo_bit, u32 hi_bit,
+                   u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
     if (n == 0) return k0;
     if (n <= (u64)RS_SMALL_CAP) {
         const u32 items = 4;
         k_pack_group<false><<<(u32)((n + items * RS_TPB - 1) / (items * RS_TPB)), RS_TPB, 0, st>>>(
-            segs, nsegs, n, pshift, k0, flags, 0, items, nullptr, 0, 0);
-        if (n > 1) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, lo_bit, hi_bit);
+            segs, nsegs, n, n_dev, pshift, k0, flags, 0, items, nullptr, 0, 0);
+        if (n > 1 || n_dev) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, n_dev, lo_bit, hi_bit);
         return k0;
     }
     const u32 items = rs_items(n);
@@ -347,12 +331,12 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, u32 pshift, u32 lo_bi
     u64 *src = k0, *dst = k1;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
         if (shift == lo_bit)
-            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, pshift, k0, flags, shift, items, hist, nblk,
-                                                       block_major);
+            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, k0, flags, shift, items, hist,
+                                                       nblk, block_major);
         else
-            k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, shift, items, hist, nblk, block_major);
+            k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, n_dev, shift, items, hist, nblk, block_major);
         if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);
-        k_rs_scatter<<<nblk, RS_TPB, 0, st>>>(src, dst, n, shift, items, hist, nblk, block_major);
+        k_rs_scatter<<<nblk, RS_TPB, 0, st>>>(src, dst, n, n_dev, shift, items, hist, nblk, block_major);
         u64 *t = src; src = dst; dst = t;
     }
     return src;

@@ -51,6 +51,7 @@ hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, Bat
 __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restrict__ pk, u64 n,
                                                     u64 *__restrict__ ukey, u64 *__restrict__ useg,
                                                     i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
+    if (g.rh) n = g.rh->n;
     const u64 pmask = (1ull << g.pshift) - 1;
     const u64 i = (u64)blockIdx.x * LOC_TILE + threadIdx.x;
     i32 eff = 0, dl = 0;
@@ -117,12 +118,14 @@ static constexpr u32 COMPACT_SELF_PREFIX_MAX = 2048;  // blocks
 // derives its prefix (and the totals) from the per-block partials itself;
 // otherwise k_scan_parts has already scanned them.
 __global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u64 *__restrict__ useg, u64 n,
-                                                     const i32 *__restrict__ pc, const i32 *__restrict__ pdl, u32 nb,
+                                                     const RoundHdr *rh, const i32 *__restrict__ pc,
+                                                     const i32 *__restrict__ pdl, u32 nb,
                                                      int self_prefix, u64 *__restrict__ ckey, u64 *__restrict__ cseg,
                                                      i32 *__restrict__ pd, Status *__restrict__ status) {
     __shared__ i32 tmp[LOC_TPB / 64 + 1];
     __shared__ i32 red[4][LOC_TPB / 64];
     i32 base_c, base_d;
+    if (rh) n = rh->n;
     if (self_prefix) {
         i32 a = 0, b = 0, ta = 0, tb = 0;
         for (u32 x = threadIdx.x; x < nb; x += LOC_TPB) {
@@ -170,7 +173,8 @@ hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchB
     k_locate<<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
     int self_prefix = nb <= COMPACT_SELF_PREFIX_MAX;
     if (!self_prefix) k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
-    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd, b.status);
+    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, g.rh, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd,
+                                      b.status);
     return hipGetLastError();
 }
 
@@ -227,9 +231,10 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
     const u32 n_eff = status_in->n_eff;
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     u32 local_flags = 0;
+    u32 pid = 0xFFFFFFFFu, lvbit = 0;
     if (j < n_eff) {
         const u64 rk = cseg[j];
-        const u32 pid = (u32)(rk >> 32);
+        pid = (u32)(rk >> 32);
         const u64 prev = j > 0 ? cseg[j - 1] : RK_NONE;
         if (j == 0 || (u32)(prev >> 32) != pid) {
             g.pstat[pid].u0 = j;
@@ -260,19 +265,28 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             if (l >= d.H) {
                 local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
                 atomicOr(&g.pstat[pid].flags, local_flags);
-            } else {
-                if (work_class(C + (u1 - u0), l, d.log2S) == CLASS_LARGE) local_flags = FLAG_LARGE;
-                // claim the window; the smallest run index wins (atomic max of ~j)
+            } else if (l > 0) {
+                // claim the window; the smallest run index wins (atomic max of
+                // ~j).  A single-segment window has one candidate, its own run,
+                // so level 0 needs no claim.
                 atomicMax((unsigned long long *)&d.owner[owner_idx(nseg_of(d), l, (u64)s >> l)],
-                          (unsigned long long)(((u64)g.epoch << 32) | (u64)(~j)));
+                          (unsigned long long)(((u64)group_epoch(g) << 32) | (u64)(~j)));
+                lvbit = 1u << l;
             }
         }
     }
-    // one atomic per wave
+    // one atomic per wave for the flags; the claimed levels per PMA: lanes of
+    // the first PMA in the wave combine theirs, the others (rare) go alone
     u32 f = local_flags;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
     if (lane_id() == 0 && f) atomicOr(&status->flags, f);
+    const u32 pid0 = __shfl(pid, 0, 64);
+    u32 lv = pid == pid0 ? lvbit : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lv |= __shfl_xor(lv, o, 64);
+    if (lane_id() == 0 && lv) atomicOr(&g.pstat[pid0].levels, lv);
+    if (pid != pid0 && lvbit) atomicOr(&g.pstat[pid].levels, lvbit);
 }
 
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
@@ -301,15 +315,19 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
         u32 L = leader ? rlevel[j] : 0;
         u64 nseg = nseg_of(d);
         u64 w = (u64)s >> L;
-        const u64 tag = (u64)g.epoch << 32;
-        if (leader && d.owner[owner_idx(nseg, L, w)] != (tag | (u64)(~j))) leader = false;
+        const u32 epoch = group_epoch(g);
+        const u64 tag = (u64)epoch << 32;
+        if (leader && L > 0 && d.owner[owner_idx(nseg, L, w)] != (tag | (u64)(~j))) leader = false;
         if (leader) {
-            // inside a larger chosen window?  the loads are independent, so
-            // they are issued together rather than as a dependent chain
+            // inside a larger chosen window?  only the levels some run of the
+            // PMA claimed can hold one; their loads are issued together
+            u32 lv = g.pstat[pid].levels & ~((2u << L) - 1u);
             bool dom = false;
-#pragma unroll 8
-            for (u32 l2 = L + 1; l2 < d.H; ++l2)
-                dom |= (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] >> 32) == (u64)g.epoch;
+            while (lv) {
+                u32 l2 = (u32)__builtin_ctz(lv);
+                lv &= lv - 1;
+                dom |= (d.owner[owner_idx(nseg, l2, (u64)s >> l2)] >> 32) == (u64)epoch;
+            }
             if (dom) leader = false;
         }
         if (leader) {
@@ -727,6 +745,19 @@ hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, 
     return hipGetLastError();
 }
 
+// Every rebalance class with grids fixed in advance (graph replays): the
+// kernels loop over the work counts the election left in the status, and
+// runs_cap bounds them.  merged must hold the live keys of every PMA of
+// the group.
+hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st) {
+    (void)rebalance_seg(g, b, runs_cap < 65536u ? runs_cap : 65536u, st);
+    const u32 gw = runs_cap < 4096 ? (runs_cap ? runs_cap : 1) : 4096;
+    k_rebalance_wave<<<gw, 64, 0, st>>>(g, b.work[CLASS_WAVE], b.status, b.ckey, b.pd);
+    const u32 gb = runs_cap < 1024 ? (runs_cap ? runs_cap : 1) : 1024;
+    k_rebalance_block<<<gb, 256, 0, st>>>(g, b.work[CLASS_BLOCK], b.status, b.ckey, b.pd);
+    return rebalance_large(g, b, merged, 1024, st);
+}
+
 // ---------------------------------------------------- global window path
 //
 // Windows too large for one workgroup's LDS, the root window, and resizes.
@@ -888,6 +919,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__re
                                                         const u32 *__restrict__ hdr, const u64 *__restrict__ ckey,
                                                         const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                                                         u64 *__restrict__ merged) {
+    if (g.rh) merged = g.rh->merged;
     __shared__ u32 pre[LG_SEG_MAX + 1];
     __shared__ u32 stmp[LG_TPB / 64 + 1];
     __shared__ u32 qr[2];
@@ -940,6 +972,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__re
 __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__restrict__ work,
                                                         const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
                                                         const u32 *__restrict__ hdr, const u64 *__restrict__ merged) {
+    if (g.rh) merged = g.rh->merged;
     const u32 nwin = hdr[0], nch = hdr[1];
     const u32 log2S = g.log2S;
     const u32 Smask = (1u << log2S) - 1;
@@ -1079,17 +1112,25 @@ __device__ void pyr_chunk(const DevState &d, u64 chunk) {
 __global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) { pyr_chunk(d, blockIdx.x); }
 
 // the chunks of every PMA of the run that took window rebalances
+// the chunks of every PMA of the run that took window rebalances; the
+// workgroups stride over the group's chunk list (its length is read from
+// the table, so the grid does not depend on the members' heights)
 __global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
-    u32 lo = 0, hi = g.npma;
-    while (hi - lo > 1) {
-        u32 mid = (lo + hi) >> 1;
-        if (g.tab[mid].pyr_blk0 <= blockIdx.x) lo = mid; else hi = mid;
+    const PmaEntry &last = g.tab[g.npma - 1];
+    const u32 total = last.pyr_blk0 + last.pyr_nblk;
+    for (u32 c = blockIdx.x; c < total; c += gridDim.x) {
+        u32 lo = 0, hi = g.npma;
+        while (hi - lo > 1) {
+            u32 mid = (lo + hi) >> 1;
+            if (g.tab[mid].pyr_blk0 <= c) lo = mid; else hi = mid;
+        }
+        const PmaStat ps = g.pstat[lo];
+        if (ps.u1 == ps.u0 || (ps.flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
+        const DevState d = g.tab[lo].d;
+        if (d.H == 0) continue;
+        pyr_chunk(d, c - g.tab[lo].pyr_blk0);
+        __syncthreads();
     }
-    const PmaStat ps = g.pstat[lo];
-    if (ps.u1 == ps.u0 || (ps.flags & (FLAG_ROOT | FLAG_RESIZE))) return;
-    const DevState d = g.tab[lo].d;
-    if (d.H == 0) return;
-    pyr_chunk(d, blockIdx.x - g.tab[lo].pyr_blk0);
 }
 
 __device__ void pyr_top_levels(const DevState &d) {
@@ -1117,7 +1158,7 @@ __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, Inline
     }
     if (threadIdx.x == 0) {
         PmaStat z;
-        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0;
+        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0; z.levels = 0;
         g.pstat[blockIdx.x] = z;
     }
     const DevState d = g.tab[blockIdx.x].d;
@@ -1128,7 +1169,7 @@ __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, Inline
 // batch_begin); nblocks = the pyr_blk0 prefix total
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
-    k_pyr_chunk_group<<<nblocks, 256, 0, st>>>(g);
+    k_pyr_chunk_group<<<nblocks < 2048 ? nblocks : 2048, 256, 0, st>>>(g);
     return hipGetLastError();
 }
 

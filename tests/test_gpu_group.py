@@ -23,13 +23,16 @@ def rand_keys(rng, V, n):
 class Fleet:
     """Group members, their solo twins and the oracle's key sets."""
 
-    def __init__(self, Vs, segs=None, lower=0.42, upper=1.0):
+    def __init__(self, Vs, segs=None, lower=0.42, upper=1.0, graph=True):
+        """graph: replay every round as a captured graph (the tests cover
+        both modes; eager rounds are the library default)."""
         segs = segs or [32] * len(Vs)
         self.Vs = list(Vs)
         self.members = [L.PMA(V, segment_slots=s, density_lower=lower, density_upper=upper) for V, s in zip(Vs, segs)]
         self.twins = [L.PMA(V, segment_slots=s, density_lower=lower, density_upper=upper) for V, s in zip(Vs, segs)]
         self.refs = [np.empty(0, dtype=np.uint64) for _ in Vs]
         self.group = L.Group(self.members)
+        self.group.set_graph(graph)
 
     def round(self, ins, dels):
         self.group.update(ins, dels)
@@ -200,4 +203,80 @@ def test_group_many_members_wide_ids():
         ins, dels = mixed_round(rng, f, 40, p_empty=0.3)
         f.round(ins, dels)
         f.compare(light=(64,))
+    f.close()
+
+
+def test_group_eager_rounds_match_solo_updates():
+    """The same rounds without graph replay (stream-ordered launches with one
+    synchronisation after the election)."""
+    say("group eager rounds")
+    rng = np.random.default_rng(28)
+    Vs = [int(v) for v in rng.integers(16, 3000, 9)]
+    f = Fleet(Vs, graph=False)
+    for r in range(8):
+        ins, dels = mixed_round(rng, f, 500)
+        f.round(ins, dels)
+        f.compare()
+    f.close()
+
+
+def test_plain_handle_graph_replays():
+    """A handle in graph mode against an eager twin: batch sizes that grow
+    (capacity changes and captures again), resizes, mixed batches, an
+    invalid batch between replays, and empty batches."""
+    say("plain handle graph replays")
+    rng = np.random.default_rng(29)
+    V = 3000
+    g = L.PMA(V)
+    g.set_graph(True)
+    e = L.PMA(V)
+    ref = np.empty(0, dtype=np.uint64)
+    sizes = [10, 100, 5000, 20, 40000, 300, 120000, 7, 2000, 60000]
+    for k, n in enumerate(sizes):
+        ins = rand_keys(rng, V, n)
+        dels = ref[rng.integers(0, ref.size, n // 3)] if ref.size else np.empty(0, dtype=np.uint64)
+        g.update_mixed(ins, dels)
+        e.update_mixed(ins, dels)
+        ref = oracle.apply(ref, ins, dels)
+        if k == 4:
+            with pytest.raises(L.LpmaError):
+                g.insert(keys_of([V + 1], [0]))
+            with pytest.raises(L.LpmaError):
+                e.insert(keys_of([V + 1], [0]))
+        if k == 6:
+            g.update_mixed(np.empty(0, dtype=np.uint64), np.empty(0, dtype=np.uint64))
+            e.update_mixed(np.empty(0, dtype=np.uint64), np.empty(0, dtype=np.uint64))
+        gk, gro = g.copy_view()
+        ek, ero = e.copy_view()
+        assert int(g.view().tree_height) == int(e.view().tree_height)
+        assert np.array_equal(L.to_host_u64(gk), L.to_host_u64(ek)), k
+        assert np.array_equal(gro.cpu().numpy(), ero.cpu().numpy()), k
+        assert np.array_equal(L.to_host_u64(g.export()[0]), ref), k
+        assert g.check() == 0
+    p = g.profile()
+    assert p.graph_launches >= len(sizes) and p.graph_captures >= 2
+    g.close()
+    e.close()
+
+
+def test_group_reserve_captures_once():
+    """After reserve(max round), rounds up to that size replay cached graphs
+    (no capture), also across member resizes, and still match solo updates."""
+    say("group reserve")
+    rng = np.random.default_rng(30)
+    Vs = [int(v) for v in rng.integers(8, 2000, 10)]
+    f = Fleet(Vs)
+    f.group.reserve(60000)
+    f.group.set_profiling(True)
+    f.group.profile()
+    for r in range(10):
+        ins, dels = mixed_round(rng, f, 800 * (r + 1), p_empty=0.1)
+        tot = sum(a.size for a in ins if a is not None) + sum(d.size for d in dels if d is not None)
+        if tot > 60000:
+            continue
+        f.round(ins, dels)
+        f.compare()
+    p = f.group.profile()
+    assert p.graph_captures == 0 and p.graph_launches > 0
+    assert f.group.stats().rebuilds > 0  # members were resized between replays
     f.close()
