@@ -184,13 +184,16 @@ def _ptr_arrays(tensors_per_graph, r):
     return p, c
 
 
-def build_fleet(w, seg, lower, upper):
+def build_fleet(w, seg, density, slots_per_lane):
     import torch
     import lpma_amd as L
+    ll, lr, ul, ur = density
     fleet = []
     for i in range(w.n):
-        p = L.PMA(w.V[i], segment_slots=seg, density_lower=lower, density_upper=upper)
+        p = L.PMA(w.V[i], segment_slots=seg, density_lower=lr, density_upper=ur, density_lower_leaf=ll,
+                  density_upper_leaf=ul)
         p.bulk_load(L.to_device(w.initial[i]))
+        p.set_launch(slots_per_lane)
         fleet.append(p)
     torch.cuda.synchronize()
     return fleet
@@ -242,7 +245,16 @@ def run_batched(args):
     ins_d = [[L.to_device(b, dev) for b in w.ins[i]] for i in range(w.n)]
     del_d = [[L.to_device(b, dev) for b in w.dels[i]] for i in range(w.n)]
     stream = torch.cuda.current_stream()
-    seg, lower, upper = args.segment_slots, 0.42, 1.0
+    seg, density, spl = args.segment_slots, (0.42, 0.42, 1.0, 1.0), 0
+    tuned = None
+    if args.tune:
+        from lpma_amd.tune import autotune
+        t = time.time()
+        best, trials = autotune(w.initial, w.V, w.ins, w.dels, rounds=min(4, total), log=log)
+        seg, density, spl = best.segment_slots, best.density, best.slots_per_lane
+        tuned = {"best": best.as_dict(), "trials": [x.as_dict() for x in trials],
+                 "sample": f"first {min(4, total)} rounds, fresh groups per trial", "seconds": time.time() - t}
+        log(f"autotuner picked S={seg} density={density} slots/lane={spl or 'default'} in {time.time() - t:.1f}s")
     warm = list(range(args.warmup))
     timed = list(range(args.warmup, total))
     n_timed = int(w.sizes[args.warmup:].sum())
@@ -252,10 +264,11 @@ def run_batched(args):
     forms = ("plain", "batched", "batched_graph") if args.forms == "both" else (args.forms,)
     for form in forms:
         t = time.time()
-        fleet = build_fleet(w, seg, lower, upper)
+        fleet = build_fleet(w, seg, density, spl)
         log(f"{form}: fleet bulk-loaded in {time.time() - t:.1f}s")
         group = L.Group(fleet) if form != "plain" else None
         if group is not None:
+            group.set_launch(spl)
             group.set_graph(form == "batched_graph" or (args.forms == "batched" and bool(args.graph)))
             # size the buffers (and capture the round graphs) before timing
             group.reserve(int(w.sizes.sum(axis=1).max()))
@@ -294,13 +307,16 @@ def run_batched(args):
         "data": "synthetic (seeded R-MAT generators)",
         "config": {"workload": "batched_64_rmat_graphs_lognormal_batches", "graphs": w.n,
                    "initial_edges": int(sum(w.edges)), "updates_timed": n_timed, "segment_slots": seg,
-                   "density": f"lower {lower} / upper {upper}", "rounds": "stream-ordered, one sync per round",
+                   "density_leaf_root": {"lower": [density[0], density[1]], "upper": [density[2], density[3]]},
+                   "seg_slots_per_lane": spl or "default", "rounds": "stream-ordered, one sync per round",
+                   "plain_form": "same configuration, one update call per graph per round",
                    "parallelism": "single GPU"},
         "plain_value": p["value"],
         "speedup_vs_plain": b["value"] / p["value"],
         "round_p50_ms": b["p50"], "round_p99_ms": b["p99"],
         "plain_round_p50_ms": p["p50"], "plain_round_p99_ms": p["p99"],
         "graph_rounds_value": bg["value"], "graph_round_p50_ms": bg["p50"], "graph_round_p99_ms": bg["p99"],
+        "autotune": tuned,
     }
     print(json.dumps(out))
 
@@ -311,6 +327,7 @@ def main():
     ap.add_argument("--workload", choices=("headline", "batched"), default="headline")
     ap.add_argument("--graph", type=int, default=0, help="--forms batched: replay rounds as hipGraphs")
     ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--tune", type=int, default=1, help="batched workload: autotune segment size / density / shape")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--batch", type=int, default=100000)

@@ -206,6 +206,13 @@ int lpma_set_graph(lpma_t *h, int on);
  * later rounds of that size capture nothing. */
 int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream);
 
+/* Workgroup shape of the single-segment window rebalance: slots handled by
+ * each lane (a power of two; 0 restores the default, 4 slots per lane up to
+ * 256-slot segments, a full wave per segment above).  The lanes of one
+ * window never exceed a wave; the LDS of a workgroup bounds it too.  No
+ * counterpart in the reference (one warp per window, rpma.cuh:1226-1243). */
+int lpma_set_launch(lpma_t *h, uint32_t seg_slots_per_lane);
+
 /* Device pool shared by all handles on a device: cap on bytes held from
  * hipMalloc (0 = unlimited), and release of cached free blocks. */
 int lpma_set_pool_budget(uint64_t bytes);
@@ -247,6 +254,7 @@ int lpma_group_set_graph(lpma_group_t *g, int on);
 /* lpma_reserve for every partition of the group (max_batch = the largest
  * total batch of one round). */
 int lpma_group_reserve(lpma_group_t *g, uint64_t max_batch, void *stream);
+int lpma_group_set_launch(lpma_group_t *g, uint32_t seg_slots_per_lane);
 int lpma_group_get_profile(lpma_group_t *g, lpma_profile_t *out);
 
 /* ---------------------------------------------------------------------------

@@ -112,6 +112,7 @@ SIGNATURES = [
     ("lpma_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_set_graph", _int, [_vp, _int]),
     ("lpma_reserve", _int, [_vp, _u64, _vp]),
+    ("lpma_set_launch", _int, [_vp, _u32]),
     ("lpma_set_pool_budget", _int, [_u64]),
     ("lpma_pool_trim", _int, []),
     ("lpma_pool_reserved_bytes", _u64, []),
@@ -125,6 +126,7 @@ SIGNATURES = [
     ("lpma_group_set_profiling", _int, [_vp, _int]),
     ("lpma_group_set_graph", _int, [_vp, _int]),
     ("lpma_group_reserve", _int, [_vp, _u64, _vp]),
+    ("lpma_group_set_launch", _int, [_vp, _u32]),
     ("lpma_group_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_gen_uniform", _int, [_u32, _u64, _u64, _vp]),
     ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
@@ -424,6 +426,9 @@ class PMA:
     def reserve(self, max_batch):
         _check(lib().lpma_reserve(self._h, int(max_batch), _stream_ptr(self.stream)), "reserve")
 
+    def set_launch(self, seg_slots_per_lane=0):
+        _check(lib().lpma_set_launch(self._h, int(seg_slots_per_lane)), "set_launch")
+
 
 class Group:
     """Independent PMAs updated by one pipeline run per round (the batched
@@ -487,6 +492,9 @@ class Group:
 
     def reserve(self, max_batch):
         _check(lib().lpma_group_reserve(self._h, int(max_batch), _stream_ptr(self.stream)), "group_reserve")
+
+    def set_launch(self, seg_slots_per_lane=0):
+        _check(lib().lpma_group_set_launch(self._h, int(seg_slots_per_lane)), "group_set_launch")
 
     def profile(self):
         p = Profile()

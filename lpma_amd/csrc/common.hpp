@@ -156,6 +156,7 @@ struct Group {
     u32 pshift;
     u32 epoch;    // tags this run's window claims
     u32 log2S;    // segment size, the same for every PMA of the run
+    u32 seg_log2G;  // lanes per window of the single-segment rebalance (log2)
     const RoundHdr *rh;  // graph replays: n and epoch come from here
 };
 

@@ -43,6 +43,7 @@ hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t 
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
 hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st);
+u32 seg_lanes_log2(u32 log2S, int slots_log2);
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st);
 u32 pyramid_blocks(u32 H);
 

@@ -158,6 +158,8 @@ struct Pipeline {
     u64 *d_stats = nullptr;
     u64 rounds = 0, effective = 0, rebuilds = 0, rebuild_slots = 0;
 
+    int seg_slots_log2 = -1;  // slots per lane of the single-segment rebalance (-1: default)
+
     // graph replays: one captured graph per round shape
     bool graph_on = false;
     hipStream_t cap_stream = nullptr;
@@ -767,6 +769,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.pshift = pshift;
     g.epoch = epoch;
     g.log2S = mem[0].h->log2S;
+    g.seg_log2G = seg_lanes_log2(g.log2S, pl.seg_slots_log2);
     g.rh = nullptr;
     BatchBufs bb = batch_bufs(pl);
     // bit 0 of a packed key is the operation: sorted only when a member mixes them
@@ -845,6 +848,7 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     g.pshift = pshift;
     g.epoch = 0;
     g.log2S = log2S;
+    g.seg_log2G = seg_lanes_log2(log2S, pl.seg_slots_log2);
     g.rh = pl.rhdr.p;
     return g;
 }
@@ -853,7 +857,7 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
 int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S, hipGraphExec_t *out,
               hipStream_t st) {
     const u32 nsegs = 2 * npma;
-    std::vector<u64> key = {cap, npma, pshift, hi_bit, log2S,
+    std::vector<u64> key = {cap, npma, pshift, hi_bit, log2S, (u64)(i64)pl.seg_slots_log2,
                             (u64)pl.pk0.p, (u64)pl.pk1.p, (u64)pl.sort_scr.p, (u64)pl.ukey.p, (u64)pl.useg.p,
                             (u64)pl.ckey.p, (u64)pl.cseg.p, (u64)pl.pd.p, (u64)pl.rlevel.p, (u64)pl.part.p,
                             (u64)pl.lchunk.p, (u64)pl.lmbase.p, (u64)pl.lhdr.p, (u64)pl.out.p, (u64)pl.tab.p,
@@ -1389,6 +1393,21 @@ int lpma_set_graph(lpma_t *h, int on) {
     return LPMA_OK;
 }
 
+static int slots_log2_of(uint32_t slots_per_lane, int *out) {
+    if (slots_per_lane == 0) {
+        *out = -1;
+        return LPMA_OK;
+    }
+    if (slots_per_lane > 1024 || (slots_per_lane & (slots_per_lane - 1))) return LPMA_ERR_INVALID_ARGUMENT;
+    *out = __builtin_ctz(slots_per_lane);
+    return LPMA_OK;
+}
+
+int lpma_set_launch(lpma_t *h, uint32_t seg_slots_per_lane) {
+    if (!h) return LPMA_ERR_INVALID_ARGUMENT;
+    return slots_log2_of(seg_slots_per_lane, &h->pl.seg_slots_log2);
+}
+
 int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream) {
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
@@ -1523,6 +1542,14 @@ int lpma_group_set_profiling(lpma_group_t *g, int on) {
 int lpma_group_set_graph(lpma_group_t *g, int on) {
     if (!g) return LPMA_ERR_INVALID_ARGUMENT;
     for (Pipeline *pl : g->pls) pl->graph_on = on != 0;
+    return LPMA_OK;
+}
+
+int lpma_group_set_launch(lpma_group_t *g, uint32_t seg_slots_per_lane) {
+    if (!g) return LPMA_ERR_INVALID_ARGUMENT;
+    int v;
+    CHECKED(slots_log2_of(seg_slots_per_lane, &v));
+    for (Pipeline *pl : g->pls) pl->seg_slots_log2 = v;
     return LPMA_OK;
 }
 

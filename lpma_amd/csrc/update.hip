@@ -726,9 +726,20 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
     }
 }
 
+// lanes per window (log2) of the single-segment rebalance: `slots_log2`
+// slots per lane, by default 4 (S/64 from 256-slot segments up), at most a
+// wave, and within the LDS of a workgroup
+u32 seg_lanes_log2(u32 log2S, int slots_log2) {
+    int g = slots_log2 < 0 ? (log2S >= 8 ? 6 : (int)log2S - 2) : (int)log2S - slots_log2;
+    if (g < 0) g = 0;
+    if (g > 6) g = 6;
+    while (g < 6 && (size_t)(SEG_TPB >> g) * (24ull << log2S) + 8 * (SEG_TPB >> g) > 160u * 1024) ++g;
+    return (u32)g;
+}
+
 hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st) {
     u32 S = 1u << g.log2S;
-    u32 log2G = g.log2S >= 8 ? 6 : (g.log2S <= 2 ? 0 : g.log2S - 2);
+    u32 log2G = g.seg_log2G;
     u32 groups_per_block = SEG_TPB >> log2G;
     size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
     u64 blocks = (windows + groups_per_block - 1) / groups_per_block;

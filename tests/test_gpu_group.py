@@ -280,3 +280,32 @@ def test_group_reserve_captures_once():
     assert p.graph_captures == 0 and p.graph_launches > 0
     assert f.group.stats().rebuilds > 0  # members were resized between replays
     f.close()
+
+
+@pytest.mark.parametrize("spl", [1, 2, 8, 64])
+def test_group_launch_shapes(spl):
+    """Every single-segment rebalance shape gives the solo layout."""
+    say(f"group launch shape {spl}")
+    rng = np.random.default_rng(31)
+    Vs = [int(v) for v in rng.integers(16, 3000, 6)]
+    f = Fleet(Vs, segs=[64] * 6, graph=False)
+    f.group.set_launch(spl)
+    for r in range(5):
+        ins, dels = mixed_round(rng, f, 400)
+        f.round(ins, dels)
+        f.compare()
+    f.close()
+
+
+def test_autotuner_smoke():
+    """The tuner runs its trials on a tiny fleet and returns a valid choice."""
+    from lpma_amd.tune import autotune
+    say("autotuner")
+    rng = np.random.default_rng(32)
+    Vs = [256, 1024, 4096]
+    initial = [np.unique(rand_keys(rng, V, 4 * V)) for V in Vs]
+    ins = [[rand_keys(rng, V, 500) for _ in range(3)] for V in Vs]
+    dels = [[initial[i][rng.integers(0, initial[i].size, 200)] for _ in range(3)] for i in range(3)]
+    best, trials = autotune(initial, Vs, ins, dels, rounds=3, segment_slots=(32, 128), slots_per_lane=(0, 2))
+    assert best.segment_slots in (32, 128) and best.value > 0
+    assert len(trials) == 2 + 3 + 1
