@@ -46,6 +46,7 @@ typedef enum lpma_op { LPMA_OP_DELETE = 0, LPMA_OP_INSERT = 1 } lpma_op;
 
 typedef struct lpma lpma_t;
 typedef struct lpma_group lpma_group_t;
+typedef struct lpma_snapshot lpma_snapshot_t;
 
 /* Structure configuration.  The reference hard-codes 32-slot segments
  * (rpma.cuh:1769) and density bounds lower = 0.42*len, upper = len at every
@@ -128,6 +129,15 @@ int lpma_update(lpma_t *h, const uint64_t *d_keys, uint64_t n, int op, void *str
 int lpma_update_mixed(lpma_t *h, const uint64_t *d_ins, uint64_t n_ins, const uint64_t *d_del, uint64_t n_del,
                       void *stream);
 
+/* `count` mixed batches applied in order as one pipeline run: the result
+ * equals calling lpma_update_mixed(d_ins[i], n_ins[i], d_del[i], n_del[i])
+ * for i = 0 .. count-1 (for each key the last batch touching it decides),
+ * with one sort, one election and one rebalance pass for all of them.  The
+ * reference applies one batch per update_rpma call (rpma.cuh:1549); this is
+ * how a server merges queued write requests.  n_ins or n_del may be NULL. */
+int lpma_update_sequence(lpma_t *h, const uint64_t *const *d_ins, const uint64_t *n_ins,
+                         const uint64_t *const *d_del, const uint64_t *n_del, uint32_t count, void *stream);
+
 /* Bulk load = one insert batch (significant_insert, gpma.cuh:720-744). */
 int lpma_bulk_load(lpma_t *h, const uint64_t *d_keys, uint64_t n, void *stream);
 
@@ -175,6 +185,27 @@ int lpma_neighbor_gather(lpma_t *h, const uint32_t *d_srcs, uint64_t n, const ui
  * vertices.  Synchronises the stream once per level. */
 int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Snapshots: a read-only device copy of the gapped view (keys, segment counts
+ * and separators, row offsets) at one version.  Reads on a snapshot see that
+ * version while updates go on through the handle, so long readers (BFS) run
+ * beside writers.  The reference reads the live arrays only (query_rpma,
+ * rpma.cuh:1974); there is no counterpart.  lpma_snapshot_create queues the
+ * copies on `stream`; reads on any stream wait for them.  Destroy a
+ * snapshot only after the reads on it have finished. */
+uint64_t lpma_snapshot_bytes(const lpma_t *h);
+int lpma_snapshot_create(lpma_snapshot_t **out, lpma_t *h, void *stream);
+int lpma_snapshot_destroy(lpma_snapshot_t *s);
+int lpma_snapshot_get_view(const lpma_snapshot_t *s, lpma_view_t *out);
+int lpma_snapshot_neighbor_count(lpma_snapshot_t *s, const uint32_t *d_srcs, uint64_t n, uint32_t *d_len,
+                                 void *stream);
+int lpma_snapshot_neighbor_gather(lpma_snapshot_t *s, const uint32_t *d_srcs, uint64_t n, const uint32_t *d_offsets,
+                                  uint32_t *d_out, void *stream);
+int lpma_snapshot_edge_query(lpma_snapshot_t *s, const uint64_t *d_keys, uint64_t n, uint32_t *d_result,
+                             void *stream);
+int lpma_snapshot_bfs(lpma_snapshot_t *s, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned,
+                      void *stream);
+
 /* PageRank over the gapped layout with nvGraph's formulation
  * (include/pagerank.hxx:66-88): from the uniform vector, until the L1 change
  * is below `tolerance` or `max_iter` iterations, then L1-normalised.  FP32
@@ -220,6 +251,65 @@ int lpma_pool_trim(void);
 uint64_t lpma_pool_reserved_bytes(void);
 
 const char *lpma_strerror(int code);
+
+/* ---------------------------------------------------------------------------
+ * Served form: a request loop over one resident PMA (no counterpart in the
+ * reference, whose drivers call update_rpma in a loop, test_update_rpma.cu).
+ * Requests are submitted from any host thread with host buffers (copied at
+ * submission) and served with continuous batching: all update requests
+ * queued when the engine comes round are applied as one pipeline run in
+ * submission order, all queued neighbor reads as one pass over the live
+ * structure after them, and each BFS on a snapshot of the version current
+ * at its admission, on a worker stream beside the engine.  The device memory
+ * each request needs is charged against a byte budget; requests wait while
+ * it has no room.  While a server runs, its handle must not be used
+ * directly. */
+typedef struct lpma_server lpma_server_t;
+
+enum { LPMA_REQ_UPDATE = 0, LPMA_REQ_READ = 1, LPMA_REQ_BFS = 2 };
+
+typedef struct {
+    uint32_t bfs_workers;         /* searches running at once (streams), default 4 */
+    uint64_t hbm_budget_bytes;    /* 0: 90% of the device memory free at creation */
+    uint64_t max_merge_keys;      /* keys of one merged update run, default 2^24 */
+    uint32_t max_merge_requests;  /* update requests of one run, default 4096 */
+    uint64_t max_merge_vertices;  /* vertices of one merged read pass, default 2^20 */
+} lpma_server_config_t;
+
+typedef struct {
+    int status;                   /* lpma_status of the request */
+    int kind;                     /* LPMA_REQ_* */
+    uint64_t applied_updates;     /* update requests applied when served (an update counts itself) */
+    uint64_t version;             /* version of the structure it saw */
+    double queue_us;              /* submission to start of service */
+    double latency_us;            /* submission to completion */
+    uint64_t count;               /* read: vertices; BFS: levels returned (0 or V) */
+    const uint32_t *offsets;      /* read: count + 1 offsets into values */
+    const uint32_t *values;       /* read: neighbor ids; BFS: levels (0xFFFFFFFF unreached) */
+    uint32_t depth;               /* BFS levels */
+    uint64_t edges_scanned;       /* BFS */
+} lpma_server_result_t;
+
+typedef struct {
+    uint64_t submitted[3], completed[3];
+    uint64_t update_runs, read_runs, snapshots, max_concurrent_bfs, admission_waits;
+    uint64_t budget_bytes, in_use_bytes, peak_bytes, applied_updates;
+} lpma_server_stats_t;
+
+void lpma_server_config_default(lpma_server_config_t *cfg);
+int lpma_server_create(lpma_server_t **out, lpma_t *h, const lpma_server_config_t *cfg);
+/* Serves every queued request, then stops. */
+int lpma_server_destroy(lpma_server_t *s);
+int lpma_server_submit_update(lpma_server_t *s, const uint64_t *h_ins, uint64_t n_ins, const uint64_t *h_del,
+                              uint64_t n_del, uint64_t *ticket);
+int lpma_server_submit_read(lpma_server_t *s, const uint32_t *h_vertices, uint64_t n, uint64_t *ticket);
+int lpma_server_submit_bfs(lpma_server_t *s, uint32_t root, int want_levels, uint64_t *ticket);
+/* Blocks until the request is done; the result's arrays stay valid until
+ * lpma_server_release(ticket). */
+int lpma_server_wait(lpma_server_t *s, uint64_t ticket, lpma_server_result_t *out);
+int lpma_server_poll(lpma_server_t *s, uint64_t ticket, int *done, lpma_server_result_t *out);
+int lpma_server_release(lpma_server_t *s, uint64_t ticket);
+int lpma_server_get_stats(lpma_server_t *s, lpma_server_stats_t *out);
 
 /* ---------------------------------------------------------------------------
  * Batched form: many independent PMAs updated by one pipeline run per round.
@@ -273,6 +363,14 @@ int lpma_gen_rmat(uint32_t scale, uint64_t n, double a, double b, double c, uint
 /* R-MAT samples with replacement (repeats kept, self loops rejected): the
  * insert stream of the batched and served workloads. */
 int lpma_gen_rmat_samples(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *out);
+
+/* R-MAT samples generated on the current device into d_out (n keys, with
+ * replacement, no self loops): sample i is a function of (seed, i) only, a
+ * SplitMix64 counter stream (Steele, Lea & Flood 2014) whose 32-bit halves
+ * pick the quadrant of one level each.  For the 2^28- and 2^29-edge
+ * workloads, which the host generators would take minutes to draw. */
+int lpma_gen_rmat_device(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *d_out,
+                         void *stream);
 
 /* Seeded Fisher-Yates shuffle of n keys in place. */
 int lpma_shuffle(uint64_t *keys, uint64_t n, uint64_t seed);

@@ -81,6 +81,30 @@ class Profile(ctypes.Structure):
                 ("graph_launches", ctypes.c_uint64), ("graph_captures", ctypes.c_uint64)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("bfs_workers", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
+                ("max_merge_keys", ctypes.c_uint64), ("max_merge_requests", ctypes.c_uint32),
+                ("max_merge_vertices", ctypes.c_uint64)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("kind", ctypes.c_int), ("applied_updates", ctypes.c_uint64),
+                ("version", ctypes.c_uint64), ("queue_us", ctypes.c_double), ("latency_us", ctypes.c_double),
+                ("count", ctypes.c_uint64), ("offsets", ctypes.POINTER(ctypes.c_uint32)),
+                ("values", ctypes.POINTER(ctypes.c_uint32)), ("depth", ctypes.c_uint32),
+                ("edges_scanned", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64 * 3), ("completed", ctypes.c_uint64 * 3),
+                ("update_runs", ctypes.c_uint64), ("read_runs", ctypes.c_uint64), ("snapshots", ctypes.c_uint64),
+                ("max_concurrent_bfs", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
+                ("budget_bytes", ctypes.c_uint64), ("in_use_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
+                ("applied_updates", ctypes.c_uint64)]
+
+
+REQ_UPDATE, REQ_READ, REQ_BFS = 0, 1, 2
+
 _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
@@ -94,6 +118,8 @@ SIGNATURES = [
     ("lpma_update", _int, [_vp, _vp, _u64, _int, _vp]),
     ("lpma_update_mixed", _int, [_vp, _vp, _u64, _vp, _u64, _vp]),
     ("lpma_bulk_load", _int, [_vp, _vp, _u64, _vp]),
+    ("lpma_update_sequence", _int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_u64), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_u64), _u32, _vp]),
     ("lpma_get_view", _int, [_vp, ctypes.POINTER(View)]),
     ("lpma_copy_view", _int, [_vp, _vp, _vp, _vp]),
     ("lpma_export_compacted", _int, [_vp, _vp, _vp, _vp]),
@@ -117,6 +143,24 @@ SIGNATURES = [
     ("lpma_pool_trim", _int, []),
     ("lpma_pool_reserved_bytes", _u64, []),
     ("lpma_strerror", ctypes.c_char_p, [_int]),
+    ("lpma_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
+    ("lpma_server_create", _int, [ctypes.POINTER(_vp), _vp, ctypes.POINTER(ServerConfig)]),
+    ("lpma_server_destroy", _int, [_vp]),
+    ("lpma_server_submit_update", _int, [_vp, _vp, _u64, _vp, _u64, ctypes.POINTER(_u64)]),
+    ("lpma_server_submit_read", _int, [_vp, _vp, _u64, ctypes.POINTER(_u64)]),
+    ("lpma_server_submit_bfs", _int, [_vp, _u32, _int, ctypes.POINTER(_u64)]),
+    ("lpma_server_wait", _int, [_vp, _u64, ctypes.POINTER(ServerResult)]),
+    ("lpma_server_poll", _int, [_vp, _u64, ctypes.POINTER(_int), ctypes.POINTER(ServerResult)]),
+    ("lpma_server_release", _int, [_vp, _u64]),
+    ("lpma_server_get_stats", _int, [_vp, ctypes.POINTER(ServerStats)]),
+    ("lpma_snapshot_bytes", _u64, [_vp]),
+    ("lpma_snapshot_create", _int, [ctypes.POINTER(_vp), _vp, _vp]),
+    ("lpma_snapshot_destroy", _int, [_vp]),
+    ("lpma_snapshot_get_view", _int, [_vp, ctypes.POINTER(View)]),
+    ("lpma_snapshot_neighbor_count", _int, [_vp, _vp, _u64, _vp, _vp]),
+    ("lpma_snapshot_neighbor_gather", _int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    ("lpma_snapshot_edge_query", _int, [_vp, _vp, _u64, _vp, _vp]),
+    ("lpma_snapshot_bfs", _int, [_vp, _u32, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _vp]),
     ("lpma_group_create", _int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32]),
     ("lpma_group_destroy", _int, [_vp]),
     ("lpma_group_update", _int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_u64), ctypes.POINTER(_vp),
@@ -132,6 +176,7 @@ SIGNATURES = [
     ("lpma_gen_rmat", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
     ("lpma_gen_rmat_samples", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp]),
     ("lpma_shuffle", _int, [_vp, _u64, _u64]),
+    ("lpma_gen_rmat_device", _int, [_u32, _u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp, _vp]),
     ("lpma_gen_absent", _int, [_u32, _vp, _u64, _u64, _u64, _vp]),
     ("lpma_read_edgelist", _int, [ctypes.c_char_p, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _u64, _vp]),
     ("lpma_mt64_fill", _int, [_u64, _u64, _vp]),
@@ -183,6 +228,16 @@ def gen_rmat_samples(scale, n, a=0.57, b=0.19, c=0.19, seed=2):
     out = _np_u64(n)
     _check(lib().lpma_gen_rmat_samples(scale, n, a, b, c, seed, out.ctypes.data), "gen_rmat_samples")
     return out
+
+
+def gen_rmat_device(scale, n, a=0.57, b=0.19, c=0.19, seed=4, device=None, stream=None):
+    """R-MAT samples drawn on the GPU (int64 tensor holding the uint64 keys)."""
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device() if device is None else int(device))
+    out = torch.empty(max(int(n), 1), dtype=torch.int64, device=dev)
+    _check(lib().lpma_gen_rmat_device(scale, int(n), a, b, c, seed, out.data_ptr(), _stream_ptr(stream)),
+           "gen_rmat_device")
+    return out[: int(n)]
 
 
 def shuffle(keys, seed):
@@ -304,6 +359,24 @@ class PMA:
         d = self._dev_keys(dels)
         _check(lib().lpma_update_mixed(self._h, i.data_ptr(), i.numel(), d.data_ptr(), d.numel(),
                                        _stream_ptr(self.stream)), "update_mixed")
+
+    def update_sequence(self, batches):
+        """Applies [(ins, dels), ...] in order as one pipeline run."""
+        n = len(batches)
+        keep = []
+        pi = (ctypes.c_void_p * max(n, 1))()
+        ni = (_u64 * max(n, 1))()
+        pd = (ctypes.c_void_p * max(n, 1))()
+        nd = (_u64 * max(n, 1))()
+        for i, (a, d) in enumerate(batches):
+            for arr, p_, n_ in ((a, pi, ni), (d, pd, nd)):
+                if arr is None or len(arr) == 0:
+                    continue
+                t = self._dev_keys(arr)
+                keep.append(t)
+                p_[i] = t.data_ptr()
+                n_[i] = t.numel()
+        _check(lib().lpma_update_sequence(self._h, pi, ni, pd, nd, n, _stream_ptr(self.stream)), "update_sequence")
 
     def bulk_load(self, keys):
         k = self._dev_keys(keys)
@@ -430,6 +503,61 @@ class PMA:
         _check(lib().lpma_set_launch(self._h, int(seg_slots_per_lane)), "set_launch")
 
 
+class Snapshot:
+    """Read-only copy of a PMA's gapped view at one version (reads on it see
+    that version while the PMA keeps changing)."""
+
+    def __init__(self, pma):
+        h = ctypes.c_void_p()
+        _check(lib().lpma_snapshot_create(ctypes.byref(h), pma._h, _stream_ptr(pma.stream)), "snapshot_create")
+        self._h = h
+        self.device = pma.device
+        self.stream = pma.stream
+        self.num_vertices = pma.num_vertices
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lpma_snapshot_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def view(self):
+        v = View()
+        _check(lib().lpma_snapshot_get_view(self._h, ctypes.byref(v)), "snapshot_view")
+        return v
+
+    def neighbors(self, srcs):
+        torch = _torch()
+        srcs = srcs.to(self.device, torch.int32).contiguous()
+        n = srcs.numel()
+        ln = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_snapshot_neighbor_count(self._h, srcs.data_ptr(), n, ln.data_ptr(),
+                                                  _stream_ptr(self.stream)), "snapshot_neighbor_count")
+        offs = torch.zeros(n + 1, dtype=torch.int64, device=self.device)
+        if n:
+            offs[1:] = torch.cumsum(ln[:n].to(torch.int64), 0)
+        total = int(offs[-1].item())
+        offs32 = offs.to(torch.int32)
+        out = torch.empty(max(total, 1), dtype=torch.int32, device=self.device)
+        _check(lib().lpma_snapshot_neighbor_gather(self._h, srcs.data_ptr(), n, offs32.data_ptr(), out.data_ptr(),
+                                                   _stream_ptr(self.stream)), "snapshot_neighbor_gather")
+        return offs32, out[:total]
+
+    def bfs(self, src):
+        torch = _torch()
+        lv = torch.empty(self.num_vertices, dtype=torch.int32, device=self.device)
+        depth = _u32(0)
+        scanned = _u64(0)
+        _check(lib().lpma_snapshot_bfs(self._h, int(src), lv.data_ptr(), ctypes.byref(depth), ctypes.byref(scanned),
+                                       _stream_ptr(self.stream)), "snapshot_bfs")
+        return lv, int(depth.value), int(scanned.value)
+
+
 class Group:
     """Independent PMAs updated by one pipeline run per round (the batched
     form; the reference has no counterpart).  Each member ends exactly as if
@@ -500,6 +628,82 @@ class Group:
         p = Profile()
         _check(lib().lpma_group_get_profile(self._h, ctypes.byref(p)), "group_profile")
         return p
+
+
+class Server:
+    """The served form over one PMA: submit update batches, neighbor reads
+    and BFS requests from any thread; results carry the number of update
+    requests applied when they were served."""
+
+    def __init__(self, pma, bfs_workers=4, hbm_budget_bytes=0, max_merge_keys=1 << 24, max_merge_requests=4096,
+                 max_merge_vertices=1 << 20):
+        cfg = ServerConfig()
+        lib().lpma_server_config_default(ctypes.byref(cfg))
+        cfg.bfs_workers = bfs_workers
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_merge_keys = max_merge_keys
+        cfg.max_merge_requests = max_merge_requests
+        cfg.max_merge_vertices = max_merge_vertices
+        h = ctypes.c_void_p()
+        _check(lib().lpma_server_create(ctypes.byref(h), pma._h, ctypes.byref(cfg)), "server_create")
+        self._h = h
+        self.pma = pma
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lpma_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_update(self, ins=None, dels=None):
+        a = np.ascontiguousarray(ins if ins is not None else np.empty(0), dtype=np.uint64)
+        d = np.ascontiguousarray(dels if dels is not None else np.empty(0), dtype=np.uint64)
+        t = _u64(0)
+        _check(lib().lpma_server_submit_update(self._h, a.ctypes.data if a.size else None, a.size,
+                                               d.ctypes.data if d.size else None, d.size, ctypes.byref(t)),
+               "submit_update")
+        return t.value
+
+    def submit_read(self, vertices):
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        t = _u64(0)
+        _check(lib().lpma_server_submit_read(self._h, v.ctypes.data if v.size else None, v.size, ctypes.byref(t)),
+               "submit_read")
+        return t.value
+
+    def submit_bfs(self, root, levels=False):
+        t = _u64(0)
+        _check(lib().lpma_server_submit_bfs(self._h, int(root), 1 if levels else 0, ctypes.byref(t)), "submit_bfs")
+        return t.value
+
+    def wait(self, ticket, release=True):
+        r = ServerResult()
+        _check(lib().lpma_server_wait(self._h, int(ticket), ctypes.byref(r)), "server_wait")
+        out = {"status": r.status, "kind": r.kind, "applied": int(r.applied_updates), "version": int(r.version),
+               "queue_us": r.queue_us, "latency_us": r.latency_us, "depth": int(r.depth),
+               "edges_scanned": int(r.edges_scanned), "offsets": None, "values": None}
+        if r.kind == REQ_READ and r.status == 0:
+            k = int(r.count)
+            offs = np.ctypeslib.as_array(r.offsets, shape=(k + 1,)).copy() if r.offsets else np.zeros(k + 1, np.uint32)
+            n = int(offs[-1]) if k + 1 else 0
+            out["offsets"] = offs
+            out["values"] = np.ctypeslib.as_array(r.values, shape=(n,)).copy() if n else np.empty(0, np.uint32)
+        elif r.kind == REQ_BFS and r.status == 0 and r.count:
+            out["values"] = np.ctypeslib.as_array(r.values, shape=(int(r.count),)).copy()
+        if release:
+            lib().lpma_server_release(self._h, int(ticket))
+        return out
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().lpma_server_get_stats(self._h, ctypes.byref(s)), "server_stats")
+        return {f: (list(getattr(s, f)) if f in ("submitted", "completed") else int(getattr(s, f)))
+                for f, _ in ServerStats._fields_}
 
 
 # ------------------------------------------- reference-named free functions

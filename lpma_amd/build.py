@@ -34,8 +34,10 @@ SOURCES = [
     ("update.hip", "hip"),
     ("views.hip", "hip"),
     ("analytics.hip", "hip"),
+    ("gen_dev.hip", "hip"),
     ("lpma.cpp", "hip"),
     ("gen.cpp", "cxx"),
+    ("server.cpp", "hip"),
 ]
 
 

@@ -135,6 +135,8 @@ struct BatchSeg {
     u32 op;
     u32 V;
     u32 vbits;
+    u32 seq;   // request order inside a merged sequence (later wins)
+    u32 pad_;
 };
 
 // Per-round values a graph replay reads from device memory (the host
@@ -147,13 +149,16 @@ struct RoundHdr {
 };
 
 // The PMAs one pipeline run updates.  Sort key of a batch element:
-// pid << pshift | (src << vbits | dst) << 1 | op, pshift = 2 * max vbits + 1.
+// pid << pshift | ((src << vbits | dst) << sbits | seq) << 1 | op,
+// pshift = 2 * max vbits + sbits + 1; seq orders the requests of a merged
+// sequence, so the last element of equal keys is the last request's.
 struct Group {
     const PmaEntry *tab;
     PmaStat *pstat;
     u64 *stats;   // ST_ROWS x ST_NUM counters of this pipeline
     u32 npma;
     u32 pshift;
+    u32 sbits;    // request-order bits below the key (0: one request)
     u32 epoch;    // tags this run's window claims
     u32 log2S;    // segment size, the same for every PMA of the run
     u32 seg_log2G;  // lanes per window of the single-segment rebalance (log2)

@@ -8,8 +8,8 @@ namespace pma {
 u64 scan_scratch_words(u64 n);
 hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
 u64 sort_scratch_words(u64 n);
-u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 lo_bit, u32 hi_bit,
-                   u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
+                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
 
 // update.hip
 struct BatchBufs {
@@ -58,6 +58,9 @@ hipError_t rebuild_pyramid(const DevState &d, bool full, hipStream_t st);
 hipError_t make_markers(u64 *out, u32 V, hipStream_t st);
 hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st);
 hipError_t fill_u64(u64 *p, u64 n, u64 v, hipStream_t st);
+
+// gen_dev.hip
+hipError_t gen_rmat_device(u32 scale, u64 n, double a, double b, double c, u64 seed, u64 *out, hipStream_t st);
 
 // views.hip
 hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64 *edge_keys, u32 *row_std,

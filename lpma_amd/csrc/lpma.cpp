@@ -205,7 +205,7 @@ struct lpma {
 
     Pipeline pl;  // plain updates, views and analytics scratch
     // analytics scratch
-    DBuf<u32> aq0, aq1, alen, acnt;
+    DBuf<u32> aq0;
     DBuf<float> af0, af1, af2, ared;
     u32 *d_err = nullptr;
 
@@ -221,6 +221,18 @@ struct lpma_group {
     std::vector<std::vector<u32>> parts;
     std::vector<Pipeline *> pls;
     u64 calls = 0;
+};
+
+// A read-only copy of a PMA's gapped view (keys, counts, separators, row
+// offsets) at one version: readers use it while writers go on.
+struct lpma_snapshot {
+    int device = 0;
+    DevState st;
+    DBuf<u64> keys, sep;
+    DBuf<u32> cnt, row_off;
+    u64 version = 0, live = 0;
+    u32 V = 0, log2S = 5;
+    hipEvent_t ready = nullptr;  // the copies are done
 };
 
 namespace {
@@ -537,12 +549,19 @@ int respread_into_new(lpma *h, u32 H, const u64 *merged, u64 m, hipStream_t st) 
 }
 
 // One PMA's share of a pipeline run.
-struct Member {
-    lpma *h;
+// One request of a member: inserts and deletes (deletes first).
+struct Req {
     const u64 *ins;
     u64 nins;
     const u64 *del;
     u64 ndel;
+};
+
+// One PMA's share of a pipeline run: its requests, applied in order.
+struct Member {
+    lpma *h;
+    const Req *reqs;
+    u32 nreq;
 };
 
 u32 bits_for(u64 n) {  // bits to hold the values 0 .. n-1
@@ -703,16 +722,23 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     pl.cur_stream = st;
     u64 n = 0, nseg_total = 0;
     bool mixed = false;
-    u32 vmax = 1, nsegs = 0;
+    u32 vmax = 1, nsegs = 0, maxreq = 0;
     for (u32 p = 0; p < npma; ++p) {
         lpma *h = mem[p].h;
         h->cur_stream = st;
         h->calls++;
-        n += mem[p].nins + mem[p].ndel;
-        mixed |= mem[p].nins && mem[p].ndel;
+        bool any_ins = false, any_del = false;
+        for (u32 r = 0; r < mem[p].nreq; ++r) {
+            const Req &q = mem[p].reqs[r];
+            n += q.nins + q.ndel;
+            any_ins |= q.nins > 0;
+            any_del |= q.ndel > 0;
+            nsegs += (q.nins ? 1 : 0) + (q.ndel ? 1 : 0);
+        }
+        mixed |= any_ins && any_del;
+        maxreq = std::max(maxreq, mem[p].nreq);
         vmax = std::max(vmax, h->vbits);
         nseg_total += 1ull << h->st.H;
-        nsegs += (mem[p].nins ? 1 : 0) + (mem[p].ndel ? 1 : 0);
     }
     pl.rounds++;
     if (n == 0) {
@@ -720,17 +746,18 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         return LPMA_OK;
     }
     if (n > 0x7FFFFFF0ull) return LPMA_ERR_TOO_LARGE;
-    const u32 pshift = 2 * vmax + 1;
+    const u32 sbits = bits_for(maxreq);
+    const u32 pshift = 2 * vmax + 1 + sbits;
     const u32 hi_bit = pshift + bits_for(npma);
-    if (hi_bit > 64) return LPMA_ERR_INVALID_ARGUMENT;
-    if (pl.graph_on) return run_pipeline_graph(pl, mem, npma, n, pshift, hi_bit, st);
+    if (hi_bit > 64) return LPMA_ERR_TOO_LARGE;
+    if (pl.graph_on && maxreq <= 1) return run_pipeline_graph(pl, mem, npma, n, pshift, hi_bit, st);
     CHECKED(ensure_batch(pl, n, npma, nsegs, std::min<u64>(n, nseg_total)));
 
     u32 epoch, pyr_blocks;
     CHECKED(begin_epoch(mem, npma, epoch, st));
     CHECKED(upload_table(pl, mem, npma, pyr_blocks, st));
 
-    // batch sources: each member's deletes (op 0), then its inserts (op 1)
+    // batch sources: each request's deletes (op 0), then its inserts (op 1)
     InlineSegs in;
     memset(&in, 0, sizeof(in));
     {
@@ -738,19 +765,24 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         u32 k = 0;
         for (u32 p = 0; p < npma; ++p) {
             lpma *h = mem[p].h;
-            for (int op = 0; op < 2; ++op) {
-                const u64 *keys = op ? mem[p].ins : mem[p].del;
-                u64 cnt = op ? mem[p].nins : mem[p].ndel;
-                if (!cnt) continue;
-                BatchSeg &b = pl.h_segs[k++];
-                b.keys = keys;
-                b.n = cnt;
-                b.off = off;
-                b.pid = p;
-                b.op = (u32)op;
-                b.V = h->V;
-                b.vbits = h->vbits;
-                off += cnt;
+            for (u32 r = 0; r < mem[p].nreq; ++r) {
+                const Req &q = mem[p].reqs[r];
+                for (int op = 0; op < 2; ++op) {
+                    const u64 *keys = op ? q.ins : q.del;
+                    u64 cnt = op ? q.nins : q.ndel;
+                    if (!cnt) continue;
+                    BatchSeg &b = pl.h_segs[k++];
+                    memset(&b, 0, sizeof(b));
+                    b.keys = keys;
+                    b.n = cnt;
+                    b.off = off;
+                    b.pid = p;
+                    b.op = (u32)op;
+                    b.V = h->V;
+                    b.vbits = h->vbits;
+                    b.seq = r;
+                    off += cnt;
+                }
             }
         }
         if (nsegs <= INLINE_SEGS) {
@@ -767,19 +799,21 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.stats = pl.d_stats;
     g.npma = npma;
     g.pshift = pshift;
+    g.sbits = sbits;
     g.epoch = epoch;
     g.log2S = mem[0].h->log2S;
     g.seg_log2G = seg_lanes_log2(g.log2S, pl.seg_slots_log2);
     g.rh = nullptr;
     BatchBufs bb = batch_bufs(pl);
-    // bit 0 of a packed key is the operation: sorted only when a member mixes them
-    const u32 lo_bit = mixed ? 0 : 1;
+    // bit 0 of a packed key is the operation (with the request order above
+    // it): sorted only when a member mixes them or merges requests
+    const u32 lo_bit = (mixed || sbits) ? 0 : 1;
     u64 *pk;
     {
         PhaseTimer t(pl, st, PH_SORT);
         HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, st));
-        pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p, pl.sort_scr.p,
-                           &bb.status->flags, st);
+        pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
+                           pl.sort_scr.p, &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
     }
     {
@@ -846,6 +880,7 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     g.stats = pl.d_stats;
     g.npma = npma;
     g.pshift = pshift;
+    g.sbits = 0;
     g.epoch = 0;
     g.log2S = log2S;
     g.seg_log2G = seg_lanes_log2(log2S, pl.seg_slots_log2);
@@ -885,7 +920,7 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
                       hipSuccess &&
                   batch_begin(g, bb.status, none, pl.segs.p, cs) == hipSuccess;
         if (ok) {
-            u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, hi_bit, pl.pk0.p, pl.pk1.p,
+            u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, 0, hi_bit, pl.pk0.p, pl.pk1.p,
                                     pl.sort_scr.p, &bb.status->flags, cs);
             ok = hipGetLastError() == hipSuccess && locate_and_compact(g, pk, cap, bb, cs) == hipSuccess &&
                  density_check(g, cap, bb, cs) == hipSuccess && elect_windows(g, cap, bb, cs) == hipSuccess &&
@@ -928,10 +963,13 @@ int run_pipeline_graph(Pipeline &pl, const Member *mem, u32 npma, u64 n, u32 psh
     u64 off = 0;
     for (u32 p = 0; p < npma; ++p) {
         lpma *h = mem[p].h;
+        const Req none{nullptr, 0, nullptr, 0};
+        const Req &q = mem[p].nreq ? mem[p].reqs[0] : none;
         for (int op = 0; op < 2; ++op) {
             BatchSeg &b = pl.h_segs[2 * p + op];
-            b.keys = op ? mem[p].ins : mem[p].del;
-            b.n = op ? mem[p].nins : mem[p].ndel;
+            memset(&b, 0, sizeof(b));
+            b.keys = op ? q.ins : q.del;
+            b.n = op ? q.nins : q.ndel;
             b.off = off;
             b.pid = p;
             b.op = (u32)op;
@@ -978,7 +1016,7 @@ int reserve_pipeline(Pipeline &pl, lpma *const *members, u32 npma, u64 max_batch
     // headroom for the live keys to grow before the scratch is reallocated
     CHECKED(prepare_graph_buffers(pl, top, npma, live_sum + live_sum / 4 + max_batch));
     u32 pyr_blocks;
-    Member m0{members[0], nullptr, 0, nullptr, 0};
+    Member m0{members[0], nullptr, 0};
     std::vector<Member> mem(npma, m0);
     for (u32 p = 0; p < npma; ++p) mem[p].h = members[p];
     CHECKED(upload_table(pl, mem.data(), npma, pyr_blocks, st));
@@ -991,7 +1029,8 @@ int reserve_pipeline(Pipeline &pl, lpma *const *members, u32 npma, u64 max_batch
 }
 
 int run_update(lpma *h, const u64 *ins, u64 nins, const u64 *del, u64 ndel, hipStream_t st) {
-    Member m{h, ins, nins, del, ndel};
+    Req q{ins, nins, del, ndel};
+    Member m{h, &q, 1};
     return run_pipeline(h->pl, &m, 1, st);
 }
 
@@ -1170,7 +1209,7 @@ int lpma_destroy(lpma_t *h) {
     pipeline_free(h->pl);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
-    release(h, h->aq0); release(h, h->aq1); release(h, h->alen); release(h, h->acnt);
+    release(h, h->aq0);
     release(h, h->af0); release(h, h->af1); release(h, h->af2); release(h, h->ared);
     if (h->d_err) (void)hipFree(h->d_err);
     delete h;
@@ -1189,6 +1228,23 @@ int lpma_update_mixed(lpma_t *h, const uint64_t *d_ins, uint64_t n_ins, const ui
     if (!h || (!d_ins && n_ins) || (!d_del && n_del)) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     return run_update(h, d_ins, n_ins, d_del, n_del, (hipStream_t)stream);
+}
+
+int lpma_update_sequence(lpma_t *h, const uint64_t *const *d_ins, const uint64_t *n_ins,
+                         const uint64_t *const *d_del, const uint64_t *n_del, uint32_t count, void *stream) {
+    if (!h || (count && !n_ins && !n_del)) return LPMA_ERR_INVALID_ARGUMENT;
+    std::vector<Req> reqs(count);
+    for (u32 i = 0; i < count; ++i) {
+        Req &q = reqs[i];
+        q.nins = n_ins ? n_ins[i] : 0;
+        q.ndel = n_del ? n_del[i] : 0;
+        if ((q.nins && (!d_ins || !d_ins[i])) || (q.ndel && (!d_del || !d_del[i]))) return LPMA_ERR_INVALID_ARGUMENT;
+        q.ins = q.nins ? d_ins[i] : nullptr;
+        q.del = q.ndel ? d_del[i] : nullptr;
+    }
+    DevGuard guard(h->device);
+    Member m{h, reqs.data(), count};
+    return run_pipeline(h->pl, &m, 1, (hipStream_t)stream);
 }
 
 int lpma_bulk_load(lpma_t *h, const uint64_t *d_keys, uint64_t n, void *stream) {
@@ -1285,37 +1341,181 @@ int lpma_check(lpma_t *h, uint64_t *violations, void *stream) {
     return LPMA_OK;
 }
 
+// Level-synchronous BFS over a DevState with scratch from the pool (so
+// concurrent searches over one structure do not share buffers).
+static int bfs_core(int device, const DevState &d, u32 src, u32 *d_levels, u32 *depth, u64 *edges_scanned,
+                    hipStream_t st) {
+    const u64 V = d.V;
+    DBuf<u32> q0, q1, len, cnt, scan;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        release_on(device, q0); release_on(device, q1); release_on(device, len); release_on(device, cnt);
+        release_on(device, scan);
+    };
+    int r;
+    if ((r = ensure_on(device, st, q0, V + 1)) || (r = ensure_on(device, st, q1, V + 1)) ||
+        (r = ensure_on(device, st, len, V + 2)) || (r = ensure_on(device, st, scan, scan_scratch_words(V + 1) + 8)) ||
+        (r = ensure_on(device, st, cnt, 4))) {
+        cleanup();
+        return r;
+    }
+    if (bfs_init(d_levels, d.V, src, q0.p, cnt.p, st) != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        return LPMA_ERR_DEVICE;
+    }
+    u32 *q = q0.p, *qn_buf = q1.p;
+    u32 qn = 1, level = 0;
+    u64 scanned = 0;
+    u32 c[2];
+    int rc = LPMA_OK;
+    while (qn) {
+        if (bfs_level(d, q, qn, len.p, scan.p, d_levels, level, qn_buf, cnt.p, st) != hipSuccess ||
+            hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess || fill_u32(cnt.p, 2, 0, st) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = LPMA_ERR_DEVICE;
+            break;
+        }
+        scanned += c[1];
+        qn = c[0];
+        std::swap(q, qn_buf);
+        ++level;
+    }
+    cleanup();
+    if (rc == LPMA_OK) {
+        if (depth) *depth = level;
+        if (edges_scanned) *edges_scanned = scanned;
+    }
+    return rc;
+}
+
 int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned, void *stream) {
     if (!h || !d_levels) return LPMA_ERR_INVALID_ARGUMENT;
     if (src >= h->V) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
+    return bfs_core(h->device, h->st, src, d_levels, depth, edges_scanned, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- snapshots
+
+uint64_t lpma_snapshot_bytes(const lpma_t *h) {
+    if (!h) return 0;
+    u64 nseg = 1ull << h->st.H;
+    return (nseg << h->log2S) * 8 + nseg * 12 + ((u64)h->V + 1) * 4;
+}
+
+int lpma_snapshot_create(lpma_snapshot_t **out, lpma_t *h, void *stream) {
+    if (!out || !h) return LPMA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    DevGuard guard(h->device);
     hipStream_t st = (hipStream_t)stream;
-    h->cur_stream = st;
-    const u64 V = h->V;
-    CHECKED(ensure(h, h->aq0, V + 1));
-    CHECKED(ensure(h, h->aq1, V + 1));
-    CHECKED(ensure(h, h->alen, V + 2));
-    h->pl.cur_stream = st;
-    CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(V + 1) + 8));
-    CHECKED(ensure(h, h->acnt, 4));
-    HIPCHECK(bfs_init(d_levels, h->V, src, h->aq0.p, h->acnt.p, st));
-    u32 *q = h->aq0.p, *qn_buf = h->aq1.p;
-    u32 qn = 1, level = 0;
-    u64 scanned = 0;
-    u32 cnt[2];
-    while (qn) {
-        HIPCHECK(bfs_level(h->st, q, qn, h->alen.p, h->pl.scan_scr.p, d_levels, level, qn_buf, h->acnt.p, st));
-        HIPCHECK(hipMemcpyAsync(cnt, h->acnt.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        HIPCHECK(fill_u32(h->acnt.p, 2, 0, st));
-        scanned += cnt[1];
-        qn = cnt[0];
-        std::swap(q, qn_buf);
-        ++level;
+    lpma_snapshot *s = new (std::nothrow) lpma_snapshot();
+    if (!s) return LPMA_ERR_OUT_OF_MEMORY;
+    s->device = h->device;
+    const u64 nseg = 1ull << h->st.H, cap = nseg << h->log2S;
+    int r;
+    if ((r = ensure_on(h->device, st, s->keys, cap)) || (r = ensure_on(h->device, st, s->sep, nseg)) ||
+        (r = ensure_on(h->device, st, s->cnt, nseg)) || (r = ensure_on(h->device, st, s->row_off, (u64)h->V + 1))) {
+        lpma_snapshot_destroy(s);
+        return r;
     }
-    if (depth) *depth = level;
-    if (edges_scanned) *edges_scanned = scanned;
+    if (hipMemcpyAsync(s->keys.p, h->st.keys, cap * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->sep.p, h->st.sep, nseg * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->cnt.p, h->st.cnt, nseg * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(s->row_off.p, h->st.row_off, ((u64)h->V + 1) * 4, hipMemcpyDeviceToDevice, st) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        lpma_snapshot_destroy(s);
+        return LPMA_ERR_DEVICE;
+    }
+    s->st = h->st;
+    s->st.keys = s->keys.p;
+    s->st.sep = s->sep.p;
+    s->st.cnt = s->cnt.p;
+    s->st.row_off = s->row_off.p;
+    s->st.pyr = nullptr;
+    s->st.owner = nullptr;
+    s->version = h->version;
+    s->live = h->live;
+    s->V = h->V;
+    s->log2S = h->log2S;
+    if (hipEventCreateWithFlags(&s->ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(s->ready, st) != hipSuccess) {
+        (void)hipGetLastError();
+        lpma_snapshot_destroy(s);
+        return LPMA_ERR_DEVICE;
+    }
+    *out = s;
     return LPMA_OK;
+}
+
+// reads on any stream start after the snapshot's copies
+static int snapshot_wait(lpma_snapshot *s, hipStream_t st) {
+    HIPCHECK(hipStreamWaitEvent(st, s->ready, 0));
+    return LPMA_OK;
+}
+
+int lpma_snapshot_destroy(lpma_snapshot_t *s) {
+    if (!s) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(s->device);
+    if (s->ready) {
+        (void)hipEventSynchronize(s->ready);
+        (void)hipEventDestroy(s->ready);
+    }
+    release_on(s->device, s->keys); release_on(s->device, s->sep); release_on(s->device, s->cnt);
+    release_on(s->device, s->row_off);
+    delete s;
+    return LPMA_OK;
+}
+
+int lpma_snapshot_get_view(const lpma_snapshot_t *s, lpma_view_t *out) {
+    if (!s || !out) return LPMA_ERR_INVALID_ARGUMENT;
+    out->keys = s->st.keys;
+    out->row_offsets = s->st.row_off;
+    out->capacity = (u64)1 << (s->st.H + s->log2S);
+    out->live_keys = s->live;
+    out->edges = s->live - s->V;
+    out->num_vertices = s->V;
+    out->segment_slots = 1u << s->log2S;
+    out->tree_height = s->st.H;
+    out->version = s->version;
+    return LPMA_OK;
+}
+
+int lpma_snapshot_neighbor_count(lpma_snapshot_t *s, const uint32_t *d_srcs, uint64_t n, uint32_t *d_len,
+                                 void *stream) {
+    if (!s || ((!d_srcs || !d_len) && n)) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(s->device);
+    CHECKED(snapshot_wait(s, (hipStream_t)stream));
+    HIPCHECK(neighbor_count(s->st, d_srcs, n, d_len, (hipStream_t)stream));
+    return LPMA_OK;
+}
+
+int lpma_snapshot_neighbor_gather(lpma_snapshot_t *s, const uint32_t *d_srcs, uint64_t n, const uint32_t *d_offsets,
+                                  uint32_t *d_out, void *stream) {
+    if (!s || ((!d_srcs || !d_offsets || !d_out) && n)) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(s->device);
+    CHECKED(snapshot_wait(s, (hipStream_t)stream));
+    HIPCHECK(neighbor_gather(s->st, d_srcs, n, d_offsets, d_out, (hipStream_t)stream));
+    return LPMA_OK;
+}
+
+int lpma_snapshot_edge_query(lpma_snapshot_t *s, const uint64_t *d_keys, uint64_t n, uint32_t *d_result,
+                             void *stream) {
+    if (!s || ((!d_keys || !d_result) && n)) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(s->device);
+    CHECKED(snapshot_wait(s, (hipStream_t)stream));
+    HIPCHECK(edge_query(s->st, d_keys, n, d_result, (hipStream_t)stream));
+    return LPMA_OK;
+}
+
+int lpma_snapshot_bfs(lpma_snapshot_t *s, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned,
+                      void *stream) {
+    if (!s || !d_levels || src >= s->V) return LPMA_ERR_INVALID_ARGUMENT;
+    DevGuard guard(s->device);
+    CHECKED(snapshot_wait(s, (hipStream_t)stream));
+    return bfs_core(s->device, s->st, src, d_levels, depth, edges_scanned, (hipStream_t)stream);
 }
 
 int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, float *d_rank, uint32_t *iterations,
@@ -1500,17 +1700,17 @@ int lpma_group_update(lpma_group_t *g, const uint64_t *const *d_ins, const uint6
     DevGuard guard(g->device);
     g->calls++;
     std::vector<Member> mem;
+    std::vector<Req> reqs(n);
+    for (u32 i = 0; i < n; ++i) {
+        Req &q = reqs[i];
+        q.nins = n_ins ? n_ins[i] : 0;
+        q.ndel = n_del ? n_del[i] : 0;
+        q.ins = q.nins ? d_ins[i] : nullptr;
+        q.del = q.ndel ? d_del[i] : nullptr;
+    }
     for (size_t k = 0; k < g->parts.size(); ++k) {
         mem.clear();
-        for (u32 i : g->parts[k]) {
-            Member m;
-            m.h = g->members[i];
-            m.nins = n_ins ? n_ins[i] : 0;
-            m.ndel = n_del ? n_del[i] : 0;
-            m.ins = m.nins ? d_ins[i] : nullptr;
-            m.del = m.ndel ? d_del[i] : nullptr;
-            mem.push_back(m);
-        }
+        for (u32 i : g->parts[k]) mem.push_back(Member{g->members[i], &reqs[i], 1});
         CHECKED(run_pipeline(*g->pls[k], mem.data(), (u32)mem.size(), (hipStream_t)stream));
     }
     return LPMA_OK;
@@ -1570,6 +1770,14 @@ int lpma_group_get_profile(lpma_group_t *g, lpma_profile_t *out) {
     DevGuard guard(g->device);
     memset(out, 0, sizeof(*out));
     for (Pipeline *pl : g->pls) add_profile(*pl, out);
+    return LPMA_OK;
+}
+
+int lpma_gen_rmat_device(uint32_t scale, uint64_t n, double a, double b, double c, uint64_t seed, uint64_t *d_out,
+                         void *stream) {
+    if ((!d_out && n) || scale < 1 || scale > 31) return LPMA_ERR_INVALID_ARGUMENT;
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return LPMA_ERR_INVALID_ARGUMENT;
+    HIPCHECK(gen_rmat_device(scale, n, a, b, c, seed, d_out, (hipStream_t)stream));
     return LPMA_OK;
 }
 

@@ -39,7 +39,7 @@ __device__ __forceinline__ u32 source_of(const BatchSeg *__restrict__ segs, u32 
 // the first radix digit of every tile.
 template <bool HIST>
 __global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
-                                                        const u64 *n_dev, u32 pshift, u64 *__restrict__ out,
+                                                        const u64 *n_dev, u32 pshift, u32 sbits, u64 *__restrict__ out,
                                                         u32 *__restrict__ flags, u32 shift, u32 items,
                                                         u32 *__restrict__ hist, u32 nblocks, int block_major) {
     __shared__ u32 h[RS_RADIX];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restric
             u64 k = sg.keys[i - sg.off];
             u32 src = (u32)(k >> 32), dst = (u32)k;
             if (src >= sg.V || dst >= sg.V) { bad = true; src = 0; dst = 0; }
-            u64 p = ((u64)sg.pid << pshift) | (((((u64)src) << sg.vbits) | dst) << 1) | sg.op;
+            u64 p = ((u64)sg.pid << pshift) | ((((((u64)src) << sg.vbits) | dst) << sbits | sg.seq) << 1) | sg.op;
             out[i] = p;
             if (HIST) atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
         }
@@ -312,13 +312,13 @@ u64 sort_scratch_words(u64 n) {
 // keys on bits [lo_bit, hi_bit): the first histogram is taken while packing.
 // With n_dev the kernels read the element count from device memory and n
 // is only a capacity that sizes the grids (graph replays).
-u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 lo_bit, u32 hi_bit,
-                   u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
+u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
+                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
     if (n == 0) return k0;
     if (n <= (u64)RS_SMALL_CAP) {
         const u32 items = 4;
         k_pack_group<false><<<(u32)((n + items * RS_TPB - 1) / (items * RS_TPB)), RS_TPB, 0, st>>>(
-            segs, nsegs, n, n_dev, pshift, k0, flags, 0, items, nullptr, 0, 0);
+            segs, nsegs, n, n_dev, pshift, sbits, k0, flags, 0, items, nullptr, 0, 0);
         if (n > 1 || n_dev) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, n_dev, lo_bit, hi_bit);
         return k0;
     }
@@ -331,8 +331,8 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
     u64 *src = k0, *dst = k1;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
         if (shift == lo_bit)
-            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, k0, flags, shift, items, hist,
-                                                       nblk, block_major);
+            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, k0, flags, shift, items,
+                                                       hist, nblk, block_major);
         else
             k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, n_dev, shift, items, hist, nblk, block_major);
         if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);

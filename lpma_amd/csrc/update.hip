@@ -46,8 +46,9 @@ hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, Bat
 
 // ------------------------------------------------ dedup, locate, presence
 
-// Dedup (keep the last of equal keys: deletes sort before inserts), route
-// each key to its segment, and resolve it against the keys present there.
+// Dedup (keep the last of equal keys: later requests sort after earlier
+// ones, and within a request deletes before inserts), route each key to
+// its segment, and resolve it against the keys present there.
 __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restrict__ pk, u64 n,
                                                     u64 *__restrict__ ukey, u64 *__restrict__ useg,
                                                     i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
@@ -57,13 +58,14 @@ __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restri
     i32 eff = 0, dl = 0;
     if (i < n) {
         const u64 p = pk[i];
-        const bool dup = (i + 1 < n) && ((pk[i + 1] >> 1) == (p >> 1));
+        const u32 ks = 1 + g.sbits;  // key (and pid) above the request order and op
+        const bool dup = (i + 1 < n) && ((pk[i + 1] >> ks) == (p >> ks));
         if (dup) {
             useg[i] = RK_NONE;
         } else {
             const u32 pid = (u32)(p >> g.pshift);
             const DevState d = g.tab[pid].d;
-            const u64 kk = (p & pmask) >> 1;
+            const u64 kk = (p & pmask) >> ks;
             const u32 op = (u32)(p & 1);
             const u64 key = ((kk >> d.vbits) << 32) | (kk & ((1ull << d.vbits) - 1));
             const u32 s = locate_segment(d.sep, d.H, key);

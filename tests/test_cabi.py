@@ -59,6 +59,14 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.lpma_group_update(None, None, None, None, None, None) == 1
     assert lib.lpma_group_get_stats(None, None) == 1
     assert lib.lpma_group_partitions(None) == 0
+    assert lib.lpma_server_create(None, None, None) == 1
+    assert lib.lpma_server_destroy(None) == 1
+    assert lib.lpma_server_submit_bfs(None, 0, 0, None) == 1
+    assert lib.lpma_server_wait(None, 1, None) == 1
+    assert lib.lpma_snapshot_create(None, None, None) == 1
+    assert lib.lpma_snapshot_destroy(None) == 1
+    assert lib.lpma_snapshot_bytes(None) == 0
+    assert lib.lpma_update_sequence(None, None, None, None, None, 0, None) == 1
 
 
 def test_mt19937_64_known_answers():

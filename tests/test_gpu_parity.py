@@ -261,3 +261,53 @@ def test_config1_full_size():
         t.compare(layout=False)
     assert t.pma.edges == 1_000_000
     t.close()
+
+
+def net_batch(batches):
+    """The single mixed batch equal to applying `batches` in order: per key,
+    the last request touching it decides (inside a request, inserts after
+    deletes)."""
+    keys, order = [], []
+    for r, (ins, dels) in enumerate(batches):
+        for arr, op in ((dels, 0), (ins, 1)):
+            arr = np.asarray(arr, dtype=np.uint64)
+            keys.append(arr)
+            order.append(np.full(arr.size, 2 * r + op, dtype=np.int64))
+    keys = np.concatenate(keys)
+    order = np.concatenate(order)
+    idx = np.lexsort((order, keys))
+    k, o = keys[idx], order[idx]
+    last = np.r_[k[1:] != k[:-1], True]
+    k, o = k[last], o[last]
+    return k[(o & 1) == 1], k[(o & 1) == 0]
+
+
+@pytest.mark.parametrize("nreq", [2, 7, 40])
+def test_update_sequence_equals_sequential_application(nreq):
+    """A merged sequence of mixed batches with overlapping keys: the layout
+    equals the model after the net batch, and the key set equals the
+    batches applied one by one on a twin."""
+    say(f"update sequence of {nreq}")
+    rng = np.random.default_rng(40 + nreq)
+    V = 300
+    t = Triple(V)
+    t.insert(rand_keys(rng, V, 4000))
+    twin = L.PMA(V)
+    twin.insert(t.ref)
+    for rep in range(4):
+        pool = np.concatenate([t.ref[rng.integers(0, t.ref.size, 200)], rand_keys(rng, V, 200)])
+        batches = []
+        for r in range(nreq):
+            ins = pool[rng.integers(0, pool.size, int(rng.integers(0, 120)))]
+            dels = pool[rng.integers(0, pool.size, int(rng.integers(0, 120)))]
+            batches.append((ins, dels))
+        t.pma.update_sequence(batches)
+        for ins, dels in batches:
+            twin.update_mixed(ins, dels)
+        ni, nd = net_batch(batches)
+        t.model.update_mixed(ni, nd)
+        t.ref = oracle.apply(t.ref, ni, nd)
+        t.compare()
+        assert np.array_equal(L.to_host_u64(twin.export()[0]), t.ref)
+    twin.close()
+    t.close()
