@@ -321,10 +321,215 @@ def run_batched(args):
     print(json.dumps(out))
 
 
+def served_requests(rng, rate, duration, V, ins_pool, del_pool, mean_batch):
+    """Poisson arrivals at `rate`/s for `duration` s: 60% update batches
+    (lognormal size with the given mean, sigma 1, half inserts from the
+    fresh-edge pool, half deletes drawn from the initial edges), 35%
+    neighbor reads of 1-1024 uniform vertices, 5% BFS from a uniform root."""
+    n = max(1, int(rate * duration))
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    kinds = rng.choice(3, n, p=(0.60, 0.35, 0.05))
+    sigma = 1.0
+    mu = np.log(mean_batch) - sigma * sigma / 2
+    reqs = []
+    for i in range(n):
+        if kinds[i] == 0:
+            k = max(2, int(rng.lognormal(mu, sigma)))
+            a = ins_pool[rng.integers(0, ins_pool.size, k // 2)]
+            d = del_pool[rng.integers(0, del_pool.size, k - k // 2)]
+            reqs.append((t[i], "update", a, d))
+        elif kinds[i] == 1:
+            reqs.append((t[i], "read", rng.integers(0, V, int(rng.integers(1, 1025))).astype(np.uint32)))
+        else:
+            reqs.append((t[i], "bfs", int(rng.integers(0, V))))
+    return reqs
+
+
+def serve_trace(pma, reqs, workers):
+    """The served form: requests submitted at their arrival times to the
+    server (continuous batching, BFS on snapshots beside the writer).
+    Returns [(kind, result, payload, arrival_s)] with result latency in us."""
+    import lpma_amd as L
+    srv = L.Server(pma, bfs_workers=workers)
+    tickets = []
+    t0 = time.perf_counter()
+    for (at, kind, *payload) in reqs:
+        while time.perf_counter() - t0 < at:
+            pass
+        if kind == "update":
+            tickets.append((kind, srv.submit_update(payload[0], payload[1]), payload, at))
+        elif kind == "read":
+            tickets.append((kind, srv.submit_read(payload[0]), payload, at))
+        else:
+            tickets.append((kind, srv.submit_bfs(payload[0]), payload, at))
+    results = [(kind, srv.wait(t, release=True), payload, at) for kind, t, payload, at in tickets]
+    st = srv.stats()
+    srv.close()
+    return results, st
+
+
+def plain_trace(pma, reqs):
+    """The plain form: one loop serving each request alone, in arrival order,
+    on the live structure (one update call per batch, reads and BFS through
+    the handle); a request waits while earlier ones run."""
+    import torch
+    import lpma_amd as L
+    results = []
+    t0 = time.perf_counter()
+    applied = 0
+    for (at, kind, *payload) in reqs:
+        while time.perf_counter() - t0 < at:
+            pass
+        start = time.perf_counter() - t0
+        if kind == "update":
+            pma.update_mixed(L.to_device(payload[0]), L.to_device(payload[1]))
+            applied += 1
+        elif kind == "read":
+            offs, vals = pma.neighbors(torch.from_numpy(payload[0].astype(np.int32)))
+        else:
+            pma.bfs(payload[0])
+        torch.cuda.synchronize()
+        done = time.perf_counter() - t0
+        results.append((kind, {"status": 0, "latency_us": (done - at) * 1e6, "queue_us": (start - at) * 1e6,
+                               "applied": applied}, payload, at))
+    return results
+
+
+def summarize_trace(results, st=None):
+    lat = {k: np.array([r["latency_us"] for kind, r, _, _ in results if kind == k]) for k in ("update", "read", "bfs")}
+    upd = [(at + r["latency_us"] / 1e6, p[0].size + p[1].size) for kind, r, p, at in results if kind == "update"]
+    first = min(at for _, _, _, at in results)
+    last_upd = max(d for d, _ in upd) if upd else first + 1e-9
+    row = {"requests": len(results), "failed": sum(1 for _, r, _, _ in results if r["status"] != 0),
+           "update_requests": len(upd), "sustained_updates_per_s": sum(k for _, k in upd) / (last_upd - first),
+           "last_update_done_s": last_upd - first,
+           "last_done_s": max(at + r["latency_us"] / 1e6 for _, r, _, at in results) - first}
+    for k, a in lat.items():
+        if a.size:
+            row[f"{k}_p50_ms"] = float(np.percentile(a, 50)) / 1e3
+            row[f"{k}_p99_ms"] = float(np.percentile(a, 99)) / 1e3
+    if st is not None:
+        row.update({"update_runs": st["update_runs"], "read_runs": st["read_runs"], "snapshots": st["snapshots"],
+                    "max_concurrent_bfs": st["max_concurrent_bfs"], "peak_admitted_bytes": st["peak_bytes"],
+                    "engine_ms": {"updates": st["update_us"] / 1e3, "reads": st["read_us"] / 1e3,
+                                  "admission": st["dispatch_us"] / 1e3, "update_staging": st["stage_us"] / 1e3,
+                                  "update_call": st["pipeline_us"] / 1e3, "update_sync": st["finish_us"] / 1e3}})
+    return row
+
+
+def run_served(args):
+    import torch
+    import lpma_amd as L
+    L.lib()
+    scale, samples = args.served_scale, args.served_samples
+    V = 1 << scale
+    t = time.time()
+    keys = L.gen_rmat_device(scale, samples, 0.57, 0.19, 0.19, 4)
+    pma = L.PMA(V)
+    pma.bulk_load(keys)
+    torch.cuda.synchronize()
+    E = pma.edges
+    log(f"served: {samples} R-MAT samples on 2^{scale} vertices -> {E} edges, bulk load {time.time() - t:.1f}s")
+    rng = np.random.default_rng(44)
+    ins_pool = L.to_host_u64(L.gen_rmat_device(scale, 1 << 24, 0.57, 0.19, 0.19, 4004))
+    del_pool = L.to_host_u64(keys[torch.randint(0, keys.numel(), (1 << 24,), device=keys.device)])
+    rates = [float(x) for x in args.rates.split(",")]
+    sweep = {}
+    checked = mismatched = 0
+    history = []  # every update request applied so far, in order
+    for mode in args.served_modes.split(","):
+        rows = []
+        for rate in rates:
+            reqs = served_requests(rng, rate, args.duration, V, ins_pool, del_pool, 1e4)
+            if mode == "served":
+                results, st = serve_trace(pma, reqs, args.bfs_workers)
+                row = summarize_trace(results, st)
+            else:
+                results = plain_trace(pma, reqs)
+                row = summarize_trace(results)
+            row["offered_req_per_s"] = rate
+            rows.append(row)
+            log(f"{mode} @ {rate:g} req/s: {row['sustained_updates_per_s']:.3e} updates/s, "
+                + ", ".join(f"{k} p50/p99 {row.get(k + '_p50_ms', 0):.2f}/{row.get(k + '_p99_ms', 0):.2f} ms"
+                            for k in ("update", "read", "bfs")) + f", all done at {row['last_done_s']:.2f}s"
+                + (f", {row['update_runs']} update runs, {row['snapshots']} snapshots, engine ms {row['engine_ms']}"
+                   if "update_runs" in row else ""))
+            if mode == "served" and rate == rates[-1]:
+                checked, mismatched = verify_served_reads(keys, history, reqs, results, V, args.verify_reads)
+            history += [(r[2], r[3]) for r in reqs if r[1] == "update"]
+        sweep[mode] = rows
+    top = sweep["served"][-1]
+    base = sweep["plain"][-1] if "plain" in sweep else top
+    out = {
+        "metric": "served form: sustained edge updates/s with p50/p99 latency per request type (R-MAT 2^24 V, 2^28 samples)",
+        "value": top["sustained_updates_per_s"],
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 0,
+        "ms_per_step": top["last_done_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (device R-MAT generator, Poisson arrivals)",
+        "config": {"workload": "served_rmat24_poisson_mix_60upd_35read_5bfs", "vertices": V, "samples": samples,
+                   "edges": int(E), "duration_s_per_rate": args.duration, "rates_req_per_s": rates,
+                   "bfs_workers": args.bfs_workers, "update_mean_batch": 1e4, "parallelism": "single GPU"},
+        "sweep": sweep,
+        "speedup_vs_plain": top["sustained_updates_per_s"] / base["sustained_updates_per_s"],
+        "plain_form": "one loop serving each request alone in arrival order on the live structure",
+        "reads_checked_against_sequential_replay": checked,
+        "reads_mismatched": mismatched,
+    }
+    print(json.dumps(out))
+
+
+def verify_served_reads(keys, history, reqs, results, V, n_check):
+    """Rebuilds the state the trace started from on a replica (the initial
+    graph, then every earlier update request one at a time), replays the
+    trace's update requests one at a time, and compares sampled reads with
+    the replica's neighbor lists after as many updates as the server
+    reported for them."""
+    import torch
+    import lpma_amd as L
+    if n_check <= 0:
+        return 0, 0
+    updates = [(r[2], r[3]) for r in reqs if r[1] == "update"]
+    reads = [(res["applied"], req[2], res) for req, (kind, res, _, _) in zip(reqs, results) if kind == "read"]
+    rng = np.random.default_rng(5)
+    pick = sorted(rng.choice(len(reads), min(n_check, len(reads)), replace=False), key=lambda i: reads[i][0])
+    rep = L.PMA(V)
+    rep.bulk_load(keys)
+    for a, b in history:
+        rep.update_mixed(L.to_device(a), L.to_device(b))
+    applied = 0
+    bad = 0
+    for i in pick:
+        a_cnt, verts, res = reads[i]
+        while applied < a_cnt:
+            ins, dels = updates[applied]
+            rep.update_mixed(L.to_device(ins), L.to_device(dels))
+            applied += 1
+        offs, vals = rep.neighbors(torch.from_numpy(verts.astype(np.int32)))
+        if not (np.array_equal(offs.cpu().numpy().view(np.uint32), res["offsets"]) and
+                np.array_equal(vals.cpu().numpy().view(np.uint32), res["values"])):
+            bad += 1
+    rep.close()
+    return len(pick), bad
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("headline", "batched"), default="headline")
+    ap.add_argument("--workload", choices=("headline", "batched", "served"), default="headline")
+    ap.add_argument("--served-scale", type=int, default=24)
+    ap.add_argument("--served-samples", type=int, default=1 << 28)
+    ap.add_argument("--rates", default="500,1000,2000,4000", help="served: offered requests/s to sweep")
+    ap.add_argument("--duration", type=float, default=1.5, help="served: seconds of arrivals per rate")
+    ap.add_argument("--bfs-workers", type=int, default=2)
+    ap.add_argument("--verify-reads", type=int, default=64)
+    ap.add_argument("--served-modes", default="served,plain")
     ap.add_argument("--graph", type=int, default=0, help="--forms batched: replay rounds as hipGraphs")
     ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
     ap.add_argument("--tune", type=int, default=1, help="batched workload: autotune segment size / density / shape")
@@ -344,6 +549,8 @@ def main():
               "sharded by key range across GPUs only by design (no multi-GPU run is part of this build).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "served":
+        return run_served(args)
     if args.workload == "batched":
         args.steps = 95 if args.steps is None else args.steps
         args.warmup = 5 if args.warmup is None else args.warmup

@@ -205,6 +205,11 @@ int lpma_snapshot_edge_query(lpma_snapshot_t *s, const uint64_t *d_keys, uint64_
                              void *stream);
 int lpma_snapshot_bfs(lpma_snapshot_t *s, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned,
                       void *stream);
+/* Workgroups a search on this snapshot expands its frontier with (0: the
+ * default 2048, enough for the whole device).  Concurrent searches beside a
+ * latency-sensitive writer use fewer, so the writer's kernels find free
+ * compute units. */
+int lpma_snapshot_set_search_width(lpma_snapshot_t *s, uint32_t workgroups);
 
 /* PageRank over the gapped layout with nvGraph's formulation
  * (include/pagerank.hxx:66-88): from the uniform vector, until the L1 change
@@ -259,8 +264,9 @@ const char *lpma_strerror(int code);
  * submission) and served with continuous batching: all update requests
  * queued when the engine comes round are applied as one pipeline run in
  * submission order, all queued neighbor reads as one pass over the live
- * structure after them, and each BFS on a snapshot of the version current
- * at its admission, on a worker stream beside the engine.  The device memory
+ * structure after them, and each BFS on a snapshot (the newest one, taken
+ * at most snapshot_max_age_us earlier), on a worker stream beside the
+ * engine.  The device memory
  * each request needs is charged against a byte budget; requests wait while
  * it has no room.  While a server runs, its handle must not be used
  * directly. */
@@ -269,11 +275,16 @@ typedef struct lpma_server lpma_server_t;
 enum { LPMA_REQ_UPDATE = 0, LPMA_REQ_READ = 1, LPMA_REQ_BFS = 2 };
 
 typedef struct {
-    uint32_t bfs_workers;         /* searches running at once (streams), default 4 */
+    uint32_t bfs_workers;         /* searches running at once, default 2: with the engine's stream and
+                                     the caller's they fill the 4 hardware queues HIP maps streams to */
     uint64_t hbm_budget_bytes;    /* 0: 90% of the device memory free at creation */
     uint64_t max_merge_keys;      /* keys of one merged update run, default 2^24 */
     uint32_t max_merge_requests;  /* update requests of one run, default 4096 */
     uint64_t max_merge_vertices;  /* vertices of one merged read pass, default 2^20 */
+    uint64_t snapshot_max_age_us; /* a BFS reuses the newest snapshot while younger, default 50000;
+                                     0: every BFS sees the version current at its admission */
+    uint32_t search_workgroups;   /* workgroups of each search's frontier expansion, default 256 of
+                                     the 2048 a lone search uses, so searches leave room for the writer */
 } lpma_server_config_t;
 
 typedef struct {
@@ -294,6 +305,8 @@ typedef struct {
     uint64_t submitted[3], completed[3];
     uint64_t update_runs, read_runs, snapshots, max_concurrent_bfs, admission_waits;
     uint64_t budget_bytes, in_use_bytes, peak_bytes, applied_updates;
+    double update_us, read_us, dispatch_us;  /* engine time serving updates, reads, admitting searches */
+    double stage_us, pipeline_us, finish_us; /* update service: staging copies, the update call, the final sync */
 } lpma_server_stats_t;
 
 void lpma_server_config_default(lpma_server_config_t *cfg);

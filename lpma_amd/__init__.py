@@ -84,7 +84,8 @@ class Profile(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("bfs_workers", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("max_merge_keys", ctypes.c_uint64), ("max_merge_requests", ctypes.c_uint32),
-                ("max_merge_vertices", ctypes.c_uint64)]
+                ("max_merge_vertices", ctypes.c_uint64), ("snapshot_max_age_us", ctypes.c_uint64),
+                ("search_workgroups", ctypes.c_uint32)]
 
 
 class ServerResult(ctypes.Structure):
@@ -100,7 +101,9 @@ class ServerStats(ctypes.Structure):
                 ("update_runs", ctypes.c_uint64), ("read_runs", ctypes.c_uint64), ("snapshots", ctypes.c_uint64),
                 ("max_concurrent_bfs", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
                 ("budget_bytes", ctypes.c_uint64), ("in_use_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
-                ("applied_updates", ctypes.c_uint64)]
+                ("applied_updates", ctypes.c_uint64), ("update_us", ctypes.c_double), ("read_us", ctypes.c_double),
+                ("dispatch_us", ctypes.c_double), ("stage_us", ctypes.c_double), ("pipeline_us", ctypes.c_double),
+                ("finish_us", ctypes.c_double)]
 
 
 REQ_UPDATE, REQ_READ, REQ_BFS = 0, 1, 2
@@ -161,6 +164,7 @@ SIGNATURES = [
     ("lpma_snapshot_neighbor_gather", _int, [_vp, _vp, _u64, _vp, _vp, _vp]),
     ("lpma_snapshot_edge_query", _int, [_vp, _vp, _u64, _vp, _vp]),
     ("lpma_snapshot_bfs", _int, [_vp, _u32, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64), _vp]),
+    ("lpma_snapshot_set_search_width", _int, [_vp, _u32]),
     ("lpma_group_create", _int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32]),
     ("lpma_group_destroy", _int, [_vp]),
     ("lpma_group_update", _int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_u64), ctypes.POINTER(_vp),
@@ -635,8 +639,8 @@ class Server:
     and BFS requests from any thread; results carry the number of update
     requests applied when they were served."""
 
-    def __init__(self, pma, bfs_workers=4, hbm_budget_bytes=0, max_merge_keys=1 << 24, max_merge_requests=4096,
-                 max_merge_vertices=1 << 20):
+    def __init__(self, pma, bfs_workers=2, hbm_budget_bytes=0, max_merge_keys=1 << 24, max_merge_requests=4096,
+                 max_merge_vertices=1 << 20, snapshot_max_age_us=50000, search_workgroups=256):
         cfg = ServerConfig()
         lib().lpma_server_config_default(ctypes.byref(cfg))
         cfg.bfs_workers = bfs_workers
@@ -644,6 +648,8 @@ class Server:
         cfg.max_merge_keys = max_merge_keys
         cfg.max_merge_requests = max_merge_requests
         cfg.max_merge_vertices = max_merge_vertices
+        cfg.snapshot_max_age_us = snapshot_max_age_us
+        cfg.search_workgroups = search_workgroups
         h = ctypes.c_void_p()
         _check(lib().lpma_server_create(ctypes.byref(h), pma._h, ctypes.byref(cfg)), "server_create")
         self._h = h
@@ -702,7 +708,8 @@ class Server:
     def stats(self):
         s = ServerStats()
         _check(lib().lpma_server_get_stats(self._h, ctypes.byref(s)), "server_stats")
-        return {f: (list(getattr(s, f)) if f in ("submitted", "completed") else int(getattr(s, f)))
+        return {f: (list(getattr(s, f)) if f in ("submitted", "completed") else
+                    (float(getattr(s, f)) if f.endswith("_us") else int(getattr(s, f))))
                 for f, _ in ServerStats._fields_}
 
 

@@ -40,25 +40,64 @@ __global__ void k_bfs_len(DevState d, const u32 *__restrict__ q, u32 qn, u32 *__
     }
 }
 
-__global__ void k_bfs_expand(DevState d, const u32 *__restrict__ q, u32 qn, const u32 *__restrict__ off,
-                             u32 *__restrict__ levels, u32 level, u32 *__restrict__ qnext, u32 *__restrict__ counters) {
+// Each workgroup expands a run of EXP_TPB consecutive slots of the scanned
+// frontier.  The frontier entries covering the run are found with one
+// binary search over the global offsets (by one lane) and staged in LDS;
+// every lane then finds its entry by a short search there instead of a
+// search over the whole frontier.
+static constexpr u32 EXP_TPB = 256;
+static constexpr u32 EXP_STAGE = 512;  // staged offsets per run (>= EXP_TPB + 1 covers every run of
+                                       // non-empty entries; empty rows fall back to the global search)
+
+__global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, const u32 *__restrict__ q, u32 qn,
+                                                         const u32 *__restrict__ off, u32 *__restrict__ levels,
+                                                         u32 level, u32 *__restrict__ qnext, u32 *__restrict__ counters) {
+    __shared__ u32 s_off[EXP_STAGE];
+    __shared__ u32 s_e0;
     const u64 total = off[qn];
     const u32 lane = lane_id();
     u32 scanned = 0;
-    for (u64 t0 = (u64)blockIdx.x * blockDim.x; t0 < total; t0 += (u64)gridDim.x * blockDim.x) {
-        u64 t = t0 + threadIdx.x;
-        u32 v = INF_LEVEL;
-        bool found = false;
-        if (t < total) {
-            // frontier entry owning slot t: last i with off[i] <= t
+    for (u64 t0 = (u64)blockIdx.x * EXP_TPB; t0 < total; t0 += (u64)gridDim.x * EXP_TPB) {
+        if (threadIdx.x == 0) {
+            // last entry with off <= t0
             u32 lo = 0, hi = qn;
             while (hi - lo > 1) {
                 u32 m = (lo + hi) >> 1;
-                if (off[m] <= t) lo = m; else hi = m;
+                if (off[m] <= t0) lo = m; else hi = m;
+            }
+            s_e0 = lo;
+        }
+        __syncthreads();
+        const u32 e0 = s_e0;
+        for (u32 k = threadIdx.x; k < EXP_STAGE; k += EXP_TPB) {
+            u64 idx = (u64)e0 + k;
+            s_off[k] = idx <= qn ? off[idx] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        const u64 t = t0 + threadIdx.x;
+        u32 v = INF_LEVEL;
+        bool found = false;
+        if (t < total) {
+            u32 ent;
+            if ((u64)s_off[EXP_STAGE - 1] > t) {
+                // last k with s_off[k] <= t (s_off[0] <= t0 <= t)
+                u32 lo = 0, hi = EXP_STAGE - 1;
+                while (hi - lo > 1) {
+                    u32 m = (lo + hi) >> 1;
+                    if ((u64)s_off[m] <= t) lo = m; else hi = m;
+                }
+                ent = e0 + lo;
+            } else {
+                u32 lo = e0 + EXP_STAGE - 1, hi = qn;
+                while (hi - lo > 1) {
+                    u32 m = (lo + hi) >> 1;
+                    if (off[m] <= t) lo = m; else hi = m;
+                }
+                ent = lo;
             }
             u64 slo, shi;
-            slot_range(d, q[lo], slo, shi);
-            u64 key = d.keys[slo + (t - off[lo])];
+            slot_range(d, q[ent], slo, shi);
+            u64 key = d.keys[slo + (t - off[ent])];
             if (key != KEY_NONE) {
                 ++scanned;
                 v = (u32)key;
@@ -73,6 +112,7 @@ __global__ void k_bfs_expand(DevState d, const u32 *__restrict__ q, u32 qn, cons
             base = __shfl(base, first, 64);
             if (found) qnext[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
         }
+        __syncthreads();  // s_off / s_e0 are rewritten by the next run
     }
     scanned = wave_sum(scanned);
     if (lane == 0 && scanned) atomicAdd(&counters[1], scanned);
@@ -84,10 +124,10 @@ hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipS
 }
 
 hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
-                     u32 *qnext, u32 *counters, hipStream_t st) {
+                     u32 *qnext, u32 *counters, u32 expand_grid, hipStream_t st) {
     k_bfs_len<<<grid_for(qn, 256, 16384), 256, 0, st>>>(d, q, qn, len);
     (void)device_excl_scan(len, qn, len, scan_scratch, st);
-    k_bfs_expand<<<2048, 256, 0, st>>>(d, q, qn, len, levels, level, qnext, counters);
+    k_bfs_expand<<<expand_grid ? expand_grid : 2048, EXP_TPB, 0, st>>>(d, q, qn, len, levels, level, qnext, counters);
     return hipGetLastError();
 }
 

@@ -58,6 +58,7 @@ hipError_t rebuild_pyramid(const DevState &d, bool full, hipStream_t st);
 hipError_t make_markers(u64 *out, u32 V, hipStream_t st);
 hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st);
 hipError_t fill_u64(u64 *p, u64 n, u64 v, hipStream_t st);
+hipError_t copy_device(void *dst, const void *src, u64 bytes, hipStream_t st);
 
 // gen_dev.hip
 hipError_t gen_rmat_device(u32 scale, u64 n, double a, double b, double c, u64 seed, u64 *out, hipStream_t st);
@@ -73,7 +74,7 @@ hipError_t check_invariants(const DevState &d, u32 *err, hipStream_t st);
 // analytics.hip
 hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipStream_t st);
 hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
-                     u32 *qnext, u32 *counters, hipStream_t st);
+                     u32 *qnext, u32 *counters, u32 expand_grid, hipStream_t st);
 hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outdeg, hipStream_t st);
 hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st);
 hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, float *c, float *acc, float *out,

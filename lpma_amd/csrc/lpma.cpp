@@ -233,6 +233,7 @@ struct lpma_snapshot {
     u64 version = 0, live = 0;
     u32 V = 0, log2S = 5;
     hipEvent_t ready = nullptr;  // the copies are done
+    u32 search_grid = 0;         // workgroups of a search's expansion (0: the full width)
 };
 
 namespace {
@@ -1269,12 +1270,8 @@ int lpma_copy_view(lpma_t *h, uint64_t *d_keys, uint32_t *d_row_offsets, void *s
     if (!h) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
     hipStream_t st = (hipStream_t)stream;
-    if (d_keys)
-        HIPCHECK(hipMemcpyAsync(d_keys, h->st.keys, ((u64)1 << (h->st.H + h->log2S)) * sizeof(u64),
-                                hipMemcpyDeviceToDevice, st));
-    if (d_row_offsets)
-        HIPCHECK(hipMemcpyAsync(d_row_offsets, h->st.row_off, ((u64)h->V + 1) * sizeof(u32), hipMemcpyDeviceToDevice,
-                                st));
+    if (d_keys) HIPCHECK(copy_device(d_keys, h->st.keys, ((u64)1 << (h->st.H + h->log2S)) * sizeof(u64), st));
+    if (d_row_offsets) HIPCHECK(copy_device(d_row_offsets, h->st.row_off, ((u64)h->V + 1) * sizeof(u32), st));
     return LPMA_OK;
 }
 
@@ -1344,7 +1341,7 @@ int lpma_check(lpma_t *h, uint64_t *violations, void *stream) {
 // Level-synchronous BFS over a DevState with scratch from the pool (so
 // concurrent searches over one structure do not share buffers).
 static int bfs_core(int device, const DevState &d, u32 src, u32 *d_levels, u32 *depth, u64 *edges_scanned,
-                    hipStream_t st) {
+                    u32 expand_grid, hipStream_t st) {
     const u64 V = d.V;
     DBuf<u32> q0, q1, len, cnt, scan;
     auto cleanup = [&]() {
@@ -1370,7 +1367,7 @@ static int bfs_core(int device, const DevState &d, u32 src, u32 *d_levels, u32 *
     u32 c[2];
     int rc = LPMA_OK;
     while (qn) {
-        if (bfs_level(d, q, qn, len.p, scan.p, d_levels, level, qn_buf, cnt.p, st) != hipSuccess ||
+        if (bfs_level(d, q, qn, len.p, scan.p, d_levels, level, qn_buf, cnt.p, expand_grid, st) != hipSuccess ||
             hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess || fill_u32(cnt.p, 2, 0, st) != hipSuccess) {
             (void)hipGetLastError();
@@ -1394,7 +1391,7 @@ int lpma_bfs(lpma_t *h, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint6
     if (!h || !d_levels) return LPMA_ERR_INVALID_ARGUMENT;
     if (src >= h->V) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(h->device);
-    return bfs_core(h->device, h->st, src, d_levels, depth, edges_scanned, (hipStream_t)stream);
+    return bfs_core(h->device, h->st, src, d_levels, depth, edges_scanned, 0, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- snapshots
@@ -1420,11 +1417,10 @@ int lpma_snapshot_create(lpma_snapshot_t **out, lpma_t *h, void *stream) {
         lpma_snapshot_destroy(s);
         return r;
     }
-    if (hipMemcpyAsync(s->keys.p, h->st.keys, cap * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(s->sep.p, h->st.sep, nseg * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(s->cnt.p, h->st.cnt, nseg * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(s->row_off.p, h->st.row_off, ((u64)h->V + 1) * 4, hipMemcpyDeviceToDevice, st) !=
-            hipSuccess) {
+    if (copy_device(s->keys.p, h->st.keys, cap * 8, st) != hipSuccess ||
+        copy_device(s->sep.p, h->st.sep, nseg * 8, st) != hipSuccess ||
+        copy_device(s->cnt.p, h->st.cnt, nseg * 4, st) != hipSuccess ||
+        copy_device(s->row_off.p, h->st.row_off, ((u64)h->V + 1) * 4, st) != hipSuccess) {
         (void)hipGetLastError();
         lpma_snapshot_destroy(s);
         return LPMA_ERR_DEVICE;
@@ -1515,7 +1511,13 @@ int lpma_snapshot_bfs(lpma_snapshot_t *s, uint32_t src, uint32_t *d_levels, uint
     if (!s || !d_levels || src >= s->V) return LPMA_ERR_INVALID_ARGUMENT;
     DevGuard guard(s->device);
     CHECKED(snapshot_wait(s, (hipStream_t)stream));
-    return bfs_core(s->device, s->st, src, d_levels, depth, edges_scanned, (hipStream_t)stream);
+    return bfs_core(s->device, s->st, src, d_levels, depth, edges_scanned, s->search_grid, (hipStream_t)stream);
+}
+
+int lpma_snapshot_set_search_width(lpma_snapshot_t *s, uint32_t workgroups) {
+    if (!s || workgroups > 65536) return LPMA_ERR_INVALID_ARGUMENT;
+    s->search_grid = workgroups;
+    return LPMA_OK;
 }
 
 int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, float *d_rank, uint32_t *iterations,

@@ -11,9 +11,9 @@
 //     their order: for each key the last request decides), then every
 //     queued neighbor read as ONE count / scan / gather pass over the live
 //     structure, so reads see the version the updates just made;
-//   * a BFS runs on a snapshot of the version current at its admission, on
-//     a worker thread with its own stream, beside the engine's updates and
-//     reads (searches admitted at one version share one snapshot);
+//   * a BFS runs on a snapshot -- the newest one, copied at most a set age
+//     ago -- on a worker thread with its own stream, beside the engine's
+//     updates and reads (searches share snapshots);
 //   * admission: the device memory a request needs (update and read
 //     staging, BFS snapshots and scratch) is charged against a byte budget;
 //     a request waits in its queue while the budget has no room for it, and
@@ -67,6 +67,7 @@ struct Snap {
     u64 applied = 0;
     int refs = 0;
     u64 bytes = 0;
+    clk::time_point taken;
 };
 
 // device buffer owned by the engine, grown on demand
@@ -186,6 +187,7 @@ void finish(lpma_server *s, Request *r, int status) {
 // fails on an invalid key, the requests are applied one by one instead so
 // that only the offending ones fail.
 void serve_updates(lpma_server *s, std::vector<Request *> &batch) {
+    auto t0 = clk::now();
     u64 K = 0;
     for (Request *r : batch) K += r->ins.size() + r->del.size();
     int rc = grow_dev(s, s->d_keys, K);
@@ -211,8 +213,10 @@ void serve_updates(lpma_server *s, std::vector<Request *> &batch) {
         }
     }
     std::vector<int> status(batch.size(), rc);
+    auto t1 = clk::now(), t2 = t1;
     if (rc == LPMA_OK) {
         rc = lpma_update_sequence(s->h, pi.data(), ni.data(), pd.data(), nd.data(), (u32)batch.size(), s->wst);
+        t2 = clk::now();
         if (rc == LPMA_ERR_INVALID_KEY && batch.size() > 1) {
             for (size_t i = 0; i < batch.size(); ++i)
                 status[i] = lpma_update_mixed(s->h, pi[i], ni[i], pd[i], nd[i], s->wst);
@@ -224,9 +228,13 @@ void serve_updates(lpma_server *s, std::vector<Request *> &batch) {
             std::fill(status.begin(), status.end(), (int)LPMA_ERR_DEVICE);
         }
     }
+    auto t3 = clk::now();
     lpma_view_t v;
     lpma_get_view(s->h, &v);
     std::lock_guard<std::mutex> g(s->mu);
+    s->st.stage_us += us_between(t0, t1);
+    s->st.pipeline_us += us_between(t1, t2);
+    s->st.finish_us += us_between(t2, t3);
     s->st.update_runs++;
     for (size_t i = 0; i < batch.size(); ++i) {
         Request *r = batch[i];
@@ -326,7 +334,13 @@ void destroy_unused_snaps(lpma_server *s) {
 void dispatch_bfs(lpma_server *s, std::unique_lock<std::mutex> &lk) {
     const u64 bb = bfs_bytes(s->V);
     while (!s->q_bfs.empty() && s->busy < (int)s->cfg.bfs_workers) {
-        const bool need_snap = !s->cur || s->cur->applied != s->applied;
+        // a search reads the newest snapshot while it is younger than the
+        // configured age (its result reports the version it saw); a newer
+        // version is copied only after that, so a stream of searches does not
+        // turn into a stream of full copies beside the writer
+        const bool stale = s->cur && s->cur->applied != s->applied &&
+                           us_between(s->cur->taken, clk::now()) >= (double)s->cfg.snapshot_max_age_us;
+        const bool need_snap = !s->cur || stale;
         if (need_snap && s->cur) {
             // the newest snapshot is out of date: it goes once unused
             s->cur = nullptr;
@@ -360,10 +374,12 @@ void dispatch_bfs(lpma_server *s, std::unique_lock<std::mutex> &lk) {
                 s->cv_done.notify_all();
                 continue;
             }
+            lpma_snapshot_set_search_width(snap, s->cfg.search_workgroups);
             Snap *p = new Snap();
             p->s = snap;
             p->applied = s->applied;
             p->bytes = snap_bytes;
+            p->taken = clk::now();
             s->snaps.push_back(p);
             s->cur = p;
             s->in_use += snap_bytes;
@@ -414,20 +430,35 @@ void engine_loop(lpma_server *s) {
         for (Request *r : ub) r->t_start = now;
         for (Request *r : rb) r->t_start = now;
         lk.unlock();
+        auto t1 = clk::now();
         if (!ub.empty()) serve_updates(s, ub);
+        auto t2 = clk::now();
         if (!rb.empty()) serve_reads(s, rb);
+        auto t3 = clk::now();
         lk.lock();
+        s->st.update_us += us_between(t1, t2);
+        s->st.read_us += us_between(t2, t3);
+        auto t4 = clk::now();
         destroy_unused_snaps(s);
         dispatch_bfs(s, lk);
+        s->st.dispatch_us += us_between(t4, clk::now());
     }
     s->stop_workers = true;
     s->cv_work.notify_all();
 }
 
+hipStream_t worker_stream() {
+    hipStream_t ws = nullptr;
+    if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        ws = nullptr;
+    }
+    return ws;
+}
+
 void worker_loop(lpma_server *s) {
     (void)hipSetDevice(s->device);
-    hipStream_t ws = nullptr;
-    (void)hipStreamCreateWithFlags(&ws, hipStreamNonBlocking);
+    hipStream_t ws = worker_stream();
     u32 *d_levels = nullptr;
     if (hipMalloc((void **)&d_levels, ((u64)s->V + 1) * 4) != hipSuccess) {
         (void)hipGetLastError();
@@ -515,11 +546,13 @@ extern "C" {
 
 void lpma_server_config_default(lpma_server_config_t *cfg) {
     if (!cfg) return;
-    cfg->bfs_workers = 4;
+    cfg->bfs_workers = 2;
     cfg->hbm_budget_bytes = 0;  // 0: 90% of the device memory free at creation
     cfg->max_merge_keys = 1ull << 24;
     cfg->max_merge_requests = 4096;
     cfg->max_merge_vertices = 1ull << 20;
+    cfg->snapshot_max_age_us = 50000;
+    cfg->search_workgroups = 256;
 }
 
 int lpma_server_create(lpma_server_t **out, lpma_t *h, const lpma_server_config_t *cfg_in) {

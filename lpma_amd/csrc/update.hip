@@ -292,7 +292,8 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
 }
 
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
-    k_density<<<grid_for(n, 256), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
+    // one thread per update (no grid-stride loop): the grid covers n
+    k_density<<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
     return hipGetLastError();
 }
 
@@ -368,8 +369,8 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
 // n bounds the number of runs (the batch size); the kernel reads the exact
 // count from the status
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
-    k_leaders<<<grid_for(n, 1024), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2], b.work[3],
-                                                   b.status);
+    k_leaders<<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2],
+                                                            b.work[3], b.status);
     return hipGetLastError();
 }
 
@@ -1216,6 +1217,30 @@ __global__ void k_fill_u32(u32 *p, u64 n, u32 v) {
 }
 __global__ void k_fill_u64(u64 *p, u64 n, u64 v) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// device-to-device copy as a kernel at HBM rate (16-byte accesses; the
+// runtime's copy of a multi-GB array can go through a DMA engine far below
+// it and hold up the stream behind)
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, u64 n16) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (u64)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(unsigned char *__restrict__ dst, const unsigned char *__restrict__ src, u64 n) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+hipError_t copy_device(void *dst, const void *src, u64 bytes, hipStream_t st) {
+    if (bytes == 0) return hipSuccess;
+    const bool aligned = ((((uintptr_t)dst) | ((uintptr_t)src)) & 15) == 0;
+    u64 n16 = aligned ? bytes / 16 : 0;
+    if (n16) k_copy16<<<grid_for(n16, 256, 16384), 256, 0, st>>>((uint4 *)dst, (const uint4 *)src, n16);
+    u64 done = n16 * 16;
+    if (done < bytes)
+        k_copy_tail<<<grid_for(bytes - done, 256, 4096), 256, 0, st>>>((unsigned char *)dst + done,
+                                                                       (const unsigned char *)src + done, bytes - done);
+    return hipGetLastError();
 }
 
 hipError_t fill_u32(u32 *p, u64 n, u32 v, hipStream_t st) {
