@@ -137,8 +137,37 @@ hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *sca
 // dangling vertices and 1-alpha elsewhere,
 //     pr' = alpha * A^T (pr / outdeg) + (sum_u a_u pr_u) / n,
 // iterated from the uniform vector; here until the L1 change is below the
-// tolerance or max_iter, then normalised by the L1 norm.  Values and
-// accumulation are FP32; the push over the slot array uses FP32 atomics.
+// tolerance or max_iter, then normalised by the L1 norm.
+//
+// Values and accumulation are FP32.  Every accumulation is compensated: an
+// atomic add returns the old value, the thread recomputes the sum and its
+// exact rounding error (TwoSum, Knuth TAOCP vol. 2, 4.2.2) and adds that
+// error to a second FP32 accumulator, so a hub that receives millions of
+// contributions keeps the precision of its partial sums.  The push goes to
+// one accumulator copy per XCD (the XCD read from the hardware register), so
+// the atomics of a hub are spread over eight addresses; the copies are
+// summed (Kahan) when the iteration finishes.
+
+static constexpr u32 PR_COPIES = 8;  // accumulator copies, one per XCD
+
+__device__ __forceinline__ u32 xcc_id() {
+    u32 x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+    return x & (PR_COPIES - 1);
+}
+
+// hi += x with the rounding error of that add going to lo
+__device__ __forceinline__ void atomic_add2(float *hi, float *lo, float x) {
+    const float old = atomicAdd(hi, x);
+    const float s = old + x;
+    const float bp = s - old;
+    const float e = (old - (s - bp)) + (x - bp);
+    if (e != 0.f) atomicAdd(lo, e);
+}
+
+// red layout (floats): [0,1] dangling mass hi/lo, [2,3] sum hi/lo,
+// [4,5] L1 change hi/lo
+static constexpr u32 PR_RED = 8;
 
 // out-degree of every row from marker ranks (rank = live slots before)
 __global__ void k_outdeg(DevState d, const u32 *__restrict__ gpre, u32 *__restrict__ outdeg) {
@@ -155,37 +184,54 @@ __global__ void k_outdeg(DevState d, const u32 *__restrict__ gpre, u32 *__restri
     }
 }
 
-// contributions pr/outdeg, the dangling mass, and a cleared accumulator
+// contributions pr/outdeg, the dangling mass, and cleared accumulators
+// (PR_COPIES hi copies, then PR_COPIES lo copies, V floats each)
 __global__ void k_pr_prep(const float *__restrict__ pr, const u32 *__restrict__ outdeg, u32 V, float *__restrict__ c,
                           float *__restrict__ acc, float *__restrict__ red) {
     float dang = 0.f;
-    for (u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x; u < V; u += (u64)gridDim.x * blockDim.x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x; u < V; u += stride) {
         u32 od = outdeg[u];
         float p = pr[u];
         c[u] = od ? p / (float)od : 0.f;
         if (!od) dang += p;
-        acc[u] = 0.f;
     }
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < 2ull * PR_COPIES * V; i += stride) acc[i] = 0.f;
     dang = wave_sum(dang);
-    if (lane_id() == 0 && dang != 0.f) atomicAdd(&red[0], dang);
+    if (lane_id() == 0 && dang != 0.f) atomic_add2(&red[0], &red[1], dang);
 }
 
 __global__ void k_pr_push(DevState d, const float *__restrict__ c, float *__restrict__ acc) {
     const u64 W = nseg_of(d) << d.log2S;
+    const u64 V = d.V;
+    float *hi = acc + (u64)xcc_id() * V;
+    float *lo = hi + PR_COPIES * V;
     for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < W; t += (u64)gridDim.x * blockDim.x) {
         u64 key = d.keys[t];
         if (key == KEY_NONE || is_marker(key)) continue;
-        atomicAdd(&acc[(u32)key], c[(u32)(key >> 32)]);
+        const u32 v = (u32)key;
+        atomic_add2(&hi[v], &lo[v], c[(u32)(key >> 32)]);
     }
 }
 
 // pr' = alpha*acc + (alpha*dangling + (1-alpha)*sum(pr)) / n; L1 change and sum
 __global__ void k_pr_finish(const float *__restrict__ acc, const float *__restrict__ pr, float *__restrict__ out,
                             u32 V, float alpha, const float *__restrict__ red_in, float *__restrict__ red_out) {
-    const float gamma = (alpha * red_in[0] + (1.f - alpha) * red_in[1]) / (float)V;
+    const float dang = red_in[0] + red_in[1];
+    const float sum_in = red_in[2] + red_in[3];
+    const float gamma = (alpha * dang + (1.f - alpha) * sum_in) / (float)V;
     float dl = 0.f, sum = 0.f;
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x) {
-        float p = alpha * acc[v] + gamma;
+        // Kahan sum of the copies, lo parts first (the small terms)
+        float a = 0.f, comp = 0.f;
+#pragma unroll
+        for (u32 k = 2 * PR_COPIES; k-- > 0;) {
+            float y = acc[(u64)k * V + v] - comp;
+            float t = a + y;
+            comp = (t - a) - y;
+            a = t;
+        }
+        float p = alpha * a + gamma;
         out[v] = p;
         dl += fabsf(p - pr[v]);
         sum += p;
@@ -193,27 +239,23 @@ __global__ void k_pr_finish(const float *__restrict__ acc, const float *__restri
     dl = wave_sum(dl);
     sum = wave_sum(sum);
     if (lane_id() == 0) {
-        atomicAdd(&red_out[2], dl);
-        atomicAdd(&red_out[1], sum);
+        atomic_add2(&red_out[4], &red_out[5], dl);
+        atomic_add2(&red_out[2], &red_out[3], sum);
     }
 }
 
 __global__ void k_pr_init(float *pr, u32 V, float *red) {
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x)
         pr[v] = 1.f / (float)V;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        red[0] = 0.f;  // dangling mass of the current vector
-        red[1] = 1.f;  // its sum
-        red[2] = 0.f;  // L1 change
-    }
+    if (blockIdx.x == 0 && threadIdx.x < PR_RED) red[threadIdx.x] = threadIdx.x == 2 ? 1.f : 0.f;  // sum = 1
 }
 
 __global__ void k_pr_reset(float *red_next) {
-    if (threadIdx.x == 0) { red_next[0] = 0.f; red_next[1] = 0.f; red_next[2] = 0.f; }
+    if (threadIdx.x < PR_RED) red_next[threadIdx.x] = 0.f;
 }
 
 __global__ void k_pr_scale(float *pr, u32 V, const float *red) {
-    const float inv = 1.f / red[1];
+    const float inv = 1.f / (red[2] + red[3]);
     for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x) pr[v] *= inv;
 }
 
@@ -222,6 +264,9 @@ hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outde
     k_outdeg<<<grid_for(d.V, 256, 16384), 256, 0, st>>>(d, gpre, outdeg);
     return hipGetLastError();
 }
+
+u64 pr_acc_floats(u32 V) { return 2ull * PR_COPIES * V; }
+u32 pr_red_floats() { return PR_RED; }
 
 hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st) {
     k_pr_init<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, V, red);
@@ -234,7 +279,7 @@ hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, flo
                       float alpha, float *red, float *red_next, hipStream_t st) {
     const u32 V = d.V;
     k_pr_reset<<<1, 64, 0, st>>>(red_next);
-    k_pr_prep<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, outdeg, V, c, acc, red);
+    k_pr_prep<<<grid_for(pr_acc_floats(V), 256, 16384), 256, 0, st>>>(pr, outdeg, V, c, acc, red);
     k_pr_push<<<grid_for(nseg_of(d) << d.log2S, 256, 65536), 256, 0, st>>>(d, c, acc);
     k_pr_finish<<<grid_for(V, 256, 16384), 256, 0, st>>>(acc, pr, out, V, alpha, red, red_next);
     return hipGetLastError();

@@ -77,6 +77,8 @@ hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *sca
                      u32 *qnext, u32 *counters, u32 expand_grid, hipStream_t st);
 hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outdeg, hipStream_t st);
 hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st);
+u64 pr_acc_floats(u32 V);
+u32 pr_red_floats();
 hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, float *c, float *acc, float *out,
                       float alpha, float *red, float *red_next, hipStream_t st);
 hipError_t pr_scale(float *pr, u32 V, const float *red, hipStream_t st);

@@ -1533,11 +1533,11 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
     CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8));
     CHECKED(ensure(h, h->aq0, V));   // out-degrees
     CHECKED(ensure(h, h->af0, V));   // contributions
-    CHECKED(ensure(h, h->af1, V));   // accumulator
+    CHECKED(ensure(h, h->af1, pr_acc_floats(h->V)));  // compensated accumulators
     CHECKED(ensure(h, h->af2, V));   // second rank vector
-    CHECKED(ensure(h, h->ared, 8));
+    CHECKED(ensure(h, h->ared, 2 * pr_red_floats()));
     HIPCHECK(pr_outdeg(h->st, h->pl.gpre.p, h->pl.scan_scr.p, h->aq0.p, st));
-    float *red[2] = {h->ared.p, h->ared.p + 4};
+    float *red[2] = {h->ared.p, h->ared.p + pr_red_floats()};
     float *vec[2] = {d_rank, h->af2.p};
     HIPCHECK(pr_init(vec[0], h->V, red[0], st));
     u32 it = 0;
@@ -1548,14 +1548,14 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
                             red[cur ^ 1], st));
         cur ^= 1;
         ++it;
-        float r[4];
+        float r[8];
         HIPCHECK(hipMemcpyAsync(r, red[cur], sizeof(r), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        delta = r[2];
+        delta = r[4] + r[5];
         if (delta < tolerance) break;
     }
     HIPCHECK(pr_scale(vec[cur], h->V, red[cur], st));
-    if (vec[cur] != d_rank) HIPCHECK(hipMemcpyAsync(d_rank, vec[cur], V * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (vec[cur] != d_rank) HIPCHECK(copy_device(d_rank, vec[cur], V * sizeof(float), st));
     HIPCHECK(hipStreamSynchronize(st));
     if (iterations) *iterations = it;
     if (last_delta) *last_delta = delta;
