@@ -122,8 +122,12 @@ struct PmaStat {
     u32 flags;     // FLAG_ROOT / FLAG_RESIZE raised by its runs
     u32 u0, u1;    // its effective updates: compacted range [u0, u1)
     i32 pd0, pd1;  // the update prefix at u0 and u1 (delta = pd1 - pd0)
-    u32 levels;    // bit l: some run claimed a window of level l >= 1
 };
+
+// claimed-level masks: bit l set when some run of the PMA claimed a window
+// of level l >= 1; LV_REPL copies per PMA (written by workgroup index) so
+// the density check's atomics do not all meet on one word
+static constexpr u32 LV_REPL = 32;
 
 // One batch source: n keys of one operation for one PMA, placed at
 // [off, off + n) of the packed batch.
@@ -155,6 +159,7 @@ struct RoundHdr {
 struct Group {
     const PmaEntry *tab;
     PmaStat *pstat;
+    u32 *lvmask;  // npma x LV_REPL claimed-level masks
     u64 *stats;   // ST_ROWS x ST_NUM counters of this pipeline
     u32 npma;
     u32 pshift;
@@ -162,10 +167,20 @@ struct Group {
     u32 epoch;    // tags this run's window claims
     u32 log2S;    // segment size, the same for every PMA of the run
     u32 seg_log2G;  // lanes per window of the single-segment rebalance (log2)
+    u32 inline_one; // one PMA whose geometry travels in `one` (no table load)
     const RoundHdr *rh;  // graph replays: n and epoch come from here
+    DevState one;
+    Density one_dn;
 };
 
 __device__ __forceinline__ u32 group_epoch(const Group &g) { return g.rh ? g.rh->epoch : g.epoch; }
+
+// geometry of PMA pid: from the kernel arguments for a lone PMA (no
+// dependent load on the critical path), else from the device table
+__device__ __forceinline__ DevState pma_of(const Group &g, u32 pid) {
+    if (g.inline_one) return g.one;
+    return g.tab[pid].d;
+}
 
 __host__ __device__ __forceinline__ u64 nseg_of(const DevState &d) { return (u64)1 << d.H; }
 

@@ -138,7 +138,7 @@ struct Pipeline {
     hipStream_t cur_stream = nullptr;  // stream of the call in progress
 
     DBuf<u64> pk0, pk1, ukey, useg, ckey, cseg, merged, lmbase;
-    DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr;
+    DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr, lvmask;
     DBuf<i32> pd, part;
     DBuf<Work> work[NUM_CLASSES];
     DBuf<PmaEntry> tab;
@@ -348,6 +348,7 @@ void pipeline_free(Pipeline &pl) {
     release_on(d, pl.ckey); release_on(d, pl.cseg); release_on(d, pl.merged); release_on(d, pl.lmbase);
     release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
     release_on(d, pl.lchunk); release_on(d, pl.lhdr); release_on(d, pl.pd); release_on(d, pl.part);
+    release_on(d, pl.lvmask);
     for (int c = 0; c < NUM_CLASSES; ++c) release_on(d, pl.work[c]);
     release_on(d, pl.tab); release_on(d, pl.segs); release_on(d, pl.out); release_on(d, pl.rhdr);
     for (auto &kv : pl.graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -494,6 +495,7 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
         (r = ensure(pl, pl.useg, n)) || (r = ensure(pl, pl.ckey, n)) || (r = ensure(pl, pl.cseg, n)) ||
         (r = ensure(pl, pl.pd, n + 1)) || (r = ensure(pl, pl.rlevel, n)) || (r = ensure(pl, pl.part, 2 * nb + 2)) ||
         (r = ensure(pl, pl.tab, npma)) || (r = ensure(pl, pl.segs, std::max<u32>(nsegs, INLINE_SEGS))) ||
+        (r = ensure(pl, pl.lvmask, (u64)npma * LV_REPL)) ||
         (r = ensure(pl, pl.out, out_words)))
         return r;
     for (int c = 0; c < NUM_CLASSES; ++c)
@@ -797,6 +799,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     Group g;
     g.tab = pl.tab.p;
     g.pstat = pstat_of(pl);
+    g.lvmask = pl.lvmask.p;
     g.stats = pl.d_stats;
     g.npma = npma;
     g.pshift = pshift;
@@ -805,6 +808,9 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.log2S = mem[0].h->log2S;
     g.seg_log2G = seg_lanes_log2(g.log2S, pl.seg_slots_log2);
     g.rh = nullptr;
+    g.inline_one = npma == 1 ? 1u : 0u;
+    g.one = mem[0].h->st;
+    g.one_dn = mem[0].h->dn;
     BatchBufs bb = batch_bufs(pl);
     // bit 0 of a packed key is the operation (with the request order above
     // it): sorted only when a member mixes them or merges requests
@@ -878,6 +884,7 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     Group g;
     g.tab = pl.tab.p;
     g.pstat = pstat_of(pl);
+    g.lvmask = pl.lvmask.p;
     g.stats = pl.d_stats;
     g.npma = npma;
     g.pshift = pshift;
@@ -886,6 +893,10 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     g.log2S = log2S;
     g.seg_log2G = seg_lanes_log2(log2S, pl.seg_slots_log2);
     g.rh = pl.rhdr.p;
+    // a captured graph outlives resizes: geometry always from the table
+    g.inline_one = 0;
+    memset(&g.one, 0, sizeof(g.one));
+    memset(&g.one_dn, 0, sizeof(g.one_dn));
     return g;
 }
 
@@ -898,7 +909,7 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
                             (u64)pl.ckey.p, (u64)pl.cseg.p, (u64)pl.pd.p, (u64)pl.rlevel.p, (u64)pl.part.p,
                             (u64)pl.lchunk.p, (u64)pl.lmbase.p, (u64)pl.lhdr.p, (u64)pl.out.p, (u64)pl.tab.p,
                             (u64)pl.segs.p, (u64)pl.rhdr.p, (u64)pl.d_stats, (u64)pl.h_out, (u64)pl.h_segs,
-                            (u64)pl.h_rhdr};
+                            (u64)pl.h_rhdr, (u64)pl.lvmask.p};
     for (int c = 0; c < NUM_CLASSES; ++c) key.push_back((u64)pl.work[c].p);
     auto it = pl.graphs.find(cap);
     if (it != pl.graphs.end() && it->second.key != key) {

@@ -49,6 +49,15 @@ hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, Bat
 // Dedup (keep the last of equal keys: later requests sort after earlier
 // ones, and within a request deletes before inserts), route each key to
 // its segment, and resolve it against the keys present there.
+// ONE: a lone PMA whose geometry is a kernel argument (scalar registers);
+// otherwise it is read from the table per element
+template <bool ONE>
+__device__ __forceinline__ DevState geom(const Group &g, u32 pid) {
+    if (ONE) return g.one;
+    return g.tab[pid].d;
+}
+
+template <bool ONE>
 __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restrict__ pk, u64 n,
                                                     u64 *__restrict__ ukey, u64 *__restrict__ useg,
                                                     i32 *__restrict__ part_cnt, i32 *__restrict__ part_dl) {
@@ -64,7 +73,7 @@ __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restri
             useg[i] = RK_NONE;
         } else {
             const u32 pid = (u32)(p >> g.pshift);
-            const DevState d = g.tab[pid].d;
+            const DevState d = geom<ONE>(g, pid);
             const u64 kk = (p & pmask) >> ks;
             const u32 op = (u32)(p & 1);
             const u64 key = ((kk >> d.vbits) << 32) | (kk & ((1ull << d.vbits) - 1));
@@ -172,7 +181,8 @@ __global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ uke
 hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st) {
     u32 nb = locate_blocks(n);
     i32 *pc = b.part, *pdl = b.part + nb + 1;
-    k_locate<<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
+    if (g.inline_one) k_locate<true><<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
+    else k_locate<false><<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
     int self_prefix = nb <= COMPACT_SELF_PREFIX_MAX;
     if (!self_prefix) k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
     k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, g.rh, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd,
@@ -228,6 +238,7 @@ __device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start,
 // window fits after the batch, walking up the count pyramid; updates of the
 // window are counted by galloping over the sorted run keys.  The first and
 // last update of each PMA record its compacted range.
+template <bool ONE>
 __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const Status *__restrict__ status_in, u32 *__restrict__ rlevel, Status *status) {
     const u32 n_eff = status_in->n_eff;
@@ -247,8 +258,8 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             g.pstat[pid].pd1 = pd[j + 1];
         }
         if (j == 0 || prev != rk) {
-            const PmaEntry *e = g.tab + pid;
-            const DevState d = e->d;
+            const Density &dn = ONE ? g.one_dn : g.tab[pid].dn;
+            const DevState d = geom<ONE>(g, pid);
             const u64 pb = (u64)pid << 32;
             const u32 s = (u32)rk;
             u32 u0 = j, u1 = j + 1;
@@ -261,7 +272,7 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
                 u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l));
                 C = *count_ptr(d, l, w);
                 i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
-                if (window_fits(e->dn, l, d.H, (u64)m)) break;
+                if (window_fits(dn, l, d.H, (u64)m)) break;
             }
             rlevel[j] = l;
             if (l >= d.H) {
@@ -283,22 +294,28 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
     if (lane_id() == 0 && f) atomicOr(&status->flags, f);
+    // (into the workgroup's replica of the PMA's mask)
     const u32 pid0 = __shfl(pid, 0, 64);
+    const u32 rep = blockIdx.x & (LV_REPL - 1);
     u32 lv = pid == pid0 ? lvbit : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) lv |= __shfl_xor(lv, o, 64);
-    if (lane_id() == 0 && lv) atomicOr(&g.pstat[pid0].levels, lv);
-    if (pid != pid0 && lvbit) atomicOr(&g.pstat[pid].levels, lvbit);
+    if (lane_id() == 0 && lv) atomicOr(&g.lvmask[(u64)pid0 * LV_REPL + rep], lv);
+    if (pid != pid0 && lvbit) atomicOr(&g.lvmask[(u64)pid * LV_REPL + rep], lvbit);
 }
 
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
     // one thread per update (no grid-stride loop): the grid covers n
-    k_density<<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
+    if (g.inline_one)
+        k_density<true><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
+    else
+        k_density<false><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------- window election
 
+template <bool ONE>
 __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
                           Status *status) {
@@ -307,13 +324,27 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
     Work wk;
+    // claimed levels of the workgroup's first PMA: its replicas or-ed once
+    __shared__ u32 s_mask, s_pid;
+    const u32 jb = blockIdx.x * blockDim.x;
+    if (threadIdx.x < 64 && jb < n_eff) {
+        const u32 pb = (u32)(cseg[jb] >> 32);
+        u32 m = threadIdx.x < LV_REPL ? g.lvmask[(u64)pb * LV_REPL + threadIdx.x] : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m |= __shfl_xor(m, o, 64);
+        if (threadIdx.x == 0) {
+            s_mask = m;
+            s_pid = pb;
+        }
+    }
+    __syncthreads();
     if (j < n_eff) {
         const u64 rk = cseg[j];
         const u32 pid = (u32)(rk >> 32);
         bool leader = (j == 0 || cseg[j - 1] != rk);
         // a root rebalance or a resize redoes the whole PMA: no windows needed
         if (leader && (g.pstat[pid].flags & (FLAG_ROOT | FLAG_RESIZE))) leader = false;
-        const DevState d = g.tab[pid].d;
+        const DevState d = geom<ONE>(g, pid);
         const u32 s = (u32)rk;
         u32 L = leader ? rlevel[j] : 0;
         u64 nseg = nseg_of(d);
@@ -324,7 +355,10 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
         if (leader) {
             // inside a larger chosen window?  only the levels some run of the
             // PMA claimed can hold one; their loads are issued together
-            u32 lv = g.pstat[pid].levels & ~((2u << L) - 1u);
+            u32 lv = s_pid == pid ? s_mask : 0u;
+            if (s_pid != pid)
+                for (u32 r = 0; r < LV_REPL; ++r) lv |= g.lvmask[(u64)pid * LV_REPL + r];
+            lv &= ~((2u << L) - 1u);
             bool dom = false;
             while (lv) {
                 u32 l2 = (u32)__builtin_ctz(lv);
@@ -369,8 +403,12 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
 // n bounds the number of runs (the batch size); the kernel reads the exact
 // count from the status
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
-    k_leaders<<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1], b.work[2],
-                                                            b.work[3], b.status);
+    if (g.inline_one)
+        k_leaders<true><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1],
+                                                                      b.work[2], b.work[3], b.status);
+    else
+        k_leaders<false><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1],
+                                                                       b.work[2], b.work[3], b.status);
     return hipGetLastError();
 }
 
@@ -506,7 +544,7 @@ __device__ void rebalance_loop(const Group &g, const Work *__restrict__ work, co
     const u32 nw = *nwork;
     for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
         Work wk = work[i];
-        const DevState d = g.tab[wk.pid].d;
+        const DevState d = pma_of(g, wk.pid);
         rebalance_window<NT, ECAP, SEGCAP>(d, wk, ckey, pd, A, B, P, pre, stmp, bs);
     }
     __syncthreads();
@@ -621,7 +659,7 @@ __global__ __launch_bounds__(64) void k_rebalance_wave(Group g, const Work *__re
     const u32 nw = status->nwork[CLASS_WAVE];
     for (u32 i = blockIdx.x; i < nw; i += gridDim.x) {
         Work wk = work[i];
-        const DevState d = g.tab[wk.pid].d;
+        const DevState d = pma_of(g, wk.pid);
         rebalance_window_wave(d, wk, ckey, pd, A, B, P, bs);
     }
     __syncthreads();
@@ -661,7 +699,7 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
     u64 bytes = 0, upd = 0, markers = 0, wins = 0;
     for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += gridDim.x * groups_per_block) {
         const Work wk = work[w];
-        const DevState d = g.tab[wk.pid].d;
+        const DevState d = pma_of(g, wk.pid);
         const u32 C = wk.C;
         const u32 k = wk.u1 - wk.u0;
         const u64 base = (u64)wk.a << g.log2S;
@@ -943,7 +981,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__re
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
-        const DevState d = g.tab[wk.pid].d;
+        const DevState d = pma_of(g, wk.pid);
         const u64 pb = (u64)wk.pid << 32;
         const u32 cl = chunk_level(log2S, wk.L);
         const u64 j = c - chunk_base[i];
@@ -998,7 +1036,7 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__re
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
-        const DevState d = g.tab[wk.pid].d;
+        const DevState d = pma_of(g, wk.pid);
         const u32 cl = chunk_level(log2S, wk.L);
         const u64 j = c - chunk_base[i];
         const u64 rel0 = j << cl;
@@ -1140,7 +1178,7 @@ __global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
         }
         const PmaStat ps = g.pstat[lo];
         if (ps.u1 == ps.u0 || (ps.flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
-        const DevState d = g.tab[lo].d;
+        const DevState d = pma_of(g, lo);
         if (d.H == 0) continue;
         pyr_chunk(d, c - g.tab[lo].pyr_blk0);
         __syncthreads();
@@ -1172,10 +1210,11 @@ __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, Inline
     }
     if (threadIdx.x == 0) {
         PmaStat z;
-        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0; z.levels = 0;
+        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0;
         g.pstat[blockIdx.x] = z;
     }
-    const DevState d = g.tab[blockIdx.x].d;
+    if (threadIdx.x < LV_REPL) g.lvmask[(u64)blockIdx.x * LV_REPL + threadIdx.x] = 0;
+    const DevState d = pma_of(g, blockIdx.x);
     pyr_top_levels(d);
 }
 
