@@ -78,11 +78,12 @@ def step_list(ins_batches, del_batches, n):
     return out
 
 
-def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile):
+def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_lane=0):
     import torch
     import lpma_amd as L
     stream = torch.cuda.current_stream()
     pma = L.PMA(V, segment_slots=seg)
+    pma.set_launch(slots_per_lane)
     t = time.time()
     pma.bulk_load(L.to_device(bulk))
     torch.cuda.synchronize()
@@ -659,6 +660,7 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--edges", type=int, default=1 << 24)
     ap.add_argument("--segment-slots", type=int, default=32)
+    ap.add_argument("--seg-slots-per-lane", type=int, default=0, help="single-segment rebalance shape (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-batches", type=int, default=15)
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -693,9 +695,11 @@ def main():
     seq = step_list(ins_b, del_b, total)
 
     # pass 1: throughput and per-batch latency, no profiling events
-    r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False)
+    r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False,
+                slots_per_lane=args.seg_slots_per_lane)
     # pass 2: same stream with per-launch HIP events for the kernel roofline
-    rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True)
+    rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True,
+                 slots_per_lane=args.seg_slots_per_lane)
 
     if args.copy_calib:
         src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device=dev)

@@ -703,8 +703,10 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         const u32 C = wk.C;
         const u32 k = wk.u1 - wk.u0;
         const u64 base = (u64)wk.a << g.log2S;
+        // lane-contiguous: each load instruction of the group covers G
+        // consecutive slots
         for (u32 r = 0; r < SPL; ++r) {
-            u32 t = lane_g * SPL + r;
+            u32 t = r * G + lane_g;
             if (t < C) buf[t] = d.keys[base + t];
         }
         const i32 pbase = pd[wk.u0];
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         const u32 M = (u32)((i32)C + P[k]);
         const u64 *U = buf + C;
         for (u32 r = 0; r < SPL; ++r) {
-            u32 t = lane_g * SPL + r;
+            u32 t = r * G + lane_g;
             if (t >= C) break;
             u64 x = buf[t];
             u32 lo = 0, hi = k;
@@ -725,7 +727,8 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
             }
             if (lo < k && U[lo] == x) continue;  // deleted
             u32 o = t + P[lo];
-            if (o < S) {
+            // a key before every update keeps its slot: nothing to write
+            if (o != t && o < S) {
                 d.keys[base + o] = x;
                 if (is_marker(x) && (u32)(x >> 32) < d.V) {
                     d.row_off[(u32)(x >> 32) + 1] = (u32)(base + o);
