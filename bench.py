@@ -662,6 +662,7 @@ def main():
     ap.add_argument("--segment-slots", type=int, default=32)
     ap.add_argument("--seg-slots-per-lane", type=int, default=0, help="single-segment rebalance shape (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-sweep", type=int, default=1, help="also run batch sizes 1e3..1e6 (config 2)")
     ap.add_argument("--cpu-batches", type=int, default=15)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -730,6 +731,31 @@ def main():
                                                     "rebuild_ms", "pyramid_ms")}
     windows = {str(l): int(st.windows_per_level[l]) for l in range(32) if st.windows_per_level[l]}
 
+    # config 2 reports every batch size: the same stream cut into batches of
+    # 1e3 (a fixed window of 1000 batches), 1e4, 1e5 and 1e6, each from a
+    # fresh bulk load, with the rebalance kernels' algorithmic GB/s
+    sweep = []
+    if args.batch_sweep:
+        for Bs in (1000, 10000, 100000, 1000000):
+            bulk_s, ins_s, del_s = make_workload(args.scale, args.edges, 2, 12, Bs, 13)
+            ins_ds = [L.to_device(b, dev) for b in ins_s]
+            del_ds = [L.to_device(b, dev) for b in del_s]
+            n_steps = 1000 if Bs == 1000 else len(ins_s) + len(del_s)
+            seq_s = step_list(ins_s, del_s, n_steps + 2)
+            rs = run_gpu(bulk_s, ins_ds, del_ds, seq_s, 2, V, args.segment_slots, profile=True,
+                         slots_per_lane=args.seg_slots_per_lane)
+            stt, pff = rs["stats"], rs["prof"]
+            rms = pff.rebalance_seg_ms + pff.rebalance_wave_ms + pff.rebalance_block_ms + pff.large_ms
+            rbytes = stt.alg_bytes_seg + stt.alg_bytes_wave + stt.alg_bytes_block + stt.alg_bytes_large
+            sweep.append({"batch": Bs, "batches": n_steps, "updates_per_s": rs["n_updates"] / rs["wall"],
+                          "p50_batch_ms": float(np.percentile(rs["lat_ms"], 50)),
+                          "p99_batch_ms": float(np.percentile(rs["lat_ms"], 99)),
+                          "rebalance_alg_GBps": rbytes / (rms / 1e3) / 1e9 if rms > 0 else 0.0,
+                          "note": "profiled pass (per-phase HIP events on)"})
+            log(f"batch {Bs}: {sweep[-1]['updates_per_s']:.3e} updates/s, p50 {sweep[-1]['p50_batch_ms']:.3f} ms, "
+                f"rebalance {sweep[-1]['rebalance_alg_GBps']:.0f} GB/s")
+            del ins_ds, del_ds
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -766,6 +792,7 @@ def main():
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
                      "launches": int(launches)},
         "cpu_baseline": cpu,
+        "by_batch_size": sweep,
         "windows_per_level": windows,
         "rebuilds": int(st.rebuilds),
         "final_state": {"edges": r["edges"], "capacity": r["capacity"], "tree_height": r["height"]},
