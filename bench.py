@@ -14,9 +14,10 @@ edges are re-inserted) when more steps are asked for than it holds.
 
 Prints one JSON line.  `value` is edge updates per second over the K timed
 batches (inputs resident in HBM, wall clock between device synchronisations).
-`roofline` describes the dominant rebalance kernel (algorithmic bytes per
-launch / average launch time, from HIP events the library records around
-each launch on the update stream in a second, profiled pass).
+`roofline` describes the dominant rebalance kernel, the one that moves the
+most algorithmic bytes (bytes per launch / average launch time, from HIP
+events the library records around each launch on the update stream in a
+second, profiled pass).
 `cpu_baseline` times the CPU oracle (sorted merge + row offsets, pthreads)
 on a bounded sample of the same batches.
 
@@ -713,13 +714,13 @@ def main():
     value = r["n_updates"] / r["wall"]
     lat = r["lat_ms"]
     st, pf = rp["stats"], rp["prof"]
-    # dominant rebalance kernel by time
+    # dominant rebalance kernel: the one that moves the most algorithmic bytes
     kern = {
         "k_rebalance_seg": (pf.rebalance_seg_ms, st.alg_bytes_seg, pf.launches_seg),
         "k_rebalance_wave": (pf.rebalance_wave_ms, st.alg_bytes_wave, pf.launches_wave),
         "k_rebalance_block": (pf.rebalance_block_ms, st.alg_bytes_block, pf.launches_block),
     }
-    name = max(kern, key=lambda k: kern[k][0])
+    name = max(kern, key=lambda k: kern[k][1])
     ms, byts, launches = kern[name]
     avg_s = ms / 1e3 / max(launches, 1)
     bytes_per_launch = byts / max(launches, 1)

@@ -245,7 +245,8 @@ int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream);
 /* Workgroup shape of the single-segment window rebalance: slots handled by
  * each lane (a power of two; 0 restores the default, 4 slots per lane up to
  * 256-slot segments, a full wave per segment above).  The lanes of one
- * window never exceed a wave; the LDS of a workgroup bounds it too.  No
+ * window never exceed a wave and handle at most 16 slots each (larger
+ * requests are lowered to 16); the LDS of a workgroup bounds it too.  No
  * counterpart in the reference (one warp per window, rpma.cuh:1226-1243). */
 int lpma_set_launch(lpma_t *h, uint32_t seg_slots_per_lane);
 

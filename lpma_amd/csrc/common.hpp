@@ -29,6 +29,9 @@ typedef uint64_t u64;
 typedef uint32_t u32;
 typedef int32_t i32;
 typedef int64_t i64;
+// a u64 in global memory: accesses through it compile to global (not flat)
+// instructions, which only the vector memory counter tracks
+typedef __attribute__((address_space(1))) u64 gu64;
 
 static constexpr u64 KEY_NONE = ~0ull;   // empty slot (rpma.cuh:38)
 // run key of a batch element: pma << 32 | segment, bit 63 the operation
@@ -39,6 +42,10 @@ static constexpr u64 RK_OP = 1ull << 63;
 static constexpr u32 FLAG_INVALID_KEY = 1u;   // src or dst >= V
 static constexpr u32 FLAG_ROOT = 2u;          // a run chose the root window
 static constexpr u32 FLAG_RESIZE = 4u;        // the root window does not fit
+static constexpr u32 FLAG_SORT_STALL = 8u;    // a sort tile waited too long (the batch fails)
+
+// most passes of the batch sort (11-bit digits over 64 bits)
+static constexpr u32 SORT_MAX_PASSES = 6;
 
 // rebalance work classes
 static constexpr int CLASS_WAVE = 0;   // one wave64 per window

@@ -8,6 +8,7 @@ namespace pma {
 u64 scan_scratch_words(u64 n);
 hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
 u64 sort_scratch_words(u64 n);
+u32 sort_zero_words();
 u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
                    u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
 
@@ -35,7 +36,9 @@ struct InlineSegs {
 };
 
 u32 locate_blocks(u64 n);
-hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, hipStream_t st);
+// also clears zero[0, nzero) (the sort's digit totals)
+hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, u32 *zero, u32 nzero,
+                       hipStream_t st);
 hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
 // fixed grids for graph replays (work counts read on the device)
 hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st);
@@ -43,6 +46,9 @@ hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t 
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
 hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st);
+// grid sized for at most `bound` windows, the count read from the status on
+// the device (launches queued before the host knows it)
+hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st);
 u32 seg_lanes_log2(u32 log2S, int slots_log2);
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st);
 u32 pyramid_blocks(u32 H);

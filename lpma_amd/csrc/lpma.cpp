@@ -181,9 +181,21 @@ struct Pipeline {
     std::vector<hipEvent_t> ev_pool;
     double phase_ms[PH_NUM] = {0};
     u64 launches_wave = 0, launches_block = 0, launches_seg = 0;
+
+    hipEvent_t ev_status = nullptr;  // the round's status has reached the host
+    u32 last_wave_windows = 0;       // wave-class windows of the last round
 };
 
 static constexpr u64 STATUS_BYTES = (sizeof(Status) + 15) / 16 * 16;
+// workgroups of the wave-class rebalance queued before its window count is
+// known (each loops over windows): the last round's count with a quarter
+// more, at least 256, at most one per batch key and 16384
+static u32 early_wave_grid(const Pipeline &pl, u64 n) {
+    u64 g = (u64)pl.last_wave_windows + pl.last_wave_windows / 4 + 64;
+    g = std::max<u64>(g, 256);
+    g = std::min<u64>(g, 16384);
+    return (u32)std::max<u64>(1, std::min<u64>(g, n));
+}
 
 // ------------------------------------------------------------------ handle
 
@@ -343,6 +355,8 @@ void pipeline_free(Pipeline &pl) {
     collect_events(pl);
     for (auto e : pl.ev_pool) (void)hipEventDestroy(e);
     pl.ev_pool.clear();
+    if (pl.ev_status) (void)hipEventDestroy(pl.ev_status);
+    pl.ev_status = nullptr;
     int d = pl.device;
     release_on(d, pl.pk0); release_on(d, pl.pk1); release_on(d, pl.ukey); release_on(d, pl.useg);
     release_on(d, pl.ckey); release_on(d, pl.cseg); release_on(d, pl.merged); release_on(d, pl.lmbase);
@@ -615,12 +629,17 @@ int upload_table(Pipeline &pl, const Member *mem, u32 npma, u32 &pyr_blocks, hip
 // After the round's status is on the host: root rebuilds / resizes of the
 // flagged members (checked before anything changes), then the bookkeeping.
 // `windows_done` tells whether the window rebalances and the pyramid have
-// already been queued (graph replays) or are still to be launched here.
+// already been queued (graph replays) or are still to be launched here;
+// `early` that the single-segment and wave classes were queued before the
+// status was read (they only touch members without a root rebuild or
+// resize, whose windows the election drops).
 int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 pyr_blocks, bool windows_done,
-                 hipStream_t st) {
+                 hipStream_t st, bool early = false) {
     const Status s = *(const Status *)pl.h_out;
     const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
+    pl.last_wave_windows = s.nwork[CLASS_WAVE];
     if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
+    if (s.flags & FLAG_SORT_STALL) return LPMA_ERR_DEVICE;
     const u64 n_eff = s.n_eff;
     if (n_eff == 0) {
         for (u32 p = 0; p < npma; ++p) mem[p].h->version++;
@@ -646,12 +665,12 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
     BatchBufs bb = batch_bufs(pl);
 
     if (!windows_done) {
-        if (s.nwork[CLASS_SEG]) {
+        if (s.nwork[CLASS_SEG] && !early) {
             PhaseTimer t(pl, st, PH_SEG);
             HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st));
             pl.launches_seg++;
         }
-        if (s.nwork[CLASS_WAVE]) {
+        if (s.nwork[CLASS_WAVE] && !early) {
             PhaseTimer t(pl, st, PH_WAVE);
             HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
             pl.launches_wave++;
@@ -818,7 +837,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     u64 *pk;
     {
         PhaseTimer t(pl, st, PH_SORT);
-        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, st));
+        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), st));
         pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
                            pl.sort_scr.p, &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
@@ -837,8 +856,30 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     }
     const u64 out_bytes = STATUS_BYTES + (u64)npma * sizeof(PmaStat);
     HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    return finish_round(pl, mem, npma, g, pyr_blocks, false, st);
+    // A lone PMA's single-segment and wave windows are queued before the
+    // host reads the status, with grids sized for the batch and the window
+    // counts read on the device, so they run while the host waits for the
+    // status and queues the rest: a PMA with a root rebuild or resize has no
+    // windows, so a round that fails after the status changes nothing.
+    const bool early = npma == 1;
+    if (early) {
+        if (!pl.ev_status) HIPCHECK(hipEventCreateWithFlags(&pl.ev_status, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(pl.ev_status, st));
+        {
+            PhaseTimer t(pl, st, PH_SEG);
+            HIPCHECK(rebalance_seg_bound(g, bb, n, st));
+            pl.launches_seg++;
+        }
+        {
+            PhaseTimer t(pl, st, PH_WAVE);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, early_wave_grid(pl, n), st));
+            pl.launches_wave++;
+        }
+        HIPCHECK(hipEventSynchronize(pl.ev_status));
+    } else {
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    return finish_round(pl, mem, npma, g, pyr_blocks, false, st, early);
 }
 
 // Graph replay of a round: the host writes the batch descriptors and the
@@ -930,7 +971,7 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
         bool ok = hipMemcpyAsync(pl.rhdr.p, pl.h_rhdr, sizeof(RoundHdr), hipMemcpyHostToDevice, cs) == hipSuccess &&
                   hipMemcpyAsync(pl.segs.p, pl.h_segs, nsegs * sizeof(BatchSeg), hipMemcpyHostToDevice, cs) ==
                       hipSuccess &&
-                  batch_begin(g, bb.status, none, pl.segs.p, cs) == hipSuccess;
+                  batch_begin(g, bb.status, none, pl.segs.p, pl.sort_scr.p, sort_zero_words(), cs) == hipSuccess;
         if (ok) {
             u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, 0, hi_bit, pl.pk0.p, pl.pk1.p,
                                     pl.sort_scr.p, &bb.status->flags, cs);

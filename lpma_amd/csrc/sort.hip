@@ -222,6 +222,233 @@ __global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ key
     for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) keys[i] = buf[cur][i];
 }
 
+// ------------------------------------------------- one kernel per pass
+//
+// Batches of 8K..16M keys sort with DB = 10 or 11-bit digits (four passes
+// for the 40-41 bits of a lone PMA with 2^20 vertices), one kernel each:
+// the pack kernel counts the digits of every pass at once (global digit
+// totals), and a pass kernel ranks its tile in LDS, publishes the tile's
+// digit counts, reads the counts of the tiles before it (decoupled look-back,
+// Merrill & Garland, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back", NVR-2016-002) and scatters.  A published word is the count in
+// bits 0..29 with OS_AGG (this tile only) or OS_PRE (this tile and every
+// tile before it) in bits 30..31; 0 means not yet published.
+//
+// Tile t waits only on tiles < t.  Workgroups are dispatched in index order,
+// so every tile it waits on is running or done; the wait is still bounded
+// (OS_SPIN_LIMIT rounds), after which the batch is flagged FLAG_SORT_STALL
+// and fails without changing anything.
+
+static constexpr int OS_TPB = 256;
+static constexpr u32 OS_AGG = 1u << 30;
+static constexpr u32 OS_PRE = 2u << 30;
+static constexpr u32 OS_CNT = (1u << 30) - 1;
+static constexpr u32 OS_SPIN_LIMIT = 1u << 20;
+static constexpr u64 OS_MAX_N = 1ull << 24;       // larger batches: the two-kernel passes
+static constexpr u32 OS_GH_WORDS = SORT_MAX_PASSES * 2048;
+
+__device__ __forceinline__ u32 os_load(const u32 *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void os_store(u32 *p, u32 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pack (as k_pack_group) and count the digits of all npass passes:
+// gh[p * R + d] += keys whose pass-p digit is d (gh zeroed by the caller);
+// also clears the look-back row of this tile for the first pass.
+template <int DB>
+__global__ __launch_bounds__(OS_TPB) void k_pack_count(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
+                                                        const u64 *n_dev, u32 pshift, u32 sbits, u64 *__restrict__ out,
+                                                        u32 *__restrict__ flags, u32 lo_bit, u32 npass, u32 items,
+                                                        u32 *__restrict__ gh, u32 *__restrict__ look0) {
+    constexpr u32 R = 1u << DB;
+    __shared__ u32 h[SORT_MAX_PASSES][R];
+    if (n_dev) n = *n_dev;
+    for (u32 i = threadIdx.x; i < npass * R; i += OS_TPB) (&h[0][0])[i] = 0;
+    for (u32 i = threadIdx.x; i < R; i += OS_TPB) look0[(u64)blockIdx.x * R + i] = 0;
+    __syncthreads();
+    const u64 base = (u64)blockIdx.x * items * OS_TPB + threadIdx.x;
+    bool bad = false;
+    if (base < n) {
+        u32 s = source_of(segs, nsegs, base);
+        BatchSeg sg = segs[s];
+        for (u32 r = 0; r < items; ++r) {
+            u64 i = base + (u64)r * OS_TPB;
+            if (i >= n) break;
+            while (i >= sg.off + sg.n) sg = segs[++s];
+            u64 k = sg.keys[i - sg.off];
+            u32 src = (u32)(k >> 32), dst = (u32)k;
+            if (src >= sg.V || dst >= sg.V) { bad = true; src = 0; dst = 0; }
+            u64 p = ((u64)sg.pid << pshift) | ((((((u64)src) << sg.vbits) | dst) << sbits | sg.seq) << 1) | sg.op;
+            out[i] = p;
+            for (u32 q = 0; q < npass; ++q) atomicAdd(&h[q][(u32)(p >> (lo_bit + q * DB)) & (R - 1)], 1u);
+        }
+    }
+    if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < npass * R; i += OS_TPB) {
+        u32 v = (&h[0][0])[i];
+        if (v) atomicAdd(&gh[i], v);
+    }
+}
+
+// One pass over bits [shift, shift + DB).  Tile t = keys [t*T, t*T + T),
+// T = OS_TPB * ITEMS; wave w holds the 64*ITEMS consecutive keys from
+// t*T + w*64*ITEMS, one per lane per round, so (wave, round, lane) order is
+// input order and the ranking is stable.  W predecessors are read per
+// look-back step.
+template <int DB, int ITEMS, int W>
+__global__ __launch_bounds__(OS_TPB) void k_os_pass(const u64 *__restrict__ in, u64 *__restrict__ out, u64 n,
+                                                     const u64 *n_dev, u32 shift, const u32 *__restrict__ gh,
+                                                     u32 *__restrict__ look, u32 *__restrict__ look_next,
+                                                     u32 *__restrict__ flags) {
+    constexpr u32 R = 1u << DB;
+    constexpr u32 NW = OS_TPB / 64;
+    constexpr u32 DPT = R / OS_TPB;  // digits per thread
+    __shared__ u32 whist[NW][R];     // per-wave digit counts, then per-wave offsets
+    __shared__ u32 gofs[R];          // keys of smaller digits in the whole batch
+    __shared__ u32 tofs[R];          // keys of this digit in earlier tiles
+    __shared__ u32 stmp[NW + 1];
+    if (n_dev) n = *n_dev;
+    const u32 t = blockIdx.x;
+    if (look_next)
+        for (u32 i = threadIdx.x; i < R; i += OS_TPB) look_next[(u64)t * R + i] = 0;
+    const u64 tbase = (u64)t * OS_TPB * ITEMS;
+    if (tbase >= n) return;  // whole workgroup: grids sized for a capacity
+    const u32 lane = lane_id();
+    const u32 w = threadIdx.x >> 6;
+    const u64 lt = (1ull << lane) - 1ull;
+
+    // batch-wide digit totals of this pass, contiguous per thread
+    u32 gv[DPT];
+#pragma unroll
+    for (u32 j = 0; j < DPT; ++j) gv[j] = gh[threadIdx.x * DPT + j];
+    u64 key[ITEMS];
+    const u64 wbase = tbase + (u64)w * 64 * ITEMS + lane;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        u64 i = wbase + (u64)r * 64;
+        key[r] = i < n ? in[i] : 0ull;
+    }
+    for (u32 i = threadIdx.x; i < NW * R; i += OS_TPB) (&whist[0][0])[i] = 0;
+    __syncthreads();
+
+    // rank inside the wave: lanes of equal digit found with DB ballots; the
+    // wave's running count of each digit lives in whist[w] (one wave reads
+    // and then updates its own row, in order)
+    u32 rk[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const bool valid = wbase + (u64)r * 64 < n;
+        const u32 dg = (u32)(key[r] >> shift) & (R - 1);
+        u64 peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < DB; ++b) {
+            u64 bb = __ballot((dg >> b) & 1u);
+            peers &= ((dg >> b) & 1u) ? bb : ~bb;
+        }
+        const u32 below = __popcll(peers & lt);
+        const u32 before = whist[w][dg];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) whist[w][dg] = before + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rk[r] = before + below;
+    }
+    __syncthreads();
+
+    // per digit: offsets of the waves, the tile's count, published at once
+    u32 cnt[DPT];
+#pragma unroll
+    for (u32 j = 0; j < DPT; ++j) {
+        const u32 d = j * OS_TPB + threadIdx.x;
+        u32 run = 0;
+#pragma unroll
+        for (u32 v = 0; v < NW; ++v) {
+            u32 c = whist[v][d];
+            whist[v][d] = run;
+            run += c;
+        }
+        cnt[j] = run;
+        os_store(&look[(u64)t * R + d], (t == 0 ? OS_PRE : OS_AGG) | run);
+    }
+    // exclusive scan of the digit totals
+    {
+        u32 sum = 0;
+#pragma unroll
+        for (u32 j = 0; j < DPT; ++j) sum += gv[j];
+        u32 tot;
+        u32 ex = block_excl_scan<OS_TPB, u32>(sum, tot, stmp);
+#pragma unroll
+        for (u32 j = 0; j < DPT; ++j) { gofs[threadIdx.x * DPT + j] = ex; ex += gv[j]; }
+    }
+
+    // look-back: keys of each digit in tiles < t
+    u32 excl[DPT];
+    int p[DPT];
+#pragma unroll
+    for (u32 j = 0; j < DPT; ++j) { excl[j] = 0; p[j] = (int)t - 1; }
+    bool stalled = false;
+    for (u32 spin = 0;; ++spin) {
+        u32 s[DPT][W];
+#pragma unroll
+        for (u32 j = 0; j < DPT; ++j) {
+            const u32 d = j * OS_TPB + threadIdx.x;
+            // unconditional loads (row 0 stands in past the first tile) so
+            // that all W * DPT of them are in flight together
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const int q = p[j] - k;
+                s[j][k] = os_load(&look[(u64)(q > 0 ? q : 0) * R + d]);
+            }
+        }
+        // consume predecessors in order until one is unpublished (read it
+        // again next round) or carries the prefix of all before it (done);
+        // branch-free so that s stays in registers
+        bool pending = false;
+#pragma unroll
+        for (u32 j = 0; j < DPT; ++j) {
+            int q = p[j];
+            bool go = q >= 0;
+            u32 acc = 0;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const u32 v = s[j][k];
+                const u32 f = v & ~OS_CNT;
+                const bool use = go && f != 0;
+                acc += use ? (v & OS_CNT) : 0u;
+                q = use ? (f == OS_PRE ? -1 : q - 1) : q;
+                go = use && f != OS_PRE;
+            }
+            excl[j] += acc;
+            p[j] = q;
+            pending |= q >= 0;
+        }
+        if (!pending) break;
+        if (spin >= OS_SPIN_LIMIT) {
+            stalled = true;
+            break;
+        }
+    }
+    if (stalled) atomicOr(flags, FLAG_SORT_STALL);
+#pragma unroll
+    for (u32 j = 0; j < DPT; ++j) {
+        const u32 d = j * OS_TPB + threadIdx.x;
+        tofs[d] = excl[j];
+        if (t != 0) os_store(&look[(u64)t * R + d], OS_PRE | ((excl[j] + cnt[j]) & OS_CNT));
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        if (wbase + (u64)r * 64 >= n) break;
+        const u32 dg = (u32)(key[r] >> shift) & (R - 1);
+        u64 pos = (u64)gofs[dg] + tofs[dg] + whist[w][dg] + rk[r];
+        if (pos >= n) pos = n - 1;  // only after a stall (flagged): stay in bounds
+        out[pos] = key[r];
+    }
+}
+
 // ------------------------------------------------------ device-wide scan
 
 static constexpr int SC_TPB = 256;
@@ -303,15 +530,54 @@ hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStr
 // small tiles keep >= ~2000 workgroups in flight up to 8M keys
 static u32 rs_items(u64 n) { return n <= (1ull << 23) ? RS_ITEMS_SMALL : RS_ITEMS_BIG; }
 
+// one-kernel passes: tiles of 2048 keys up to 2M keys, 4096 above
+static u32 os_items(u64 n) { return n <= (1ull << 21) ? 8 : 16; }
+
+// digit width for a sort over `width` bits: 10 unless 11 saves a pass
+static u32 os_digit_bits(u32 width) { return (width + 9) / 10 == (width + 10) / 11 ? 10 : 11; }
+
 u64 sort_scratch_words(u64 n) {
     u64 nb = (n + (u64)rs_items(n) * RS_TPB - 1) / ((u64)rs_items(n) * RS_TPB);
-    return (u64)RS_RADIX * nb + 1 + scan_scratch_words((u64)RS_RADIX * nb);
+    u64 two = (u64)RS_RADIX * nb + 1 + scan_scratch_words((u64)RS_RADIX * nb);
+    u64 tile = (u64)os_items(n) * OS_TPB;
+    u64 one = 2 * ((n + tile - 1) / tile) * 2048;
+    return OS_GH_WORDS + (n <= OS_MAX_N ? std::max(two, one) : two);
+}
+
+u32 sort_zero_words() { return OS_GH_WORDS; }
+
+template <int DB>
+static void os_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
+                    u32 npass, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, u64 **result, hipStream_t st) {
+    constexpr u32 R = 1u << DB;
+    const u32 items = os_items(n);
+    const u64 tile = (u64)items * OS_TPB;
+    const u32 ntiles = (u32)((n + tile - 1) / tile);
+    u32 *gh = scratch;
+    u32 *look[2] = {scratch + OS_GH_WORDS, scratch + OS_GH_WORDS + (u64)ntiles * R};
+    k_pack_count<DB><<<ntiles, OS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, k0, flags, lo_bit, npass, items,
+                                                gh, look[0]);
+    u64 *src = k0, *dst = k1;
+    for (u32 q = 0; q < npass; ++q) {
+        const u32 shift = lo_bit + q * DB;
+        u32 *nxt = q + 1 < npass ? look[(q + 1) & 1] : nullptr;
+        if (items == 8)
+            k_os_pass<DB, 8, (DB == 10 ? 16 : 8)><<<ntiles, OS_TPB, 0, st>>>(src, dst, n, n_dev, shift, gh + q * R,
+                                                                          look[q & 1], nxt, flags);
+        else
+            k_os_pass<DB, 16, (DB == 10 ? 16 : 8)><<<ntiles, OS_TPB, 0, st>>>(src, dst, n, n_dev, shift, gh + q * R,
+                                                                           look[q & 1], nxt, flags);
+        u64 *t = src; src = dst; dst = t;
+    }
+    *result = src;
 }
 
 // Packs a group batch (n elements from nsegs sources) and sorts the packed
 // keys on bits [lo_bit, hi_bit): the first histogram is taken while packing.
 // With n_dev the kernels read the element count from device memory and n
-// is only a capacity that sizes the grids (graph replays).
+// is only a capacity that sizes the grids (graph replays).  The first
+// sort_zero_words() words of `scratch` must be zero on entry (the batch's
+// first kernel clears them).
 u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
                    u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
     if (n == 0) return k0;
@@ -322,6 +588,18 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
         if (n > 1 || n_dev) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, n_dev, lo_bit, hi_bit);
         return k0;
     }
+    if (n <= OS_MAX_N) {
+        const u32 width = hi_bit - lo_bit;
+        const u32 db = os_digit_bits(width);
+        const u32 npass = (width + db - 1) / db;
+        u64 *res = k0;
+        if (db == 10)
+            os_sort<10>(segs, nsegs, n, n_dev, pshift, sbits, lo_bit, npass, k0, k1, scratch, flags, &res, st);
+        else
+            os_sort<11>(segs, nsegs, n, n_dev, pshift, sbits, lo_bit, npass, k0, k1, scratch, flags, &res, st);
+        return res;
+    }
+    scratch += OS_GH_WORDS;
     const u32 items = rs_items(n);
     const u64 tile = (u64)items * RS_TPB;
     const u32 nblk = (u32)((n + tile - 1) / tile);

@@ -37,10 +37,12 @@ u32 locate_blocks(u64 n) { return (u32)((n + LOC_TILE - 1) / LOC_TILE); }
 // the previous batch: the levels above the per-workgroup chunks of
 // k_pyr_chunk (nseg >> 11 entries and fewer), so no separate launch is
 // needed for them.
-__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs);
+__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs, u32 *zero,
+                                                      u32 nzero);
 
-hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, hipStream_t st) {
-    k_batch_begin<<<g.npma, 1024, 0, st>>>(g, status, in, segs);
+hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, u32 *zero, u32 nzero,
+                       hipStream_t st) {
+    k_batch_begin<<<g.npma, 1024, 0, st>>>(g, status, in, segs, zero, nzero);
     return hipGetLastError();
 }
 
@@ -69,7 +71,8 @@ __global__ __launch_bounds__(LOC_TPB) void k_locate(Group g, const u64 *__restri
         const u64 p = pk[i];
         const u32 ks = 1 + g.sbits;  // key (and pid) above the request order and op
         const bool dup = (i + 1 < n) && ((pk[i + 1] >> ks) == (p >> ks));
-        if (dup) {
+        // a PMA id out of range only follows a failed sort (FLAG_SORT_STALL)
+        if (dup || (u32)(p >> g.pshift) >= g.npma) {
             useg[i] = RK_NONE;
         } else {
             const u32 pid = (u32)(p >> g.pshift);
@@ -319,7 +322,7 @@ template <bool ONE>
 __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
                           Status *status) {
-    if (status->flags & FLAG_INVALID_KEY) return;
+    if (status->flags & (FLAG_INVALID_KEY | FLAG_SORT_STALL)) return;
     const u32 n_eff = status->n_eff;
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
@@ -673,29 +676,50 @@ __global__ __launch_bounds__(256) void k_rebalance_block(Group g, const Work *__
                                                     ST_ALG_BYTES_BLOCK);
 }
 
-// Single-segment windows (level 0, E <= 2S): a group of G = S/4 lanes
-// (at most 64) rebalances one segment, so a wave works on 64/G segments at
-// once and no workgroup barrier is needed.  With one segment the even spread
-// is the merged order itself, front-packed, so every old key and every
-// insert stores straight to its rank; the old keys were staged in the
-// group's LDS slice first, so the in-place stores cannot overwrite a key
-// another lane has yet to read.
+// Single-segment windows (level 0, E <= 2S): a group of G = S/SPL lanes (at
+// most 64) rebalances one segment, SPL slots per lane, so a wave works on
+// 64/G segments at once and no workgroup barrier is needed.  With one
+// segment the even spread is the merged order itself, front-packed.  Every
+// global load of a window (its S slots, its first G updates and their
+// prefixes) is issued before the first of them is used, so a window costs
+// one memory latency, and the old keys stay in registers: for each update
+// in turn the group compares it with all of its keys at once, which gives
+// every old key its shift (the net inserts before it) and the update its
+// rank among the old keys (a popcount of the group's ballot), with no
+// dependent LDS reads.  Only the updates and their prefixes go through the
+// group's LDS slice.  Every key is loaded before the first store, so the
+// in-place stores cannot overwrite a key another lane has yet to read.
 static constexpr int SEG_TPB = 256;
+// persistent grid: four workgroups per CU of 256 at most; each group loops
+// over windows (fewer, longer-lived workgroups keep more of the device busy
+// than one short workgroup per 32 windows)
+static constexpr u32 SEG_GRID_MAX = 1024;
 
+// u32 words of a group's LDS slice: the updates (k <= 2S, u64) and their
+// prefixes (k + 1 <= 2S + 1, i32), padded so that the slices of a wave's
+// groups start on different banks
+__host__ __device__ constexpr u32 seg_slice_words(u32 S) { return 4 * S + (2 * S + 2) + 16; }
+
+template <int SPL>
 __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *__restrict__ work,
                                                            const Status *__restrict__ status,
                                                            const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
-                                                           u32 log2G) {
+                                                           u32 log2G, u32 nw_host) {
     extern __shared__ u64 lds[];
+    __shared__ u64 red[4];
     const u32 G = 1u << log2G;
-    const u32 S = 1u << g.log2S;
-    const u32 SPL = S >> log2G;  // slots per lane
+    const u32 S = G * SPL;
     const u32 lane_g = threadIdx.x & (G - 1);
     const u32 grp = threadIdx.x >> log2G;
     const u32 groups_per_block = SEG_TPB >> log2G;
-    u64 *buf = lds + (u64)grp * (2 * S);                                   // old keys then updates
-    i32 *P = (i32 *)(lds + (u64)groups_per_block * 2 * S) + (u64)grp * (2 * S + 2);  // update prefix
-    const u32 nw = status->nwork[CLASS_SEG];
+    u64 *U = (u64 *)((u32 *)lds + (u64)grp * seg_slice_words(S));  // updates
+    i32 *P = (i32 *)(U + 2 * S);                                      // their prefix
+    // the group's lanes within the wave
+    const u32 gshift = (threadIdx.x & 63) & ~(G - 1);
+    const u64 gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gshift;
+    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    // the host passes the window count when it knows it
+    const u32 nw = nw_host != ~0u ? nw_host : status->nwork[CLASS_SEG];
     u64 bytes = 0, upd = 0, markers = 0, wins = 0;
     for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += gridDim.x * groups_per_block) {
         const Work wk = work[w];
@@ -703,52 +727,64 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         const u32 C = wk.C;
         const u32 k = wk.u1 - wk.u0;
         const u64 base = (u64)wk.a << g.log2S;
-        // lane-contiguous: each load instruction of the group covers G
-        // consecutive slots
-        for (u32 r = 0; r < SPL; ++r) {
-            u32 t = r * G + lane_g;
-            if (t < C) buf[t] = d.keys[base + t];
-        }
+        gu64 *keys = (gu64 *)d.keys + base;
+        u64 v[SPL];
+#pragma unroll
+        for (int r = 0; r < SPL; ++r) v[r] = keys[r * G + lane_g];  // all S slots exist
         const i32 pbase = pd[wk.u0];
-        for (u32 q = lane_g; q < k; q += G) buf[C + q] = ckey[wk.u0 + q];
-        for (u32 q = lane_g; q <= k; q += G) P[q] = pd[wk.u0 + q] - pbase;
-        __threadfence_block();
+        const u64 uk = lane_g < k ? ckey[wk.u0 + lane_g] : 0ull;
+        const i32 up = lane_g <= k ? pd[wk.u0 + lane_g] : 0;
+        if (lane_g < k) U[lane_g] = uk;
+        if (lane_g <= k) P[lane_g] = up - pbase;
+        for (u32 q = lane_g + G; q < k; q += G) U[q] = ckey[wk.u0 + q];
+        for (u32 q = lane_g + G; q <= k; q += G) P[q] = pd[wk.u0 + q] - pbase;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const u32 M = (u32)((i32)C + P[k]);
-        const u64 *U = buf + C;
-        for (u32 r = 0; r < SPL; ++r) {
-            u32 t = r * G + lane_g;
-            if (t >= C) break;
-            u64 x = buf[t];
-            u32 lo = 0, hi = k;
-            while (lo < hi) {
-                u32 m = (lo + hi) >> 1;
-                if (U[m] < x) lo = m + 1; else hi = m;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        // slots past C read as the largest key: never before an update
+#pragma unroll
+        for (int r = 0; r < SPL; ++r) v[r] = r * G + lane_g < C ? v[r] : KEY_NONE;
+        i32 shift[SPL];
+        u32 gone = 0;  // bit r: old key r is deleted
+#pragma unroll
+        for (int r = 0; r < SPL; ++r) shift[r] = 0;
+        i32 pq = 0;
+        for (u32 q = 0; q < k; ++q) {
+            const u64 y = U[q];
+            const i32 pn = P[q + 1];
+            const i32 sgn = pn - pq;  // +1 insert, -1 delete
+            u32 below = 0;            // old keys of the window before y
+#pragma unroll
+            for (int r = 0; r < SPL; ++r) {
+                shift[r] += y < v[r] ? sgn : 0;
+                gone |= y == v[r] ? 1u << r : 0u;  // only deletes meet a present key
+                below += __popcll(__ballot(v[r] < y) & gmask);
             }
-            if (lo < k && U[lo] == x) continue;  // deleted
-            u32 o = t + P[lo];
+            if (sgn == 1 && lane_g == (q & (G - 1))) {
+                const u32 o = below + (u32)pq;
+                if (o < S) keys[o] = y;  // inserts are never markers
+            }
+            pq = pn;
+        }
+        const u32 M = (u32)((i32)C + pq);
+#pragma unroll
+        for (int r = 0; r < SPL; ++r) {
+            const u32 t = r * G + lane_g;
+            if (t >= C || ((gone >> r) & 1u)) continue;
+            const u32 o = (u32)((i32)t + shift[r]);
             // a key before every update keeps its slot: nothing to write
             if (o != t && o < S) {
-                d.keys[base + o] = x;
+                const u64 x = v[r];
+                keys[o] = x;
                 if (is_marker(x) && (u32)(x >> 32) < d.V) {
                     d.row_off[(u32)(x >> 32) + 1] = (u32)(base + o);
                     ++markers;
                 }
             }
         }
-        for (u32 q = lane_g; q < k; q += G) {
-            if (P[q + 1] - P[q] != 1) continue;
-            u64 y = U[q];
-            u32 lo = 0, hi = C;
-            while (lo < hi) {
-                u32 m = (lo + hi) >> 1;
-                if (buf[m] < y) lo = m + 1; else hi = m;
-            }
-            u32 o = lo + P[q];
-            if (o < S) d.keys[base + o] = y;  // inserts are never markers
-        }
         // slots emptied by deletes
-        for (u32 t = M + lane_g; t < C; t += G) d.keys[base + t] = KEY_NONE;
+        for (u32 t = M + lane_g; t < C; t += G) keys[t] = KEY_NONE;
         if (lane_g == 0) {
             d.cnt[wk.a] = M;
             bytes += 16ull * S + 8ull * k;
@@ -757,39 +793,68 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         }
         __builtin_amdgcn_wave_barrier();
     }
+    // statistics: one set of atomics per workgroup
     markers = wave_sum(markers);
     bytes = wave_sum(bytes);
     upd = wave_sum(upd);
     wins = wave_sum(wins);
-    u64 *row = g.stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    __syncthreads();
     if (lane_id() == 0 && wins) {
-        atomicAdd((unsigned long long *)&row[ST_WINDOWS + 0], (unsigned long long)wins);
-        atomicAdd((unsigned long long *)&row[ST_UPDATES], (unsigned long long)upd);
-        atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)markers);
-        atomicAdd((unsigned long long *)&row[ST_ALG_BYTES_SEG], (unsigned long long)(bytes + 4ull * markers));
+        atomicAdd((unsigned long long *)&red[0], (unsigned long long)wins);
+        atomicAdd((unsigned long long *)&red[1], (unsigned long long)upd);
+        atomicAdd((unsigned long long *)&red[2], (unsigned long long)markers);
+        atomicAdd((unsigned long long *)&red[3], (unsigned long long)(bytes + 4ull * markers));
+    }
+    __syncthreads();
+    u64 *row = g.stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    if (threadIdx.x == 0 && red[0]) {
+        atomicAdd((unsigned long long *)&row[ST_WINDOWS + 0], (unsigned long long)red[0]);
+        atomicAdd((unsigned long long *)&row[ST_UPDATES], (unsigned long long)red[1]);
+        atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)red[2]);
+        atomicAdd((unsigned long long *)&row[ST_ALG_BYTES_SEG], (unsigned long long)red[3]);
     }
 }
 
 // lanes per window (log2) of the single-segment rebalance: `slots_log2`
 // slots per lane, by default 4 (S/64 from 256-slot segments up), at most a
-// wave, and within the LDS of a workgroup
+// wave, at most 16 slots per lane, and within the LDS of a workgroup
 u32 seg_lanes_log2(u32 log2S, int slots_log2) {
     int g = slots_log2 < 0 ? (log2S >= 8 ? 6 : (int)log2S - 2) : (int)log2S - slots_log2;
+    if (g < (int)log2S - 4) g = (int)log2S - 4;
     if (g < 0) g = 0;
     if (g > 6) g = 6;
-    while (g < 6 && (size_t)(SEG_TPB >> g) * (24ull << log2S) + 8 * (SEG_TPB >> g) > 160u * 1024) ++g;
+    auto lds_bytes = [&](int gg) { return (size_t)(SEG_TPB >> gg) * seg_slice_words(1u << log2S) * 4; };
+    while (g < 6 && lds_bytes(g) > 160u * 1024) ++g;
     return (u32)g;
 }
 
-hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st) {
+// `windows`: the exact window count (eager rounds) or a bound that sizes the
+// grid, the kernel then reading the count from the status (graph replays)
+static hipError_t rebalance_seg_launch(const Group &g, const BatchBufs &b, u64 windows, bool exact, hipStream_t st) {
     u32 S = 1u << g.log2S;
     u32 log2G = g.seg_log2G;
     u32 groups_per_block = SEG_TPB >> log2G;
-    size_t lds = (size_t)groups_per_block * (2 * S) * sizeof(u64) + (size_t)groups_per_block * (2 * S + 2) * sizeof(i32);
+    size_t lds = (size_t)groups_per_block * seg_slice_words(S) * sizeof(u32);
     u64 blocks = (windows + groups_per_block - 1) / groups_per_block;
-    u32 grid = (u32)(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks));
-    k_rebalance_seg<<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G);
+    u32 grid = (u32)(blocks < 1 ? 1 : (blocks > SEG_GRID_MAX ? SEG_GRID_MAX : blocks));
+    u32 nw = exact ? (u32)windows : ~0u;
+    switch (S >> log2G) {
+        case 1: k_rebalance_seg<1><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        case 2: k_rebalance_seg<2><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        case 4: k_rebalance_seg<4><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        case 8: k_rebalance_seg<8><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        case 16: k_rebalance_seg<16><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st) {
+    return rebalance_seg_launch(g, b, windows, true, st);
+}
+
+hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st) {
+    return rebalance_seg_launch(g, b, bound, false, st);
 }
 
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
@@ -805,7 +870,7 @@ hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, 
 // runs_cap bounds them.  merged must hold the live keys of every PMA of
 // the group.
 hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st) {
-    (void)rebalance_seg(g, b, runs_cap < 65536u ? runs_cap : 65536u, st);
+    (void)rebalance_seg_launch(g, b, runs_cap < 65536u ? runs_cap : 65536u, false, st);
     const u32 gw = runs_cap < 4096 ? (runs_cap ? runs_cap : 1) : 4096;
     k_rebalance_wave<<<gw, 64, 0, st>>>(g, b.work[CLASS_WAVE], b.status, b.ckey, b.pd);
     const u32 gb = runs_cap < 1024 ? (runs_cap ? runs_cap : 1) : 1024;
@@ -1201,8 +1266,10 @@ __device__ void pyr_top_levels(const DevState &d) {
 
 __global__ __launch_bounds__(1024) void k_pyr_top(DevState d) { pyr_top_levels(d); }
 
-__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs) {
+__global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs, u32 *zero,
+                                                      u32 nzero) {
     if (blockIdx.x == 0) {
+        for (u32 i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
         if (threadIdx.x == 0) {
             s->flags = 0;
             s->n_eff = 0;
