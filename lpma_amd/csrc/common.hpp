@@ -177,7 +177,6 @@ struct Group {
     u32 inline_one; // one PMA whose geometry travels in `one` (no table load)
     const RoundHdr *rh;  // graph replays: n and epoch come from here
     DevState one;
-    Density one_dn;
 };
 
 __device__ __forceinline__ u32 group_epoch(const Group &g) { return g.rh ? g.rh->epoch : g.epoch; }

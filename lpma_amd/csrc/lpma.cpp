@@ -829,7 +829,6 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.rh = nullptr;
     g.inline_one = npma == 1 ? 1u : 0u;
     g.one = mem[0].h->st;
-    g.one_dn = mem[0].h->dn;
     BatchBufs bb = batch_bufs(pl);
     // bit 0 of a packed key is the operation (with the request order above
     // it): sorted only when a member mixes them or merges requests
@@ -855,12 +854,12 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         HIPCHECK(elect_windows(g, n, bb, st));
     }
     const u64 out_bytes = STATUS_BYTES + (u64)npma * sizeof(PmaStat);
-    HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
     // A lone PMA's single-segment and wave windows are queued before the
     // host reads the status, with grids sized for the batch and the window
     // counts read on the device, so they run while the host waits for the
     // status and queues the rest: a PMA with a root rebuild or resize has no
     // windows, so a round that fails after the status changes nothing.
+    HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
     const bool early = npma == 1;
     if (early) {
         if (!pl.ev_status) HIPCHECK(hipEventCreateWithFlags(&pl.ev_status, hipEventDisableTiming));
@@ -937,7 +936,6 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     // a captured graph outlives resizes: geometry always from the table
     g.inline_one = 0;
     memset(&g.one, 0, sizeof(g.one));
-    memset(&g.one_dn, 0, sizeof(g.one_dn));
     return g;
 }
 

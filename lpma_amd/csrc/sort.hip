@@ -224,8 +224,11 @@ __global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ key
 
 // ------------------------------------------------- one kernel per pass
 //
-// Batches of 8K..16M keys sort with DB = 10 or 11-bit digits (four passes
-// for the 40-41 bits of a lone PMA with 2^20 vertices), one kernel each:
+// Batches of 8K..4M keys sort with DB = 10 or 11-bit digits (four passes
+// for the 40-41 bits of a lone PMA with 2^20 vertices), one kernel each,
+// over tiles of 4096 keys (1024 threads, 4 keys each): few enough tiles that
+// a tile finds the counts of all tiles before it in few look-back steps,
+// small enough that a tile's 16 waves rank it in 4 rounds each:
 // the pack kernel counts the digits of every pass at once (global digit
 // totals), and a pass kernel ranks its tile in LDS, publishes the tile's
 // digit counts, reads the counts of the tiles before it (decoupled look-back,
@@ -239,13 +242,15 @@ __global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ key
 // (OS_SPIN_LIMIT rounds), after which the batch is flagged FLAG_SORT_STALL
 // and fails without changing anything.
 
-static constexpr int OS_TPB = 256;
+static constexpr int OS_TPB = 1024;  // large tiles: few tiles, short look-back walks
 static constexpr u32 OS_AGG = 1u << 30;
 static constexpr u32 OS_PRE = 2u << 30;
 static constexpr u32 OS_CNT = (1u << 30) - 1;
 static constexpr u32 OS_SPIN_LIMIT = 1u << 20;
-static constexpr u64 OS_MAX_N = 1ull << 24;       // larger batches: the two-kernel passes
+static constexpr u64 OS_MAX_N = 1ull << 22;       // larger batches: the two-kernel passes
 static constexpr u32 OS_GH_WORDS = SORT_MAX_PASSES * 2048;
+static constexpr int OS_ITEMS = 4;   // keys per thread: tiles of 4096
+static constexpr int OS_LOOK = 32;   // look-back loads in flight per thread
 
 __device__ __forceinline__ u32 os_load(const u32 *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -306,9 +311,10 @@ __global__ __launch_bounds__(OS_TPB) void k_os_pass(const u64 *__restrict__ in, 
     constexpr u32 R = 1u << DB;
     constexpr u32 NW = OS_TPB / 64;
     constexpr u32 DPT = R / OS_TPB;  // digits per thread
-    __shared__ u32 whist[NW][R];     // per-wave digit counts, then per-wave offsets
-    __shared__ u32 gofs[R];          // keys of smaller digits in the whole batch
-    __shared__ u32 tofs[R];          // keys of this digit in earlier tiles
+    extern __shared__ u32 os_lds[];
+    u32(*whist)[R] = (u32(*)[R])os_lds;  // per-wave digit counts, then per-wave offsets
+    u32 *gofs = os_lds + NW * R;          // keys of smaller digits in the whole batch
+    u32 *tofs = gofs + R;                 // keys of this digit in earlier tiles
     __shared__ u32 stmp[NW + 1];
     if (n_dev) n = *n_dev;
     const u32 t = blockIdx.x;
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(OS_TPB) void k_os_pass(const u64 *__restrict__ in, 
         u64 i = wbase + (u64)r * 64;
         key[r] = i < n ? in[i] : 0ull;
     }
-    for (u32 i = threadIdx.x; i < NW * R; i += OS_TPB) (&whist[0][0])[i] = 0;
+    for (u32 i = threadIdx.x; i < NW * R; i += OS_TPB) os_lds[i] = 0;
     __syncthreads();
 
     // rank inside the wave: lanes of equal digit found with DB ballots; the
@@ -530,8 +536,7 @@ hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStr
 // small tiles keep >= ~2000 workgroups in flight up to 8M keys
 static u32 rs_items(u64 n) { return n <= (1ull << 23) ? RS_ITEMS_SMALL : RS_ITEMS_BIG; }
 
-// one-kernel passes: tiles of 2048 keys up to 2M keys, 4096 above
-static u32 os_items(u64 n) { return n <= (1ull << 21) ? 8 : 16; }
+static u32 os_items(u64) { return OS_ITEMS; }
 
 // digit width for a sort over `width` bits: 10 unless 11 saves a pass
 static u32 os_digit_bits(u32 width) { return (width + 9) / 10 == (width + 10) / 11 ? 10 : 11; }
@@ -561,12 +566,9 @@ static void os_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u3
     for (u32 q = 0; q < npass; ++q) {
         const u32 shift = lo_bit + q * DB;
         u32 *nxt = q + 1 < npass ? look[(q + 1) & 1] : nullptr;
-        if (items == 8)
-            k_os_pass<DB, 8, (DB == 10 ? 16 : 8)><<<ntiles, OS_TPB, 0, st>>>(src, dst, n, n_dev, shift, gh + q * R,
-                                                                          look[q & 1], nxt, flags);
-        else
-            k_os_pass<DB, 16, (DB == 10 ? 16 : 8)><<<ntiles, OS_TPB, 0, st>>>(src, dst, n, n_dev, shift, gh + q * R,
-                                                                           look[q & 1], nxt, flags);
+        const size_t lds = ((size_t)(OS_TPB / 64) * R + 2 * R) * sizeof(u32);
+        k_os_pass<DB, OS_ITEMS, OS_LOOK / (R / OS_TPB)><<<ntiles, OS_TPB, lds, st>>>(src, dst, n, n_dev, shift,
+                                                                                   gh + q * R, look[q & 1], nxt, flags);
         u64 *t = src; src = dst; dst = t;
     }
     *result = src;

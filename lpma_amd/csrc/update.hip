@@ -261,7 +261,10 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             g.pstat[pid].pd1 = pd[j + 1];
         }
         if (j == 0 || prev != rk) {
-            const Density &dn = ONE ? g.one_dn : g.tab[pid].dn;
+            // density bounds from the device table (a lone PMA's entry at a
+            // fixed address): they stay out of the kernel arguments, which
+            // every launch of the pipeline copies
+            const Density &dn = g.tab[ONE ? 0 : pid].dn;
             const DevState d = geom<ONE>(g, pid);
             const u64 pb = (u64)pid << 32;
             const u32 s = (u32)rk;
