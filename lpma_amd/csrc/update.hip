@@ -731,9 +731,15 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         const u32 k = wk.u1 - wk.u0;
         const u64 base = (u64)wk.a << g.log2S;
         gu64 *keys = (gu64 *)d.keys + base;
+        // only the lines that hold live keys are read; the slots past C read
+        // as the largest key, which never sorts before an update
         u64 v[SPL];
 #pragma unroll
-        for (int r = 0; r < SPL; ++r) v[r] = keys[r * G + lane_g];  // all S slots exist
+        for (int r = 0; r < SPL; ++r) {
+            u64 x = KEY_NONE;
+            if (r * G + lane_g < C) x = keys[r * G + lane_g];
+            v[r] = x;
+        }
         const i32 pbase = pd[wk.u0];
         const u64 uk = lane_g < k ? ckey[wk.u0 + lane_g] : 0ull;
         const i32 up = lane_g <= k ? pd[wk.u0 + lane_g] : 0;
@@ -745,9 +751,6 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-        // slots past C read as the largest key: never before an update
-#pragma unroll
-        for (int r = 0; r < SPL; ++r) v[r] = r * G + lane_g < C ? v[r] : KEY_NONE;
         i32 shift[SPL];
         u32 gone = 0;  // bit r: old key r is deleted
 #pragma unroll
