@@ -195,12 +195,13 @@ hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchB
 
 // ------------------------------------------------------------ density check
 
-// first index >= start with a[idx] >= target, given a[start-1] < target
-__device__ __forceinline__ u32 gallop_fwd(const u64 *__restrict__ a, u32 start, u32 n, u64 target) {
+// first index >= start with a[idx] >= target, given a[start-1] < target;
+// the exponential search starts with strides of `step` (a guess of the
+// distance)
+__device__ __forceinline__ u32 gallop_fwd(const u64 *__restrict__ a, u32 start, u32 n, u64 target, u32 step) {
     u32 lo = start;
     if (lo >= n || a[lo] >= target) return lo;
-    u32 step = 1;
-    u32 hi = lo + 1;
+    u32 hi = (n - lo > step) ? lo + step : n;
     while (hi < n && a[hi] < target) {
         lo = hi;
         step <<= 1;
@@ -215,11 +216,11 @@ __device__ __forceinline__ u32 gallop_fwd(const u64 *__restrict__ a, u32 start, 
     return l;
 }
 
-// smallest index <= start whose suffix [idx, start] is >= target, given a[start] >= target
-__device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start, u64 target) {
+// smallest index <= start whose suffix [idx, start] is >= target, given
+// a[start] >= target; strides start at `step` as above
+__device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start, u64 target, u32 step) {
     if (start == 0 || a[start - 1] < target) return start;
     u32 hi = start - 1;  // a[hi] >= target
-    u32 step = 1;
     u32 lo;
     while (true) {
         lo = hi >= step ? hi - step : 0;
@@ -274,8 +275,12 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             for (; l <= d.H; ++l) {
                 u64 w = (u64)s >> l;
                 u64 a = w << l;
-                u0 = gallop_back(cseg, u0, pb + a);
-                u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l));
+                // the window grows by its sibling, which holds about as many
+                // runs as the window had: stride guess for the one boundary
+                // that moves
+                const u32 guess = max(1u, u1 - u0);
+                u0 = gallop_back(cseg, u0, pb + a, guess);
+                u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l), guess);
                 C = *count_ptr(d, l, w);
                 i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
                 if (window_fits(dn, l, d.H, (u64)m)) break;
@@ -376,8 +381,8 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
         if (leader) {
             const u64 pb = (u64)pid << 32;
             u64 a = w << L;
-            u32 u0 = gallop_back(cseg, j, pb + a);
-            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, pb + a + (1ull << L));
+            u32 u0 = gallop_back(cseg, j, pb + a, 1);
+            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, pb + a + (1ull << L), 1);
             u32 C = *count_ptr(d, L, w);
             wk.pid = pid;
             wk.a = (u32)a;
