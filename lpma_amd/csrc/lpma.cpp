@@ -709,9 +709,10 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
             const u64 M = Mnew[p];
             u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
             u64 nseg = 1ull << h->st.H;
-            CHECKED(ensure(pl, pl.gpre, nseg + 1));
+            CHECKED(ensure(pl, pl.gpre, 2 * nseg + 2));
             CHECKED(ensure(pl, pl.scan_scr, scan_scratch_words(nseg) + 8));
-            HIPCHECK(rebuild_push(h->st, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.scan_scr.p, st));
+            HIPCHECK(rebuild_push(h->st, p, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.gpre.p + nseg + 1,
+                                  pl.scan_scr.p, st));
             if (newH != h->st.H) {
                 CHECKED(respread_into_new(h, newH, pl.merged.p, M, st));
             } else {

@@ -897,8 +897,16 @@ hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64
 // slot reads the key the even spread assigns to it.  Source and destination
 // are the same window (in place) or the whole old and new arrays (resize).
 
+// first update of every segment's run (or of the next run): one search per
+// segment, so that k_push_old searches each key's segment run only
+__global__ void k_run_starts(const u64 *__restrict__ cseg, u32 u0, u32 u1, u64 pb, u64 nseg, u32 *__restrict__ ubeg) {
+    for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x <= nseg; x += (u64)gridDim.x * blockDim.x)
+        ubeg[x] = x == nseg ? u1 : (u32)lower_bound_dev<u64>(cseg, u0, u1, pb | x);
+}
+
 __global__ void k_push_old(DevState s, u64 a, u32 L, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
-                           const i32 *__restrict__ pd, u32 u0, u32 u1, u64 *__restrict__ merged, u64 m) {
+                           const i32 *__restrict__ pd, u32 u0, const u32 *__restrict__ ubeg,
+                           u64 *__restrict__ merged, u64 m) {
     const u64 W = (1ull << L) << s.log2S;
     const u64 base = a << s.log2S;
     const u32 Smask = (1u << s.log2S) - 1;
@@ -909,8 +917,11 @@ __global__ void k_push_old(DevState s, u64 a, u32 L, const u32 *__restrict__ gpr
         if (lane >= s.cnt[a + seg]) continue;
         u64 x = s.keys[base + t];
         u64 r = (u64)gpre[seg] + lane;
-        u64 u = lower_bound_dev<u64>(ckey, u0, u1, x);
-        if (u < u1 && ckey[u] == x) continue;
+        // the updates before x: all runs of earlier segments, then those of
+        // its own segment's run below x
+        const u32 r1 = ubeg[seg + 1];
+        u64 u = lower_bound_dev<u64>(ckey, ubeg[seg], r1, x);
+        if (u < r1 && ckey[u] == x) continue;
         u64 o = r + (i64)(pd[u] - pbase);
         if (o < m) merged[o] = x;
     }
@@ -1181,12 +1192,13 @@ hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64
 
 // full push of all live keys of one PMA and its effective updates, the
 // compacted range [u0, u1) (root rebalance / resize)
-hipError_t rebuild_push(const DevState &src, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m, u32 *gpre,
-                        u32 *scan_scratch, hipStream_t st) {
+hipError_t rebuild_push(const DevState &src, u32 pid, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m,
+                        u32 *gpre, u32 *ubeg, u32 *scan_scratch, hipStream_t st) {
     u64 nseg = nseg_of(src);
     u64 W = nseg << src.log2S;
     (void)device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
-    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, u0, u1, merged, m);
+    k_run_starts<<<grid_for(nseg + 1, 256, 16384), 256, 0, st>>>(b.cseg, u0, u1, (u64)pid << 32, nseg, ubeg);
+    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, u0, ubeg, merged, m);
     if (u1 > u0)
         k_push_ins<<<grid_for(u1 - u0, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, u0, u1, merged,
                                                                   m);
