@@ -442,6 +442,7 @@ struct Geometry {
     DBuf<u32> cnt, pyr;
 };
 
+// arrays for height H; the caller spreads keys into every slot and segment
 int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
     u64 nseg = 1ull << H;
     u64 cap = nseg << h->log2S;
@@ -451,9 +452,10 @@ int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
         release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
         return r;
     }
-    if (fill_u64(g.keys.p, cap, KEY_NONE, st) != hipSuccess || fill_u32(g.cnt.p, nseg, 0, st) != hipSuccess ||
-        fill_u32(g.pyr.p, nseg, 0, st) != hipSuccess || fill_u64(g.sep.p, nseg, 0, st) != hipSuccess ||
-        fill_u64(g.owner.p, 2 * nseg, 0, st) != hipSuccess) {
+    // keys, counts, separators and the pyramid are written whole by the
+    // spread that follows every allocation (k_pull, k_meta, the pyramid
+    // rebuild); the window claims must start clear
+    if (fill_u64(g.owner.p, 2 * nseg, 0, st) != hipSuccess) {
         (void)hipGetLastError();
         release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
         return LPMA_ERR_DEVICE;

@@ -1094,8 +1094,10 @@ __global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__re
             u32 p0 = pre[seg];
             if (lane >= pre[seg + 1] - p0) continue;
             u64 x = d.keys[slot0 + t];
-            u64 u = lower_bound_dev<u64>(ckey, wk.u0, wk.u1, x);
-            if (u < wk.u1 && ckey[u] == x) continue;  // deleted
+            // updates outside the chunk's segments are all below or all
+            // above its keys: search only the chunk's
+            u64 u = lower_bound_dev<u64>(ckey, qr[0], qr[1], x);
+            if (u < qr[1] && ckey[u] == x) continue;  // deleted
             u64 o = obase + p0 + lane + (i64)(pd[u] - pbase);
             if (o < wk.M) merged[mb + o] = x;
         }
