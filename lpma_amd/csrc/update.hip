@@ -279,9 +279,9 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
                 // runs as the window had: stride guess for the one boundary
                 // that moves
                 const u32 guess = max(1u, u1 - u0);
+                C = *count_ptr(d, l, w);  // independent of the searches: in flight meanwhile
                 u0 = gallop_back(cseg, u0, pb + a, guess);
                 u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l), guess);
-                C = *count_ptr(d, l, w);
                 i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
                 if (window_fits(dn, l, d.H, (u64)m)) break;
             }
