@@ -571,11 +571,12 @@ def run_analytics(args):
         lv, depth, scanned = pma.bfs(0)
         torch.cuda.synchronize()
         bfs_s = time.perf_counter() - t0
-        # algorithmic bytes: every slot read once per iteration, per live
-        # edge the source's contribution and the destination's atomic add,
-        # per vertex the prep / finish vectors; BFS: the frontier rows'
-        # slots and one level probe per scanned edge
-        pr_bytes = iters * (8 * cap + 8 * E + 28 * V)
+        # algorithmic bytes of the binned push: per iteration the live keys
+        # and row markers (8 B each), per edge its (dst, contribution) pair
+        # written and read back (16 B), per vertex the degree, contribution,
+        # rank and partial-sum vectors (about 28 B); once per call the count
+        # pass over the live keys
+        pr_bytes = iters * (8 * (E + V) + 16 * E + 28 * V) + 8 * (E + V)
         reached = int((lv >= 0).sum().item())
         row = {"batch": b, "op": op, "update_ms": upd_ms, "edges": E, "capacity_slots": cap,
                "pagerank_ms": pr_s * 1e3, "pagerank_iterations": iters, "pagerank_last_l1_change": delta,

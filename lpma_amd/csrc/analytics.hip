@@ -165,6 +165,21 @@ __device__ __forceinline__ void atomic_add2(float *hi, float *lo, float x) {
     if (e != 0.f) atomicAdd(lo, e);
 }
 
+// sum over a workgroup of PR_TPB threads (every thread gets it); tmp holds
+// PR_TPB / 64 floats of LDS
+static constexpr u32 PR_TPB = 256;
+static constexpr u32 PR_GRID = 2048;  // vector passes: few enough blocks for one atomic each
+__device__ __forceinline__ float block_sum(float v, float *tmp) {
+    v = wave_sum(v);
+    __syncthreads();  // tmp may still be read from a previous call
+    if (lane_id() == 0) tmp[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float t = 0.f;
+#pragma unroll
+    for (u32 w = 0; w < PR_TPB / 64; ++w) t += tmp[w];
+    return t;
+}
+
 // red layout (floats): [0,1] dangling mass hi/lo, [2,3] sum hi/lo,
 // [4,5] L1 change hi/lo
 static constexpr u32 PR_RED = 8;
@@ -186,8 +201,8 @@ __global__ void k_outdeg(DevState d, const u32 *__restrict__ gpre, u32 *__restri
 
 // contributions pr/outdeg, the dangling mass, and cleared accumulators
 // (PR_COPIES hi copies, then PR_COPIES lo copies, V floats each)
-__global__ void k_pr_prep(const float *__restrict__ pr, const u32 *__restrict__ outdeg, u32 V, float *__restrict__ c,
-                          float *__restrict__ acc, float *__restrict__ red) {
+__global__ __launch_bounds__(PR_TPB) void k_pr_prep(const float *__restrict__ pr, const u32 *__restrict__ outdeg, u32 V, float *__restrict__ c,
+                          float *__restrict__ acc, u64 nclear, float *__restrict__ red) {
     float dang = 0.f;
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 u = (u64)blockIdx.x * blockDim.x + threadIdx.x; u < V; u += stride) {
@@ -196,9 +211,10 @@ __global__ void k_pr_prep(const float *__restrict__ pr, const u32 *__restrict__ 
         c[u] = od ? p / (float)od : 0.f;
         if (!od) dang += p;
     }
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < 2ull * PR_COPIES * V; i += stride) acc[i] = 0.f;
-    dang = wave_sum(dang);
-    if (lane_id() == 0 && dang != 0.f) atomic_add2(&red[0], &red[1], dang);
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nclear; i += stride) acc[i] = 0.f;
+    __shared__ float tmp[PR_TPB / 64];
+    dang = block_sum(dang, tmp);
+    if (threadIdx.x == 0 && dang != 0.f) atomic_add2(&red[0], &red[1], dang);
 }
 
 __global__ void k_pr_push(DevState d, const float *__restrict__ c, float *__restrict__ acc) {
@@ -215,7 +231,7 @@ __global__ void k_pr_push(DevState d, const float *__restrict__ c, float *__rest
 }
 
 // pr' = alpha*acc + (alpha*dangling + (1-alpha)*sum(pr)) / n; L1 change and sum
-__global__ void k_pr_finish(const float *__restrict__ acc, const float *__restrict__ pr, float *__restrict__ out,
+__global__ __launch_bounds__(PR_TPB) void k_pr_finish(const float *__restrict__ acc, const float *__restrict__ pr, float *__restrict__ out,
                             u32 V, float alpha, const float *__restrict__ red_in, float *__restrict__ red_out) {
     const float dang = red_in[0] + red_in[1];
     const float sum_in = red_in[2] + red_in[3];
@@ -236,9 +252,10 @@ __global__ void k_pr_finish(const float *__restrict__ acc, const float *__restri
         dl += fabsf(p - pr[v]);
         sum += p;
     }
-    dl = wave_sum(dl);
-    sum = wave_sum(sum);
-    if (lane_id() == 0) {
+    __shared__ float tmp[PR_TPB / 64];
+    dl = block_sum(dl, tmp);
+    sum = block_sum(sum, tmp);
+    if (threadIdx.x == 0) {
         atomic_add2(&red_out[4], &red_out[5], dl);
         atomic_add2(&red_out[2], &red_out[3], sum);
     }
@@ -279,14 +296,316 @@ hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, flo
                       float alpha, float *red, float *red_next, hipStream_t st) {
     const u32 V = d.V;
     k_pr_reset<<<1, 64, 0, st>>>(red_next);
-    k_pr_prep<<<grid_for(pr_acc_floats(V), 256, 16384), 256, 0, st>>>(pr, outdeg, V, c, acc, red);
+    k_pr_prep<<<grid_for(pr_acc_floats(V), PR_TPB, PR_GRID), PR_TPB, 0, st>>>(pr, outdeg, V, c, acc, pr_acc_floats(V),
+                                                                         red);
     k_pr_push<<<grid_for(nseg_of(d) << d.log2S, 256, 65536), 256, 0, st>>>(d, c, acc);
-    k_pr_finish<<<grid_for(V, 256, 16384), 256, 0, st>>>(acc, pr, out, V, alpha, red, red_next);
+    k_pr_finish<<<grid_for(V, PR_TPB, PR_GRID), PR_TPB, 0, st>>>(acc, pr, out, V, alpha, red, red_next);
     return hipGetLastError();
 }
 
 hipError_t pr_scale(float *pr, u32 V, const float *red, hipStream_t st) {
     k_pr_scale<<<grid_for(V, 256, 16384), 256, 0, st>>>(pr, V, red);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- binned push (PageRank)
+//
+// The push above sends every edge's contribution to a random destination in
+// HBM: one returning atomic per edge on a 64 MB vector, which no L2 holds.
+// Propagation blocking (Beamer, Asanovic, Patterson, "Reducing PageRank
+// communication via propagation blocking", IPDPS 2017) splits it in two
+// streaming passes over the same gapped layout:
+//   bin pass   the slot array is cut into parts of 2^part_log slots; each
+//              part's workgroup writes (dst mod PB_BIN, contribution) pairs
+//              into per-destination-bin runs, at offsets fixed once per call
+//              by a count pass (bin-major exclusive scan of count[bin][part]);
+//   acc pass   each chunk of a bin's pairs is summed in LDS (PB_BIN vertices,
+//              compensated like the atomics above) and written as one
+//              partial vector; the finish pass adds a bin's partials (Kahan).
+// Every access to HBM is then a stream; the random adds stay in LDS.  The
+// graph does not change during a call, so the count pass runs once.
+
+static constexpr u32 PB_BIN_LOG = 13;  // destination vertices per bin
+static constexpr u32 PB_BIN = 1u << PB_BIN_LOG;
+static constexpr u32 PB_MAX_BINS = 16384;  // LDS cursors of the bin pass: 64 KB
+static constexpr u32 PB_TPB = 1024;
+static constexpr u32 PB_ITEMS = 8;
+static constexpr u32 PB_MAX_PARTS = 1024;
+
+// Lanes holding a live edge of the same bin form runs (adjacent lanes; keys
+// are sorted, so a row's edges to one bin are adjacent).  Returns the lane
+// that leads this lane's run, and for a leader the run's length in `len`.
+__device__ __forceinline__ u32 bin_run(bool valid, u32 bin, u32 &len) {
+    const u32 lane = lane_id();
+    const u32 bb = valid ? bin : 0xFFFFFFFFu;
+    const u32 prev = __shfl_up(bb, 1, 64);
+    const u32 next = __shfl_down(bb, 1, 64);
+    const u64 starts = __ballot(valid && (lane == 0 || prev != bb));
+    const u64 ends = __ballot(valid && (lane == 63 || next != bb));
+    // the sentinel bits keep both masks non-zero; a valid lane's own run
+    // bits are the highest start at or below it and the lowest end at or above
+    const u32 lead = 63 - (u32)__clzll((starts & ((2ull << lane) - 1ull)) | 1ull);
+    const u32 end = (u32)__builtin_ctzll((ends & ~((1ull << lane) - 1ull)) | (1ull << 63));
+    len = end - lane + 1;
+    return lead;
+}
+
+// PB_ITEMS slots t0 + i * PB_TPB + threadIdx.x of a part ending at p1: the
+// live ones' keys, KEY_NONE elsewhere.  All segment counts are loaded first
+// (clamped, unconditional), then the keys with exec-masked global loads, so a
+// thread has PB_ITEMS loads in flight per phase and dead lines are not read.
+__device__ __forceinline__ void pb_load(const DevState &d, u64 t0, u64 p1, u64 (&key)[PB_ITEMS]) {
+    const u32 Smask = (1u << d.log2S) - 1;
+    const gu64 *keys = (const gu64 *)d.keys;
+    u32 live[PB_ITEMS];
+#pragma unroll
+    for (u32 i = 0; i < PB_ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * PB_TPB + threadIdx.x;
+        live[i] = d.cnt[(t < p1 ? t : p1 - 1) >> d.log2S];
+    }
+#pragma unroll
+    for (u32 i = 0; i < PB_ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * PB_TPB + threadIdx.x;
+        u64 k = KEY_NONE;
+        if (t < p1 && (u32)(t & Smask) < live[i]) k = keys[t];
+        key[i] = k;
+    }
+}
+
+__device__ __forceinline__ bool pb_edge(u64 key) { return key != KEY_NONE && !is_marker(key); }
+
+// count[bin * P + part] = live edges of `part` into `bin`
+__global__ __launch_bounds__(PB_TPB) void k_pb_count(DevState d, u32 part_log, u32 P, u32 NB, u32 *__restrict__ count) {
+    extern __shared__ u32 pb_lds[];
+    for (u32 b = threadIdx.x; b < NB; b += PB_TPB) pb_lds[b] = 0;
+    __syncthreads();
+    const u64 W = nseg_of(d) << d.log2S;
+    const u64 p0 = (u64)blockIdx.x << part_log;
+    const u64 p1 = min(W, p0 + (1ull << part_log));
+    for (u64 t0 = p0; t0 < p1; t0 += (u64)PB_TPB * PB_ITEMS) {
+        u64 key[PB_ITEMS];
+        pb_load(d, t0, p1, key);
+#pragma unroll
+        for (u32 i = 0; i < PB_ITEMS; ++i) {
+            const bool valid = pb_edge(key[i]);
+            const u32 bin = (u32)key[i] >> PB_BIN_LOG;
+            u32 len;
+            const u32 lead = bin_run(valid, bin, len);
+            if (valid && lead == lane_id()) atomicAdd(&pb_lds[bin], len);
+        }
+    }
+    __syncthreads();
+    for (u32 b = threadIdx.x; b < NB; b += PB_TPB) count[(u64)b * P + blockIdx.x] = pb_lds[b];
+}
+
+// pairs[off[bin * P + part] + rank] = (contribution bits << 32) | (dst mod PB_BIN)
+// Phases per step: all key loads, all contribution loads, all cursor adds,
+// then the stores, so each thread keeps PB_ITEMS loads in flight at a time.
+__global__ __launch_bounds__(PB_TPB) void k_pb_bin(DevState d, u32 part_log, u32 P, u32 NB, const u32 *__restrict__ off,
+                                                   const float *__restrict__ c, u64 *__restrict__ pairs, u64 npairs) {
+    extern __shared__ u32 pb_lds[];
+    for (u32 b = threadIdx.x; b < NB; b += PB_TPB) pb_lds[b] = off[(u64)b * P + blockIdx.x];
+    __syncthreads();
+    const u64 W = nseg_of(d) << d.log2S;
+    const u64 p0 = (u64)blockIdx.x << part_log;
+    const u64 p1 = min(W, p0 + (1ull << part_log));
+    const u32 lane = lane_id();
+    for (u64 t0 = p0; t0 < p1; t0 += (u64)PB_TPB * PB_ITEMS) {
+        u64 key[PB_ITEMS];
+        pb_load(d, t0, p1, key);
+        float x[PB_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < PB_ITEMS; ++i) x[i] = c[pb_edge(key[i]) ? (u32)(key[i] >> 32) : 0u];  // always in bounds
+        u32 lead[PB_ITEMS], base[PB_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < PB_ITEMS; ++i) {
+            const bool valid = pb_edge(key[i]);
+            u32 len;
+            lead[i] = bin_run(valid, (u32)key[i] >> PB_BIN_LOG, len);
+            base[i] = 0;
+            if (valid && lead[i] == lane) base[i] = atomicAdd(&pb_lds[(u32)key[i] >> PB_BIN_LOG], len);
+            if (!valid) lead[i] = lane;
+        }
+#pragma unroll
+        for (u32 i = 0; i < PB_ITEMS; ++i) {
+            const u32 b0 = __shfl(base[i], lead[i], 64);
+            const u64 pos = (u64)b0 + (lane - lead[i]);
+            if (pb_edge(key[i]) && pos < npairs)
+                pairs[pos] = ((u64)__float_as_uint(x[i]) << 32) | ((u32)key[i] & (PB_BIN - 1));
+        }
+    }
+}
+
+// chunks per bin: nch[b] = ceil(pairs of bin b / chunk)
+__global__ void k_pb_nchunks(const u32 *__restrict__ off, u32 P, u32 NB, u32 chunk, u32 *__restrict__ nch) {
+    for (u32 b = blockIdx.x * blockDim.x + threadIdx.x; b < NB; b += gridDim.x * blockDim.x) {
+        const u32 n = off[(u64)(b + 1) * P] - off[(u64)b * P];
+        nch[b] = (n + chunk - 1) / chunk;
+    }
+}
+
+// acc[v] += x, compensated: acc[v] packs (hi, lo) as two FP32 halves of one
+// 64-bit LDS word; the new hi is the FP32 sum hi + x and its exact rounding
+// error (TwoSum) goes to lo, both installed by one compare-and-swap.  FP32
+// add atomics on LDS run about 20x slower than integer ones here (0.33 vs
+// 6.5 lanes per clock per CU measured on MI355X), the CAS form at 1.4
+// (uniform addresses) to 0.25 (R-MAT-skewed) and keeps FP32 arithmetic.
+__device__ __forceinline__ u64 add2(u64 cur, float x) {
+    const float hi = __uint_as_float((u32)cur), lo = __uint_as_float((u32)(cur >> 32));
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float e = (hi - (s - bp)) + (x - bp);
+    return ((u64)__float_as_uint(lo + e) << 32) | __float_as_uint(s);
+}
+
+// retry from a failed compare: cur is the word's value the CAS returned
+__device__ __forceinline__ void lds_add2_retry(u64 *acc, u32 v, float x, u64 cur) {
+    while (true) {
+        const u64 prev = atomicCAS((unsigned long long *)&acc[v], (unsigned long long)cur,
+                                   (unsigned long long)add2(cur, x));
+        if (prev == cur) break;
+        cur = prev;
+    }
+}
+
+// chunk k of bin b (cb = exclusive scan of nch, cb[NB] = chunks) summed in
+// LDS into partial[k * PB_BIN + v] = (hi, lo)
+__global__ __launch_bounds__(PB_TPB) void k_pb_acc(const u64 *__restrict__ pairs, const u32 *__restrict__ off, u32 P,
+                                                   u32 NB, const u32 *__restrict__ cb, u32 chunk,
+                                                   float2 *__restrict__ partial) {
+    extern __shared__ u64 pb_acc[];  // PB_BIN words: (lo << 32) | hi
+    const u32 k = blockIdx.x;
+    if (k >= cb[NB]) return;  // the grid is an upper bound
+    u32 a = 0, z = NB;        // last b with cb[b] <= k
+    while (z - a > 1) {
+        const u32 m = (a + z) >> 1;
+        if (cb[m] <= k) a = m; else z = m;
+    }
+    const u32 b = a;
+    const u64 e1 = off[(u64)(b + 1) * P];
+    const u64 s0 = (u64)off[(u64)b * P] + (u64)(k - cb[b]) * chunk;
+    const u64 s1 = min(e1, s0 + chunk);
+    for (u32 v = threadIdx.x; v < PB_BIN; v += PB_TPB) pb_acc[v] = 0ull;
+    __syncthreads();
+    for (u64 i0 = s0; i0 < s1; i0 += (u64)PB_TPB * PB_ITEMS) {
+        u64 pr[PB_ITEMS];
+#pragma unroll
+        for (u32 r = 0; r < PB_ITEMS; ++r) {
+            const u64 i = i0 + (u64)r * PB_TPB + threadIdx.x;
+            pr[r] = pairs[i < s1 ? i : s1 - 1];  // s1 > s0: a chunk is never empty
+            if (i >= s1) pr[r] = 0ull;           // x = +0 adds nothing
+        }
+        // all reads, then all first compare-and-swaps, then the retries of
+        // the lanes that lost a race: two LDS round trips per step when
+        // nothing collides
+        u64 cur[PB_ITEMS];
+#pragma unroll
+        for (u32 r = 0; r < PB_ITEMS; ++r) cur[r] = pb_acc[(u32)pr[r] & (PB_BIN - 1)];
+        u64 got[PB_ITEMS];
+#pragma unroll
+        for (u32 r = 0; r < PB_ITEMS; ++r) {
+            const float x = __uint_as_float((u32)(pr[r] >> 32));
+            got[r] = cur[r];
+            if (x != 0.f)
+                got[r] = atomicCAS((unsigned long long *)&pb_acc[(u32)pr[r] & (PB_BIN - 1)],
+                                   (unsigned long long)cur[r], (unsigned long long)add2(cur[r], x));
+        }
+#pragma unroll
+        for (u32 r = 0; r < PB_ITEMS; ++r)
+            if (got[r] != cur[r])
+                lds_add2_retry(pb_acc, (u32)pr[r] & (PB_BIN - 1), __uint_as_float((u32)(pr[r] >> 32)), got[r]);
+    }
+    __syncthreads();
+    float2 *out = partial + (u64)k * PB_BIN;
+    for (u32 v = threadIdx.x; v < PB_BIN; v += PB_TPB) {
+        const u64 w = pb_acc[v];
+        out[v] = make_float2(__uint_as_float((u32)w), __uint_as_float((u32)(w >> 32)));
+    }
+}
+
+// as k_pr_finish, with vertex v's sum taken from its bin's chunk partials
+__global__ __launch_bounds__(PR_TPB) void k_pb_finish(const float2 *__restrict__ partial, const u32 *__restrict__ cb,
+                            const float *__restrict__ pr, float *__restrict__ out, u32 V, float alpha,
+                            const float *__restrict__ red_in, float *__restrict__ red_out) {
+    const float dang = red_in[0] + red_in[1];
+    const float sum_in = red_in[2] + red_in[3];
+    const float gamma = (alpha * dang + (1.f - alpha) * sum_in) / (float)V;
+    float dl = 0.f, sum = 0.f;
+    for (u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (u64)gridDim.x * blockDim.x) {
+        const u32 b = (u32)(v >> PB_BIN_LOG);
+        const u32 k0 = cb[b], k1 = cb[b + 1];
+        float a = 0.f, comp = 0.f;
+        for (u32 k = k0; k < k1; ++k) {
+            const float2 h = partial[(u64)k * PB_BIN + (v & (PB_BIN - 1))];
+            float y = h.y - comp;  // lo first: the small term
+            float t = a + y;
+            comp = (t - a) - y;
+            a = t;
+            y = h.x - comp;
+            t = a + y;
+            comp = (t - a) - y;
+            a = t;
+        }
+        float p = alpha * a + gamma;
+        out[v] = p;
+        dl += fabsf(p - pr[v]);
+        sum += p;
+    }
+    __shared__ float tmp[PR_TPB / 64];
+    dl = block_sum(dl, tmp);
+    sum = block_sum(sum, tmp);
+    if (threadIdx.x == 0) {
+        atomic_add2(&red_out[4], &red_out[5], dl);
+        atomic_add2(&red_out[2], &red_out[3], sum);
+    }
+}
+
+bool pr_bins_fit(u32 V) { return ((u64)V + PB_BIN - 1) / PB_BIN <= PB_MAX_BINS; }
+
+PrBins pr_bins_plan(const DevState &d) {
+    PrBins b{};
+    const u64 W = nseg_of(d) << d.log2S;
+    u32 lw = 0;
+    while ((1ull << lw) < W) ++lw;
+    b.part_log = std::max<u32>(d.log2S, lw > 10 ? lw - 10 : 0);  // at most PB_MAX_PARTS parts
+    b.parts = (u32)((W + (1ull << b.part_log) - 1) >> b.part_log);
+    b.bins = (u32)(((u64)d.V + PB_BIN - 1) / PB_BIN);
+    return b;
+}
+
+u64 pr_bins_count_words(const PrBins &b) { return (u64)b.bins * b.parts + 1; }
+
+hipError_t pr_bins_count(const DevState &d, const PrBins &b, u32 *off, u32 *scan_scratch, hipStream_t st) {
+    if (b.parts > PB_MAX_PARTS || b.bins > PB_MAX_BINS || b.bins == 0) return hipErrorInvalidValue;
+    k_pb_count<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off);
+    return device_excl_scan(off, (u64)b.bins * b.parts, off, scan_scratch, st);
+}
+
+void pr_bins_chunking(PrBins &b, u64 pairs) {
+    b.pairs = pairs;
+    b.chunk = (u32)std::max<u64>(4096, (pairs + 2047) / 2048);
+    b.max_chunks = (u32)((pairs + b.chunk - 1) / b.chunk) + b.bins;
+}
+
+u64 pr_bins_partial_floats(const PrBins &b) { return 2ull * PB_BIN * b.max_chunks; }
+
+hipError_t pr_bins_chunks(const PrBins &b, const u32 *off, u32 *cb, u32 *scan_scratch, hipStream_t st) {
+    k_pb_nchunks<<<grid_for(b.bins, 256, 1024), 256, 0, st>>>(off, b.parts, b.bins, b.chunk, cb);
+    return device_excl_scan(cb, b.bins, cb, scan_scratch, st);
+}
+
+hipError_t pr_iterate_binned(const DevState &d, const PrBins &b, const float *pr, const u32 *outdeg, float *c,
+                             const u32 *off, const u32 *cb, u64 *pairs, float *partial, float *out, float alpha,
+                             float *red, float *red_next, hipStream_t st) {
+    const u32 V = d.V;
+    k_pr_reset<<<1, 64, 0, st>>>(red_next);
+    k_pr_prep<<<grid_for(V, PR_TPB, PR_GRID), PR_TPB, 0, st>>>(pr, outdeg, V, c, nullptr, 0, red);
+    k_pb_bin<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off, c, pairs, b.pairs);
+    if (b.max_chunks)
+        k_pb_acc<<<b.max_chunks, PB_TPB, PB_BIN * sizeof(u64), st>>>(pairs, off, b.parts, b.bins, cb, b.chunk,
+                                                                          (float2 *)partial);
+    k_pb_finish<<<grid_for(V, PR_TPB, PR_GRID), PR_TPB, 0, st>>>((const float2 *)partial, cb, pr, out, V, alpha, red,
+                                                         red_next);
     return hipGetLastError();
 }
 

@@ -89,5 +89,24 @@ u32 pr_red_floats();
 hipError_t pr_iterate(const DevState &d, const float *pr, const u32 *outdeg, float *c, float *acc, float *out,
                       float alpha, float *red, float *red_next, hipStream_t st);
 hipError_t pr_scale(float *pr, u32 V, const float *red, hipStream_t st);
+// binned push: geometry of one PageRank call
+struct PrBins {
+    u32 part_log;    // slots per part (log2)
+    u32 parts;       // parts of the slot array (<= 1024)
+    u32 bins;        // destination bins of 8192 vertices
+    u32 chunk;       // pairs per accumulation chunk
+    u32 max_chunks;  // bound on the chunks of all bins (the acc grid)
+    u64 pairs;       // live edges
+};
+bool pr_bins_fit(u32 V);
+PrBins pr_bins_plan(const DevState &d);
+u64 pr_bins_count_words(const PrBins &b);
+hipError_t pr_bins_count(const DevState &d, const PrBins &b, u32 *off, u32 *scan_scratch, hipStream_t st);
+void pr_bins_chunking(PrBins &b, u64 pairs);
+u64 pr_bins_partial_floats(const PrBins &b);
+hipError_t pr_bins_chunks(const PrBins &b, const u32 *off, u32 *cb, u32 *scan_scratch, hipStream_t st);
+hipError_t pr_iterate_binned(const DevState &d, const PrBins &b, const float *pr, const u32 *outdeg, float *c,
+                             const u32 *off, const u32 *cb, u64 *pairs, float *partial, float *out, float alpha,
+                             float *red, float *red_next, hipStream_t st);
 
 }  // namespace pma

@@ -217,8 +217,9 @@ struct lpma {
 
     Pipeline pl;  // plain updates, views and analytics scratch
     // analytics scratch
-    DBuf<u32> aq0;
-    DBuf<float> af0, af1, af2, ared;
+    DBuf<u32> aq0, apoff, apcb;
+    DBuf<float> af0, af1, af2, ared, apart;
+    DBuf<u64> apairs;
     u32 *d_err = nullptr;
 
     // host counters
@@ -1263,8 +1264,9 @@ int lpma_destroy(lpma_t *h) {
     pipeline_free(h->pl);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
-    release(h, h->aq0);
-    release(h, h->af0); release(h, h->af1); release(h, h->af2); release(h, h->ared);
+    release(h, h->aq0); release(h, h->apoff); release(h, h->apcb);
+    release(h, h->af0); release(h, h->af1); release(h, h->af2); release(h, h->ared); release(h, h->apart);
+    release(h, h->apairs);
     if (h->d_err) (void)hipFree(h->d_err);
     delete h;
     return LPMA_OK;
@@ -1586,10 +1588,30 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
     CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8));
     CHECKED(ensure(h, h->aq0, V));   // out-degrees
     CHECKED(ensure(h, h->af0, V));   // contributions
-    CHECKED(ensure(h, h->af1, pr_acc_floats(h->V)));  // compensated accumulators
     CHECKED(ensure(h, h->af2, V));   // second rank vector
     CHECKED(ensure(h, h->ared, 2 * pr_red_floats()));
     HIPCHECK(pr_outdeg(h->st, h->pl.gpre.p, h->pl.scan_scr.p, h->aq0.p, st));
+    // Binned push where the destination bins fit the bin pass's LDS, else
+    // the atomic push into per-XCD accumulators.
+    const bool binned = pr_bins_fit(h->st.V);
+    PrBins bins{};
+    if (binned) {
+        bins = pr_bins_plan(h->st);
+        const u64 nc = pr_bins_count_words(bins);
+        CHECKED(ensure(h, h->apoff, nc));
+        CHECKED(ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nc) + 8));
+        HIPCHECK(pr_bins_count(h->st, bins, h->apoff.p, h->pl.scan_scr.p, st));
+        u32 total = 0;
+        HIPCHECK(hipMemcpyAsync(&total, h->apoff.p + (nc - 1), sizeof(u32), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        pr_bins_chunking(bins, total);
+        CHECKED(ensure(h, h->apcb, (u64)bins.bins + 1));
+        CHECKED(ensure(h, h->apairs, std::max<u64>(1, total)));
+        CHECKED(ensure(h, h->apart, pr_bins_partial_floats(bins)));
+        HIPCHECK(pr_bins_chunks(bins, h->apoff.p, h->apcb.p, h->pl.scan_scr.p, st));
+    } else {
+        CHECKED(ensure(h, h->af1, pr_acc_floats(h->V)));  // compensated accumulators
+    }
     float *red[2] = {h->ared.p, h->ared.p + pr_red_floats()};
     float *vec[2] = {d_rank, h->af2.p};
     HIPCHECK(pr_init(vec[0], h->V, red[0], st));
@@ -1597,8 +1619,12 @@ int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, 
     float delta = 0.f;
     int cur = 0;
     while (it < max_iter) {
-        HIPCHECK(pr_iterate(h->st, vec[cur], h->aq0.p, h->af0.p, h->af1.p, vec[cur ^ 1], damping, red[cur],
-                            red[cur ^ 1], st));
+        if (binned)
+            HIPCHECK(pr_iterate_binned(h->st, bins, vec[cur], h->aq0.p, h->af0.p, h->apoff.p, h->apcb.p, h->apairs.p,
+                                       h->apart.p, vec[cur ^ 1], damping, red[cur], red[cur ^ 1], st));
+        else
+            HIPCHECK(pr_iterate(h->st, vec[cur], h->aq0.p, h->af0.p, h->af1.p, vec[cur ^ 1], damping, red[cur],
+                                red[cur ^ 1], st));
         cur ^= 1;
         ++it;
         float r[8];

@@ -44,7 +44,8 @@ def test_bfs_levels_bit_exact(scale, E, deletes):
     pma.close()
 
 
-@pytest.mark.parametrize("scale,E,deletes", [(10, 6000, 0), (14, 150000, 20000), (16, 400000, 100000)])
+@pytest.mark.parametrize("scale,E,deletes", [(10, 6000, 0), (14, 150000, 20000), (16, 400000, 100000),
+                                             (18, 2000000, 300000)])
 def test_pagerank_fp32_within_1e6_of_fp64(scale, E, deletes):
     say(f"pagerank scale={scale}")
     V, pma, ref = build(scale, E, 7, deletes)
