@@ -19,12 +19,29 @@ __device__ __forceinline__ void slot_range(const DevState &d, u32 r, u64 &lo, u6
 // ------------------------------------------------------------------ BFS
 //
 // Level-synchronous, top-down.  A level expands the frontier's slot ranges
-// with one thread per slot (the range lengths are scanned, each thread finds
-// its vertex by binary search), so hub vertices are split over many threads.
+// with one thread per slot, so hub vertices are split over many threads:
+//   k_bfs_len   each frontier row's slot count and first slot;
+//   scan        the counts' exclusive prefix: slot t of the level belongs to
+//               the last entry whose offset is <= t;
+//   k_bfs_runs  the first entry of every run of EXP_RUN slots (one binary
+//               search per run, all in parallel);
+//   k_bfs_expand  a workgroup per run stages the run's entries (offsets and
+//               first slots) in LDS; each thread takes EXP_ITEMS slots and
+//               issues their loads phase by phase (segment counts, keys,
+//               visited words), so dead lines of the gapped rows are not read
+//               and a thread has EXP_ITEMS loads in flight per phase.
+// A visited bitmap (V bits: 2 MB at V = 2^24, held by every XCD's L2) filters
+// the edges; the levels array (32 bits per vertex) is only written.  A vertex
+// is claimed by the atomic OR that sets its bit, so it enters the next
+// frontier once; a plain read of a word another XCD has just changed can only
+// miss bits (they go from 0 to 1), which costs an extra atomic, never a wrong
+// level.
 
-__global__ void k_bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters) {
+__global__ void k_bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, u32 *visited) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (u64)gridDim.x * blockDim.x)
         levels[i] = i == src ? 0u : INF_LEVEL;
+    for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < bfs_visited_words(V); w += (u64)gridDim.x * blockDim.x)
+        visited[w] = w == (src >> 5) ? 1u << (src & 31) : 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         queue[0] = src;
         counters[0] = 0;  // next frontier size
@@ -32,102 +49,158 @@ __global__ void k_bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counter
     }
 }
 
-__global__ void k_bfs_len(DevState d, const u32 *__restrict__ q, u32 qn, u32 *__restrict__ len) {
+__global__ void k_bfs_len(DevState d, const u32 *__restrict__ q, u32 qn, u32 *__restrict__ len,
+                          u64 *__restrict__ first) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < qn; i += (u64)gridDim.x * blockDim.x) {
         u64 lo, hi;
         slot_range(d, q[i], lo, hi);
         len[i] = (u32)(hi - lo);
+        first[i] = lo;
     }
 }
 
-// Each workgroup expands a run of EXP_TPB consecutive slots of the scanned
-// frontier.  The frontier entries covering the run are found with one
-// binary search over the global offsets (by one lane) and staged in LDS;
-// every lane then finds its entry by a short search there instead of a
-// search over the whole frontier.
 static constexpr u32 EXP_TPB = 256;
-static constexpr u32 EXP_STAGE = 512;  // staged offsets per run (>= EXP_TPB + 1 covers every run of
-                                       // non-empty entries; empty rows fall back to the global search)
+static constexpr u32 EXP_ITEMS = 4;
+static constexpr u32 EXP_RUN = EXP_TPB * EXP_ITEMS;  // slots per run
+static constexpr u32 EXP_STAGE = EXP_RUN + 1;        // staged entries: every entry of a run of non-empty
+                                                     // rows; empty rows past it use the global search
 
-__global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, const u32 *__restrict__ q, u32 qn,
-                                                         const u32 *__restrict__ off, u32 *__restrict__ levels,
-                                                         u32 level, u32 *__restrict__ qnext, u32 *__restrict__ counters) {
-    __shared__ u32 s_off[EXP_STAGE];
-    __shared__ u32 s_e0;
+// last entry e in [lo, hi) with off[e] <= t (off[lo] <= t)
+__device__ __forceinline__ u32 entry_of(const u32 *__restrict__ off, u32 lo, u32 hi, u64 t) {
+    while (hi - lo > 1) {
+        const u32 m = (lo + hi) >> 1;
+        if ((u64)off[m] <= t) lo = m; else hi = m;
+    }
+    return lo;
+}
+
+// runs[r] = entry holding slot r * EXP_RUN
+__global__ void k_bfs_runs(const u32 *__restrict__ off, u32 qn, u32 *__restrict__ runs) {
     const u64 total = off[qn];
+    const u64 nruns = (total + EXP_RUN - 1) / EXP_RUN;
+    for (u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x; r < nruns; r += (u64)gridDim.x * blockDim.x)
+        runs[r] = entry_of(off, 0, qn, r * EXP_RUN);
+}
+
+// Moves the workgroup's discovered vertices from LDS to the next frontier:
+// one atomic on the frontier size per flush.
+__device__ __forceinline__ void bfs_flush(u32 *s_q, u32 &s_n, u32 &s_base, u32 *__restrict__ qnext,
+                                          u32 *__restrict__ counters) {
+    const u32 n = s_n;  // read by all after the barrier that precedes the call
+    if (threadIdx.x == 0) s_base = atomicAdd(&counters[0], n);
+    __syncthreads();
+    const u32 base = s_base;
+    for (u32 k = threadIdx.x; k < n; k += EXP_TPB) qnext[base + k] = s_q[k];
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, u32 qn, const u32 *__restrict__ off,
+                                                         const u64 *__restrict__ first,
+                                                         const u32 *__restrict__ runs, u32 *__restrict__ levels,
+                                                         u32 *__restrict__ visited, u32 level,
+                                                         u32 *__restrict__ qnext, u32 *__restrict__ counters) {
+    __shared__ u32 s_off[EXP_STAGE];
+    __shared__ u64 s_first[EXP_STAGE];
+    __shared__ u32 s_q[2 * EXP_RUN];  // discovered vertices; flushed once it holds EXP_RUN
+    __shared__ u32 s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    const u64 total = off[qn];
+    const u64 nruns = (total + EXP_RUN - 1) / EXP_RUN;
     const u32 lane = lane_id();
+    const u32 Smask = (1u << d.log2S) - 1;
+    const gu64 *keys = (const gu64 *)d.keys;
     u32 scanned = 0;
-    for (u64 t0 = (u64)blockIdx.x * EXP_TPB; t0 < total; t0 += (u64)gridDim.x * EXP_TPB) {
-        if (threadIdx.x == 0) {
-            // last entry with off <= t0
-            u32 lo = 0, hi = qn;
-            while (hi - lo > 1) {
-                u32 m = (lo + hi) >> 1;
-                if (off[m] <= t0) lo = m; else hi = m;
-            }
-            s_e0 = lo;
-        }
-        __syncthreads();
-        const u32 e0 = s_e0;
+    for (u64 run = blockIdx.x; run < nruns; run += gridDim.x) {
+        const u64 t0 = run * EXP_RUN;
+        const u32 e0 = runs[run];
         for (u32 k = threadIdx.x; k < EXP_STAGE; k += EXP_TPB) {
-            u64 idx = (u64)e0 + k;
+            const u64 idx = (u64)e0 + k;
             s_off[k] = idx <= qn ? off[idx] : 0xFFFFFFFFu;
+            s_first[k] = idx < qn ? first[idx] : 0ull;
         }
         __syncthreads();
-        const u64 t = t0 + threadIdx.x;
-        u32 v = INF_LEVEL;
-        bool found = false;
-        if (t < total) {
-            u32 ent;
-            if ((u64)s_off[EXP_STAGE - 1] > t) {
-                // last k with s_off[k] <= t (s_off[0] <= t0 <= t)
-                u32 lo = 0, hi = EXP_STAGE - 1;
+        // slot of each item: the staged entry holding it, or the global search
+        // past the stage (only when empty rows fill it)
+        u64 slot[EXP_ITEMS];
+        bool ok[EXP_ITEMS];
+        const bool staged_all = (u64)s_off[EXP_STAGE - 1] > t0 + EXP_RUN - 1;
+#pragma unroll
+        for (u32 i = 0; i < EXP_ITEMS; ++i) {
+            const u64 t = t0 + (u64)i * EXP_TPB + threadIdx.x;
+            ok[i] = t < total;
+            slot[i] = 0;
+            if (!ok[i]) continue;
+            if (staged_all || (u64)s_off[EXP_STAGE - 1] > t) {
+                u32 lo = 0, hi = EXP_STAGE - 1;  // s_off[0] <= t0 <= t
                 while (hi - lo > 1) {
-                    u32 m = (lo + hi) >> 1;
+                    const u32 m = (lo + hi) >> 1;
                     if ((u64)s_off[m] <= t) lo = m; else hi = m;
                 }
-                ent = e0 + lo;
+                slot[i] = s_first[lo] + (t - s_off[lo]);
             } else {
-                u32 lo = e0 + EXP_STAGE - 1, hi = qn;
-                while (hi - lo > 1) {
-                    u32 m = (lo + hi) >> 1;
-                    if (off[m] <= t) lo = m; else hi = m;
-                }
-                ent = lo;
+                const u32 e = entry_of(off, e0 + EXP_STAGE - 1, qn, t);
+                slot[i] = first[e] + (t - off[e]);
             }
-            u64 slo, shi;
-            slot_range(d, q[ent], slo, shi);
-            u64 key = d.keys[slo + (t - off[ent])];
-            if (key != KEY_NONE) {
+        }
+        u32 live[EXP_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < EXP_ITEMS; ++i) live[i] = d.cnt[slot[i] >> d.log2S];  // slot 0 when !ok: in bounds
+        u64 key[EXP_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < EXP_ITEMS; ++i) {
+            u64 k = KEY_NONE;
+            if (ok[i] && (u32)(slot[i] & Smask) < live[i]) k = keys[slot[i]];
+            key[i] = k;
+        }
+        u32 vw[EXP_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < EXP_ITEMS; ++i) vw[i] = key[i] != KEY_NONE ? visited[(u32)key[i] >> 5] : ~0u;
+#pragma unroll
+        for (u32 i = 0; i < EXP_ITEMS; ++i) {
+            bool found = false;
+            if (key[i] != KEY_NONE) {
                 ++scanned;
-                v = (u32)key;
-                if (levels[v] == INF_LEVEL && atomicCAS(&levels[v], INF_LEVEL, level + 1) == INF_LEVEL) found = true;
+                const u32 v = (u32)key[i], bit = 1u << (v & 31);
+                if (!(vw[i] & bit) && !(atomicOr(&visited[v >> 5], bit) & bit)) {
+                    found = true;
+                    levels[v] = level + 1;
+                }
+            }
+            const u64 m = __ballot(found);
+            if (m) {
+                const u32 lead = (u32)__builtin_ctzll(m);
+                u32 base = 0;
+                if (lane == lead) base = atomicAdd(&s_n, (u32)__popcll(m));  // LDS
+                base = __shfl(base, lead, 64);
+                if (found) s_q[base + __popcll(m & ((1ull << lane) - 1ull))] = (u32)key[i];
             }
         }
-        u64 m = __ballot(found);
-        if (m) {
-            u32 first = (u32)__builtin_ctzll(m);
-            u32 base = 0;
-            if (lane == first) base = atomicAdd(&counters[0], (u32)__popcll(m));
-            base = __shfl(base, first, 64);
-            if (found) qnext[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
-        }
-        __syncthreads();  // s_off / s_e0 are rewritten by the next run
+        __syncthreads();  // the stage is rewritten by the next run; s_n is final for this run
+        if (s_n >= EXP_RUN) bfs_flush(s_q, s_n, s_base, qnext, counters);  // a run adds at most EXP_RUN
     }
+    __syncthreads();
+    if (s_n) bfs_flush(s_q, s_n, s_base, qnext, counters);
     scanned = wave_sum(scanned);
     if (lane == 0 && scanned) atomicAdd(&counters[1], scanned);
 }
 
-hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipStream_t st) {
-    k_bfs_init<<<grid_for(V, 256, 16384), 256, 0, st>>>(levels, V, src, queue, counters);
+hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, u32 *visited, hipStream_t st) {
+    k_bfs_init<<<grid_for(V, 256, 16384), 256, 0, st>>>(levels, V, src, queue, counters, visited);
     return hipGetLastError();
 }
 
-hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
-                     u32 *qnext, u32 *counters, u32 expand_grid, hipStream_t st) {
-    k_bfs_len<<<grid_for(qn, 256, 16384), 256, 0, st>>>(d, q, qn, len);
+u64 bfs_run_words(u64 slots) { return slots / EXP_RUN + 2; }
+
+hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u64 *first, u32 *runs, u32 *scan_scratch,
+                     u32 *levels, u32 *visited, u32 level, u32 *qnext, u32 *counters, u32 expand_grid,
+                     hipStream_t st) {
+    k_bfs_len<<<grid_for(qn, 256, 16384), 256, 0, st>>>(d, q, qn, len, first);
     (void)device_excl_scan(len, qn, len, scan_scratch, st);
-    k_bfs_expand<<<expand_grid ? expand_grid : 2048, EXP_TPB, 0, st>>>(d, q, qn, len, levels, level, qnext, counters);
+    k_bfs_runs<<<grid_for(bfs_run_words(nseg_of(d) << d.log2S), 256, 16384), 256, 0, st>>>(len, qn, runs);
+    k_bfs_expand<<<expand_grid ? expand_grid : 2048, EXP_TPB, 0, st>>>(d, qn, len, first, runs, levels, visited,
+                                                                      level, qnext, counters);
     return hipGetLastError();
 }
 

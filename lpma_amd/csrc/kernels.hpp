@@ -79,9 +79,12 @@ hipError_t neighbor_gather(const DevState &d, const u32 *srcs, u64 n, const u32 
 hipError_t check_invariants(const DevState &d, u32 *err, hipStream_t st);
 
 // analytics.hip
-hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, hipStream_t st);
-hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u32 *scan_scratch, u32 *levels, u32 level,
-                     u32 *qnext, u32 *counters, u32 expand_grid, hipStream_t st);
+__host__ __device__ __forceinline__ u64 bfs_visited_words(u64 V) { return (V + 31) / 32; }
+hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, u32 *visited, hipStream_t st);
+u64 bfs_run_words(u64 slots);
+hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u64 *first, u32 *runs, u32 *scan_scratch,
+                     u32 *levels, u32 *visited, u32 level, u32 *qnext, u32 *counters, u32 expand_grid,
+                     hipStream_t st);
 hipError_t pr_outdeg(const DevState &d, u32 *gpre, u32 *scan_scratch, u32 *outdeg, hipStream_t st);
 hipError_t pr_init(float *pr, u32 V, float *red, hipStream_t st);
 u64 pr_acc_floats(u32 V);

@@ -1398,20 +1398,23 @@ int lpma_check(lpma_t *h, uint64_t *violations, void *stream) {
 static int bfs_core(int device, const DevState &d, u32 src, u32 *d_levels, u32 *depth, u64 *edges_scanned,
                     u32 expand_grid, hipStream_t st) {
     const u64 V = d.V;
-    DBuf<u32> q0, q1, len, cnt, scan;
+    DBuf<u32> q0, q1, len, cnt, scan, runs, visited;
+    DBuf<u64> first;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(st);
         release_on(device, q0); release_on(device, q1); release_on(device, len); release_on(device, cnt);
-        release_on(device, scan);
+        release_on(device, scan); release_on(device, runs); release_on(device, first); release_on(device, visited);
     };
     int r;
     if ((r = ensure_on(device, st, q0, V + 1)) || (r = ensure_on(device, st, q1, V + 1)) ||
         (r = ensure_on(device, st, len, V + 2)) || (r = ensure_on(device, st, scan, scan_scratch_words(V + 1) + 8)) ||
-        (r = ensure_on(device, st, cnt, 4))) {
+        (r = ensure_on(device, st, cnt, 4)) || (r = ensure_on(device, st, first, V + 1)) ||
+        (r = ensure_on(device, st, runs, bfs_run_words(nseg_of(d) << d.log2S))) ||
+        (r = ensure_on(device, st, visited, bfs_visited_words(V)))) {
         cleanup();
         return r;
     }
-    if (bfs_init(d_levels, d.V, src, q0.p, cnt.p, st) != hipSuccess) {
+    if (bfs_init(d_levels, d.V, src, q0.p, cnt.p, visited.p, st) != hipSuccess) {
         (void)hipGetLastError();
         cleanup();
         return LPMA_ERR_DEVICE;
@@ -1422,7 +1425,8 @@ static int bfs_core(int device, const DevState &d, u32 src, u32 *d_levels, u32 *
     u32 c[2];
     int rc = LPMA_OK;
     while (qn) {
-        if (bfs_level(d, q, qn, len.p, scan.p, d_levels, level, qn_buf, cnt.p, expand_grid, st) != hipSuccess ||
+        if (bfs_level(d, q, qn, len.p, first.p, runs.p, scan.p, d_levels, visited.p, level, qn_buf, cnt.p, expand_grid,
+                      st) != hipSuccess ||
             hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess || fill_u32(cnt.p, 2, 0, st) != hipSuccess) {
             (void)hipGetLastError();

@@ -173,7 +173,13 @@ int grow_pin(Pin<T> &b, u64 n, hipStream_t st) {
     return LPMA_OK;
 }
 
-u64 bfs_bytes(u32 V) { return 24ull * ((u64)V + 2) + (1ull << 20); }
+// device scratch of one search: per vertex two queues, the counts, the first
+// slots and the levels (32 B) and a visited bit; one run entry per 1024 slots
+// (a snapshot holds at least 8 B per slot, so snapshot bytes / 2048 covers
+// it); the scan
+u64 bfs_bytes(u32 V, u64 snap_bytes) {
+    return 32ull * ((u64)V + 2) + (u64)V / 8 + 64 + snap_bytes / 2048 + (1ull << 20);
+}
 
 void finish(lpma_server *s, Request *r, int status) {
     // caller holds s->mu
@@ -332,7 +338,7 @@ void destroy_unused_snaps(lpma_server *s) {
 
 // Admits queued searches while workers and budget allow; caller holds mu.
 void dispatch_bfs(lpma_server *s, std::unique_lock<std::mutex> &lk) {
-    const u64 bb = bfs_bytes(s->V);
+    const u64 bb = bfs_bytes(s->V, lpma_snapshot_bytes(s->h));
     while (!s->q_bfs.empty() && s->busy < (int)s->cfg.bfs_workers) {
         // a search reads the newest snapshot while it is younger than the
         // configured age (its result reports the version it saw); a newer
