@@ -350,12 +350,12 @@ def served_requests(rng, rate, duration, V, ins_pool, del_pool, mean_batch):
     return reqs
 
 
-def serve_trace(pma, reqs, workers):
+def serve_trace(pma, reqs, workers, search_workgroups):
     """The served form: requests submitted at their arrival times to the
     server (continuous batching, BFS on snapshots beside the writer).
     Returns [(kind, result, payload, arrival_s)] with result latency in us."""
     import lpma_amd as L
-    srv = L.Server(pma, bfs_workers=workers)
+    srv = L.Server(pma, bfs_workers=workers, search_workgroups=search_workgroups)
     tickets = []
     t0 = time.perf_counter()
     for (at, kind, *payload) in reqs:
@@ -447,7 +447,7 @@ def run_served(args):
         for rate in rates:
             reqs = served_requests(rng, rate, args.duration, V, ins_pool, del_pool, 1e4)
             if mode == "served":
-                results, st = serve_trace(pma, reqs, args.bfs_workers)
+                results, st = serve_trace(pma, reqs, args.bfs_workers, args.search_workgroups)
                 row = summarize_trace(results, st)
             else:
                 results = plain_trace(pma, reqs)
@@ -480,9 +480,12 @@ def run_served(args):
         "data": "synthetic (device R-MAT generator, Poisson arrivals)",
         "config": {"workload": "served_rmat24_poisson_mix_60upd_35read_5bfs", "vertices": V, "samples": samples,
                    "edges": int(E), "duration_s_per_rate": args.duration, "rates_req_per_s": rates,
-                   "bfs_workers": args.bfs_workers, "update_mean_batch": 1e4, "parallelism": "single GPU"},
+                   "bfs_workers": args.bfs_workers, "search_workgroups": args.search_workgroups,
+                   "update_mean_batch": 1e4, "parallelism": "single GPU"},
         "sweep": sweep,
         "speedup_vs_plain": top["sustained_updates_per_s"] / base["sustained_updates_per_s"],
+        "update_p99_ms": top.get("update_p99_ms"),
+        "plain_update_p99_ms": base.get("update_p99_ms"),
         "plain_form": "one loop serving each request alone in arrival order on the live structure",
         "reads_checked_against_sequential_replay": checked,
         "reads_mismatched": mismatched,
@@ -651,9 +654,11 @@ def main():
     ap.add_argument("--analytics-check-iters", type=int, default=5)
     ap.add_argument("--served-scale", type=int, default=24)
     ap.add_argument("--served-samples", type=int, default=1 << 28)
-    ap.add_argument("--rates", default="500,1000,2000,4000", help="served: offered requests/s to sweep")
+    ap.add_argument("--rates", default="500,1000,2000,4000,8000,16000", help="served: offered requests/s to sweep")
     ap.add_argument("--duration", type=float, default=1.5, help="served: seconds of arrivals per rate")
     ap.add_argument("--bfs-workers", type=int, default=2)
+    ap.add_argument("--search-workgroups", type=int, default=256,
+                    help="served: workgroups each concurrent search expands its frontier with")
     ap.add_argument("--verify-reads", type=int, default=64)
     ap.add_argument("--served-modes", default="served,plain")
     ap.add_argument("--graph", type=int, default=0,

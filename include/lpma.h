@@ -215,7 +215,8 @@ int lpma_snapshot_set_search_width(lpma_snapshot_t *s, uint32_t workgroups);
  * (include/pagerank.hxx:66-88): from the uniform vector, until the L1 change
  * is below `tolerance` or `max_iter` iterations, then L1-normalised.  FP32
  * values and accumulation.  d_rank has V entries.  Synchronises the stream
- * once per iteration. */
+ * once to size the per-call edge bins (V <= 2^27) and once per iteration.
+ * Device scratch grows to about 8 B per edge plus 32 B per vertex. */
 int lpma_pagerank(lpma_t *h, float damping, float tolerance, uint32_t max_iter, float *d_rank, uint32_t *iterations,
                   float *last_delta, void *stream);
 
