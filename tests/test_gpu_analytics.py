@@ -56,3 +56,54 @@ def test_pagerank_fp32_within_1e6_of_fp64(scale, E, deletes):
     l1 = np.abs(got - want).sum()
     assert l1 < 1e-6, (l1, iters, delta)
     pma.close()
+
+
+def test_bfs_frontier_of_empty_rows():
+    """Level 1 is 6000 rows of which all but every 97th are empty, so one run
+    of 1024 slots spans more frontier entries than the expansion stages in
+    LDS and the rest are found by the global search."""
+    say("bfs empty-row frontier")
+    from gpu_util import keys_of
+    V = 1 << 14
+    hub = np.arange(1, 6001)
+    srcs = [np.zeros(hub.size, dtype=np.int64)]
+    dsts = [hub]
+    for i in range(1, 6001, 97):
+        srcs.append(np.full(12, i))
+        dsts.append(6001 + (i * 13 + np.arange(12) * 131) % 9000)
+    keys = oracle.sort_unique(keys_of(np.concatenate(srcs), np.concatenate(dsts)))
+    pma = L.PMA(V)
+    pma.insert(keys)
+    for src in (0, 97):
+        lv, depth, scanned = pma.bfs(src)
+        want = oracle.bfs(keys, V, src)
+        assert np.array_equal(lv.cpu().numpy().view(np.uint32), want)
+        reached = want != 0xFFFFFFFF
+        ro, _ = oracle.csr_of(keys, V)
+        deg = np.diff(ro.astype(np.int64))
+        assert scanned == int(deg[reached].sum())
+        assert depth == int(want[reached].max()) + 1
+    pma.close()
+
+
+def test_pagerank_partial_last_bin():
+    """V = 20000: three destination bins of 8192 vertices, the last one
+    partial; uniform edges with a tenth of the vertices dangling."""
+    say("pagerank V=20000")
+    from gpu_util import keys_of
+    V = 20000
+    rng = np.random.default_rng(11)
+    src = rng.integers(0, V, 300000)
+    src = src[src % 10 != 3]  # vertices 3, 13, ... have no out-edges
+    dst = rng.integers(0, V, src.size)
+    keep = src != dst
+    keys = oracle.sort_unique(keys_of(src[keep], dst[keep]))
+    pma = L.PMA(V)
+    pma.insert(keys)
+    pr, iters, delta = pma.pagerank(0.85, 1e-6, 50)
+    got = pr.cpu().numpy().astype(np.float64)
+    want, _ = oracle.pagerank(keys, V, 0.85, iters)
+    assert abs(got.sum() - 1.0) < 1e-5
+    l1 = np.abs(got - want).sum()
+    assert l1 < 1e-6, (l1, iters, delta)
+    pma.close()
