@@ -404,6 +404,10 @@ static constexpr u32 PB_MAX_BINS = 16384;  // LDS cursors of the bin pass: 64 KB
 static constexpr u32 PB_TPB = 1024;
 static constexpr u32 PB_ITEMS = 8;
 static constexpr u32 PB_MAX_PARTS = 1024;
+static constexpr u32 PB_CHUNK = 4;                // pairs per staged chunk (32 B, one HBM sector)
+static constexpr u32 PB_STAGE_MAX_BINS = 2048;    // staged bin pass LDS: 40 B per bin
+static constexpr u32 PBS_TPB = 512;               // staged bin pass: two workgroups per CU (80 KB LDS each)
+static constexpr u32 PBS_ITEMS = 8;
 
 // Lanes holding a live edge of the same bin form runs (adjacent lanes; keys
 // are sorted, so a row's edges to one bin are adjacent).  Returns the lane
@@ -427,18 +431,19 @@ __device__ __forceinline__ u32 bin_run(bool valid, u32 bin, u32 &len) {
 // live ones' keys, KEY_NONE elsewhere.  All segment counts are loaded first
 // (clamped, unconditional), then the keys with exec-masked global loads, so a
 // thread has PB_ITEMS loads in flight per phase and dead lines are not read.
-__device__ __forceinline__ void pb_load(const DevState &d, u64 t0, u64 p1, u64 (&key)[PB_ITEMS]) {
+template <u32 TPB = PB_TPB, u32 ITEMS = PB_ITEMS>
+__device__ __forceinline__ void pb_load(const DevState &d, u64 t0, u64 p1, u64 (&key)[ITEMS]) {
     const u32 Smask = (1u << d.log2S) - 1;
     const gu64 *keys = (const gu64 *)d.keys;
-    u32 live[PB_ITEMS];
+    u32 live[ITEMS];
 #pragma unroll
-    for (u32 i = 0; i < PB_ITEMS; ++i) {
-        const u64 t = t0 + (u64)i * PB_TPB + threadIdx.x;
+    for (u32 i = 0; i < ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * TPB + threadIdx.x;
         live[i] = d.cnt[(t < p1 ? t : p1 - 1) >> d.log2S];
     }
 #pragma unroll
-    for (u32 i = 0; i < PB_ITEMS; ++i) {
-        const u64 t = t0 + (u64)i * PB_TPB + threadIdx.x;
+    for (u32 i = 0; i < ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * TPB + threadIdx.x;
         u64 k = KEY_NONE;
         if (t < p1 && (u32)(t & Smask) < live[i]) k = keys[t];
         key[i] = k;
@@ -447,8 +452,42 @@ __device__ __forceinline__ void pb_load(const DevState &d, u64 t0, u64 p1, u64 (
 
 __device__ __forceinline__ bool pb_edge(u64 key) { return key != KEY_NONE && !is_marker(key); }
 
-// count[bin * P + part] = live edges of `part` into `bin`
-__global__ __launch_bounds__(PB_TPB) void k_pb_count(DevState d, u32 part_log, u32 P, u32 NB, u32 *__restrict__ count) {
+// pb_load in separate phases, for a loop that prefetches its next step:
+// the segment counts, then the keys (after the counts), then the
+// contributions of the keys' sources (after the keys)
+template <u32 TPB, u32 ITEMS>
+__device__ __forceinline__ void pb_counts(const DevState &d, u64 t0, u64 p1, u32 (&live)[ITEMS]) {
+#pragma unroll
+    for (u32 i = 0; i < ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * TPB + threadIdx.x;
+        live[i] = d.cnt[(t < p1 ? t : p1 - 1) >> d.log2S];
+    }
+}
+
+template <u32 TPB, u32 ITEMS>
+__device__ __forceinline__ void pb_keys(const DevState &d, u64 t0, u64 p1, const u32 (&live)[ITEMS],
+                                        u64 (&key)[ITEMS]) {
+    const u32 Smask = (1u << d.log2S) - 1;
+    const gu64 *keys = (const gu64 *)d.keys;
+#pragma unroll
+    for (u32 i = 0; i < ITEMS; ++i) {
+        const u64 t = t0 + (u64)i * TPB + threadIdx.x;
+        u64 k = KEY_NONE;
+        if (t < p1 && (u32)(t & Smask) < live[i]) k = keys[t];
+        key[i] = k;
+    }
+}
+
+template <u32 ITEMS>
+__device__ __forceinline__ void pb_contrib(const float *__restrict__ c, const u64 (&key)[ITEMS], float (&x)[ITEMS]) {
+#pragma unroll
+    for (u32 i = 0; i < ITEMS; ++i) x[i] = c[pb_edge(key[i]) ? (u32)(key[i] >> 32) : 0u];  // always in bounds
+}
+
+// count[bin * P + part] = live edges of `part` into `bin`, rounded up to a
+// multiple of pad + 1 (pad = PB_CHUNK - 1 for the staged bin pass, else 0)
+__global__ __launch_bounds__(PB_TPB) void k_pb_count(DevState d, u32 part_log, u32 P, u32 NB, u32 pad,
+                                                     u32 *__restrict__ count) {
     extern __shared__ u32 pb_lds[];
     for (u32 b = threadIdx.x; b < NB; b += PB_TPB) pb_lds[b] = 0;
     __syncthreads();
@@ -468,7 +507,7 @@ __global__ __launch_bounds__(PB_TPB) void k_pb_count(DevState d, u32 part_log, u
         }
     }
     __syncthreads();
-    for (u32 b = threadIdx.x; b < NB; b += PB_TPB) count[(u64)b * P + blockIdx.x] = pb_lds[b];
+    for (u32 b = threadIdx.x; b < NB; b += PB_TPB) count[(u64)b * P + blockIdx.x] = (pb_lds[b] + pad) & ~pad;
 }
 
 // pairs[off[bin * P + part] + rank] = (contribution bits << 32) | (dst mod PB_BIN)
@@ -506,6 +545,121 @@ __global__ __launch_bounds__(PB_TPB) void k_pb_bin(DevState d, u32 part_log, u32
             if (pb_edge(key[i]) && pos < npairs)
                 pairs[pos] = ((u64)__float_as_uint(x[i]) << 32) | ((u32)key[i] & (PB_BIN - 1));
         }
+    }
+}
+
+// Staged form of the bin pass (bins <= PB_STAGE_MAX_BINS): a bin's pairs
+// reach HBM only as whole aligned chunks of PB_CHUNK pairs (32 B), so no
+// partly written line is evicted from L2 and written twice.  Every run of
+// the count is padded to a multiple of PB_CHUNK; padding pairs are zero (no
+// contribution).  Per step, phase by phase:
+//   1. ranks: each edge takes rank r on its bin's fill counter (run-aggregated);
+//   2. r < PB_CHUNK goes to the bin's LDS chunk, r inside a further complete
+//      chunk straight to HBM (a run's lanes are adjacent: coalesced), the
+//      rest waits;
+//   3. full LDS chunks are written out and the bin's position advances;
+//   4. the waiting edges start the next LDS chunk.
+// At the end of the part each bin's last chunk is written with zero padding.
+__global__ __launch_bounds__(PBS_TPB) void k_pb_bin_staged(DevState d, u32 part_log, u32 P, u32 NB,
+                                                          const u32 *__restrict__ off, const float *__restrict__ c,
+                                                          u64 *__restrict__ pairs, u64 npairs) {
+    extern __shared__ u64 pb_stage[];
+    u64 *buf = pb_stage;                                  // NB * PB_CHUNK staged pairs
+    u32 *fill = (u32 *)(buf + (u64)NB * PB_CHUNK);        // staged pairs per bin (< PB_CHUNK between steps)
+    u32 *gpos = fill + NB;                                // pair index of the bin's LDS chunk in HBM
+    for (u32 b = threadIdx.x; b < NB; b += PBS_TPB) {
+        fill[b] = 0;
+        gpos[b] = off[(u64)b * P + blockIdx.x];
+    }
+    __syncthreads();
+    const u64 W = nseg_of(d) << d.log2S;
+    const u64 p0 = (u64)blockIdx.x << part_log;
+    const u64 p1 = min(W, p0 + (1ull << part_log));
+    const u32 lane = lane_id();
+    // the next step's loads go out one phase ahead of their use: counts
+    // during the ranking, keys during the placement, contributions during
+    // the flush (the barriers wait for LDS only)
+    constexpr u64 STEP = (u64)PBS_TPB * PBS_ITEMS;
+    u32 live_n[PBS_ITEMS];
+    u64 key_n[PBS_ITEMS];
+    float x_n[PBS_ITEMS];
+    pb_counts<PBS_TPB, PBS_ITEMS>(d, p0, p1, live_n);
+    pb_keys<PBS_TPB, PBS_ITEMS>(d, p0, p1, live_n, key_n);
+    pb_contrib<PBS_ITEMS>(c, key_n, x_n);
+    for (u64 t0 = p0; t0 < p1; t0 += STEP) {
+        u64 key[PBS_ITEMS];
+        float x[PBS_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < PBS_ITEMS; ++i) {
+            key[i] = key_n[i];
+            x[i] = x_n[i];
+        }
+        const bool more = t0 + STEP < p1;  // uniform
+        if (more) pb_counts<PBS_TPB, PBS_ITEMS>(d, t0 + STEP, p1, live_n);
+        u32 rank[PBS_ITEMS];
+#pragma unroll
+        for (u32 i = 0; i < PBS_ITEMS; ++i) {
+            const bool valid = pb_edge(key[i]);
+            u32 len;
+            u32 lead = bin_run(valid, (u32)key[i] >> PB_BIN_LOG, len);
+            u32 base = 0;
+            if (valid && lead == lane) base = atomicAdd(&fill[(u32)key[i] >> PB_BIN_LOG], len);
+            if (!valid) lead = lane;
+            rank[i] = __shfl(base, lead, 64) + (lane - lead);
+        }
+        __syncthreads();
+        if (more) pb_keys<PBS_TPB, PBS_ITEMS>(d, t0 + STEP, p1, live_n, key_n);
+        u32 full[PBS_ITEMS];
+        u32 wait = 0;
+#pragma unroll
+        for (u32 i = 0; i < PBS_ITEMS; ++i) {
+            full[i] = 0;
+            if (!pb_edge(key[i])) continue;
+            const u32 bin = (u32)key[i] >> PB_BIN_LOG;
+            const u64 pair = ((u64)__float_as_uint(x[i]) << 32) | ((u32)key[i] & (PB_BIN - 1));
+            full[i] = fill[bin] & ~(PB_CHUNK - 1);
+            if (rank[i] < PB_CHUNK) {
+                buf[bin * PB_CHUNK + rank[i]] = pair;
+            } else if (rank[i] < full[i]) {
+                const u64 pos = (u64)gpos[bin] + rank[i];
+                if (pos < npairs) pairs[pos] = pair;
+            } else {
+                wait |= 1u << i;
+            }
+        }
+        __syncthreads();
+        if (more) pb_contrib<PBS_ITEMS>(c, key_n, x_n);
+        for (u32 b = threadIdx.x; b < NB; b += PBS_TPB) {
+            const u32 F = fill[b];
+            if (F >= PB_CHUNK) {
+                const u64 pos = gpos[b];
+                if (pos + PB_CHUNK <= npairs) {
+                    const uint4 *src = (const uint4 *)(buf + b * PB_CHUNK);
+                    uint4 *dst = (uint4 *)(pairs + pos);
+#pragma unroll
+                    for (u32 k = 0; k < PB_CHUNK / 2; ++k) dst[k] = src[k];
+                }
+                gpos[b] = (u32)(pos + (F & ~(PB_CHUNK - 1)));
+                fill[b] = F & (PB_CHUNK - 1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (u32 i = 0; i < PBS_ITEMS; ++i)
+            if (wait & (1u << i)) {
+                const u32 bin = (u32)key[i] >> PB_BIN_LOG;
+                buf[bin * PB_CHUNK + (rank[i] - full[i])] =
+                    ((u64)__float_as_uint(x[i]) << 32) | ((u32)key[i] & (PB_BIN - 1));
+            }
+        // the next step's ranks read fill (settled before the last barrier);
+        // its LDS chunk writes come after its first barrier
+    }
+    __syncthreads();
+    for (u32 b = threadIdx.x; b < NB; b += PBS_TPB) {
+        const u32 F = fill[b];
+        const u64 pos = gpos[b];
+        if (F && pos + PB_CHUNK <= npairs)
+            for (u32 k = 0; k < PB_CHUNK; ++k) pairs[pos + k] = k < F ? buf[b * PB_CHUNK + k] : 0ull;
     }
 }
 
@@ -643,6 +797,7 @@ PrBins pr_bins_plan(const DevState &d) {
     b.part_log = std::max<u32>(d.log2S, lw > 10 ? lw - 10 : 0);  // at most PB_MAX_PARTS parts
     b.parts = (u32)((W + (1ull << b.part_log) - 1) >> b.part_log);
     b.bins = (u32)(((u64)d.V + PB_BIN - 1) / PB_BIN);
+    b.staged = b.bins <= PB_STAGE_MAX_BINS;
     return b;
 }
 
@@ -650,7 +805,8 @@ u64 pr_bins_count_words(const PrBins &b) { return (u64)b.bins * b.parts + 1; }
 
 hipError_t pr_bins_count(const DevState &d, const PrBins &b, u32 *off, u32 *scan_scratch, hipStream_t st) {
     if (b.parts > PB_MAX_PARTS || b.bins > PB_MAX_BINS || b.bins == 0) return hipErrorInvalidValue;
-    k_pb_count<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off);
+    k_pb_count<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins,
+                                                             b.staged ? PB_CHUNK - 1 : 0u, off);
     return device_excl_scan(off, (u64)b.bins * b.parts, off, scan_scratch, st);
 }
 
@@ -673,7 +829,12 @@ hipError_t pr_iterate_binned(const DevState &d, const PrBins &b, const float *pr
     const u32 V = d.V;
     k_pr_reset<<<1, 64, 0, st>>>(red_next);
     k_pr_prep<<<grid_for(V, PR_TPB, PR_GRID), PR_TPB, 0, st>>>(pr, outdeg, V, c, nullptr, 0, red);
-    k_pb_bin<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off, c, pairs, b.pairs);
+    if (b.staged)
+        k_pb_bin_staged<<<b.parts, PBS_TPB, b.bins * (PB_CHUNK * sizeof(u64) + 2 * sizeof(u32)), st>>>(
+            d, b.part_log, b.parts, b.bins, off, c, pairs, b.pairs);
+    else
+        k_pb_bin<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off, c, pairs,
+                                                                b.pairs);
     if (b.max_chunks)
         k_pb_acc<<<b.max_chunks, PB_TPB, PB_BIN * sizeof(u64), st>>>(pairs, off, b.parts, b.bins, cb, b.chunk,
                                                                           (float2 *)partial);

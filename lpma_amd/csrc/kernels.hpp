@@ -99,6 +99,7 @@ struct PrBins {
     u32 bins;        // destination bins of 8192 vertices
     u32 chunk;       // pairs per accumulation chunk
     u32 max_chunks;  // bound on the chunks of all bins (the acc grid)
+    u32 staged;      // bin pass stages whole 32-byte chunks in LDS (runs padded)
     u64 pairs;       // live edges
 };
 bool pr_bins_fit(u32 V);

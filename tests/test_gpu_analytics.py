@@ -107,3 +107,25 @@ def test_pagerank_partial_last_bin():
     l1 = np.abs(got - want).sum()
     assert l1 < 1e-6, (l1, iters, delta)
     pma.close()
+
+
+def test_pagerank_more_bins_than_staged_pass():
+    """V = 2^24 + 5: 2049 destination bins, past the staged bin pass's LDS,
+    so the unstaged bin pass (no padding) runs."""
+    say("pagerank V=2^24+5")
+    from gpu_util import keys_of
+    V = (1 << 24) + 5
+    rng = np.random.default_rng(12)
+    src = rng.integers(0, V, 200000)
+    dst = np.concatenate([rng.integers(0, 1 << 12, 100000), rng.integers(V - 4096, V, 100000)])
+    keep = src != dst
+    keys = oracle.sort_unique(keys_of(src[keep], dst[keep]))
+    pma = L.PMA(V)
+    pma.insert(keys)
+    pr, iters, delta = pma.pagerank(0.85, 1e-6, 50)
+    got = pr.cpu().numpy().astype(np.float64)
+    want, _ = oracle.pagerank(keys, V, 0.85, iters)
+    assert abs(got.sum() - 1.0) < 1e-5
+    l1 = np.abs(got - want).sum()
+    assert l1 < 1e-6, (l1, iters, delta)
+    pma.close()
