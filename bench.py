@@ -143,10 +143,13 @@ def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
     times = []
     n = 0
     empty = np.empty(0, dtype=np.uint64)
+    # two output buffers, touched once before timing, used in turn
+    cap = S.size + sum(int(b.size) for b in ins_batches[:n_ins]) + 1
+    bufs = [np.zeros(cap, dtype=np.uint64), np.zeros(cap, dtype=np.uint64)]
     for i in range(n_ins):
         t = time.perf_counter()
-        S = oracle.apply(S, ins_batches[i], empty, threads=threads)
-        oracle.row_offsets(S, V)
+        S = oracle.apply(S, ins_batches[i], empty, threads=threads, out=bufs[i & 1])
+        oracle.row_offsets(S, V, threads=threads)
         times.append(time.perf_counter() - t)
         n += ins_batches[i].size
     # deletes sampled from the edges live at this point
@@ -154,8 +157,8 @@ def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
     for i in range(n_del):
         d = S[rng.choice(S.size, del_batches[i].size, replace=False)]
         t = time.perf_counter()
-        S = oracle.apply(S, empty, d, threads=threads)
-        oracle.row_offsets(S, V)
+        S = oracle.apply(S, empty, d, threads=threads, out=bufs[(n_ins + i) & 1])
+        oracle.row_offsets(S, V, threads=threads)
         times.append(time.perf_counter() - t)
         n += d.size
     return n / sum(times), times
@@ -772,11 +775,23 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         nb = min(args.cpu_batches, len(ins_b), len(del_b))
         t = time.time()
-        v_cpu, _ = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, threads)
+        v_cpu, times = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, threads)
         log(f"cpu baseline ({threads} threads): {v_cpu:.3e} updates/s in {time.time() - t:.1f}s")
-        cpu = {"value": v_cpu, "unit": "edge updates/s", "cores": threads, "kind": "port",
-               "sample": f"CPU oracle (sorted merge + CSR offsets, pthreads) on the first {nb} insert and {nb} "
-                         f"delete batches of {B} of the same stream, from the same bulk-loaded state"}
+        # the same oracle on one thread over the same sample; the baseline is
+        # the faster of the two (the merge is memory-bound at this size)
+        v_one, times_one = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, 1)
+        log(f"cpu baseline (1 thread): {v_one:.3e} updates/s")
+        runs = {threads: (v_cpu, times), 1: (v_one, times_one)}
+        best = max(runs, key=lambda k: runs[k][0])
+        cpu = {"value": runs[best][0], "unit": "edge updates/s", "cores": best, "kind": "port",
+               "sample": f"CPU oracle (sorted merge + CSR offsets) on the first {nb} insert and {nb} delete batches "
+                         f"of {B} of the same stream, from the same bulk-loaded state, on {threads} threads and on "
+                         f"one; the faster is reported",
+               "p50_batch_ms": float(np.percentile(runs[best][1], 50)) * 1e3,
+               "p99_batch_ms": float(np.percentile(runs[best][1], 99)) * 1e3,
+               "by_threads": {str(k): {"value": v, "p50_batch_ms": float(np.percentile(ts, 50)) * 1e3,
+                                       "p99_batch_ms": float(np.percentile(ts, 99)) * 1e3}
+                              for k, (v, ts) in runs.items()}}
 
     out = {
         "metric": "edge updates/sec (R-MAT 2^20 V, 16M E, batch 1e5); p99 batch latency; HBM GB/s",

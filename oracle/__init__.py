@@ -47,6 +47,8 @@ def lib():
         L.or_apply.restype = ctypes.c_size_t
         L.or_row_offsets.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint32, _u32p]
         L.or_row_offsets.restype = ctypes.c_int
+        L.or_row_offsets_mt.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint32, _u32p, ctypes.c_int]
+        L.or_row_offsets_mt.restype = ctypes.c_int
         L.or_is_sorted_unique.argtypes = [_u64p, ctypes.c_size_t]
         L.or_is_sorted_unique.restype = ctypes.c_int
         L.or_pma_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
@@ -96,21 +98,30 @@ def sort_unique(a):
     return a[:n]
 
 
-def apply(S, ins=(), dels=(), threads=1):
-    """(S \\ dels) | ins as a sorted unique uint64 array.  S must be sorted unique."""
+def apply(S, ins=(), dels=(), threads=1, out=None):
+    """(S \\ dels) | ins as a sorted unique uint64 array.  S must be sorted unique.
+    `out` (optional, not overlapping S): a contiguous uint64 buffer of at least
+    S.size + len(ins) + 1 entries to merge into; the result is a view of it."""
     S = _as_u64(S)
     I = _as_u64(ins)
     D = _as_u64(dels)
-    out = np.empty(S.size + I.size + 1, dtype=np.uint64)
+    need = S.size + I.size + 1
+    if out is None:
+        out = np.empty(need, dtype=np.uint64)
+    elif out.dtype != np.uint64 or not out.flags.c_contiguous or out.size < need or np.may_share_memory(out, S):
+        raise ValueError("out must be a contiguous uint64 buffer of S.size + len(ins) + 1 entries apart from S")
     n = lib().or_apply(_p64(S), S.size, _p64(I), I.size, _p64(D), D.size, _p64(out), int(threads))
     return out[:n]
 
 
-def row_offsets(keys, V):
+def row_offsets(keys, V, threads=1):
     """Standard CSR offsets: ro[v] = #edge keys with src < v (row markers skipped)."""
     keys = _as_u64(keys)
     ro = np.empty(int(V) + 1, dtype=np.uint32)
-    r = lib().or_row_offsets(_p64(keys), keys.size, int(V), ro.ctypes.data_as(_u32p))
+    if threads > 1:
+        r = lib().or_row_offsets_mt(_p64(keys), keys.size, int(V), ro.ctypes.data_as(_u32p), int(threads))
+    else:
+        r = lib().or_row_offsets(_p64(keys), keys.size, int(V), ro.ctypes.data_as(_u32p))
     if r != 0:
         raise ValueError("key with src >= V")
     return ro

@@ -107,7 +107,8 @@ static size_t lower_bound_u64(const u64 *a, size_t n, u64 x) {
 /* ------------------------------------------------------------------------- */
 
 /* Sequential kernel of or_apply: merge sorted-unique S, I, D restricted to
- * the given index ranges; out = (S \ D) u I.  Returns the output length. */
+ * the given index ranges; out = (S \ D) u I.  Returns the output length;
+ * out == NULL only counts. */
 static size_t merge_range(const u64 *S, size_t s0, size_t s1, const u64 *I, size_t i0, size_t i1,
                           const u64 *D, size_t d0, size_t d1, u64 *out) {
     size_t o = 0, s = s0, i = i0, d = d0;
@@ -122,7 +123,8 @@ static size_t merge_range(const u64 *S, size_t s0, size_t s1, const u64 *I, size
         } else {
             x = S[s++]; ++i; /* present and re-inserted: delete-then-insert keeps it */
         }
-        out[o++] = x;
+        if (out) out[o] = x;
+        ++o;
     }
     return o;
 }
@@ -130,14 +132,21 @@ static size_t merge_range(const u64 *S, size_t s0, size_t s1, const u64 *I, size
 typedef struct {
     const u64 *S, *I, *D;
     size_t s0, s1, i0, i1, d0, d1;
-    u64 *buf;
+    u64 *out;  /* NULL: count only */
     size_t n;
 } merge_job;
 
 static void *merge_worker(void *p) {
     merge_job *j = (merge_job *)p;
-    j->n = merge_range(j->S, j->s0, j->s1, j->I, j->i0, j->i1, j->D, j->d0, j->d1, j->buf);
+    j->n = merge_range(j->S, j->s0, j->s1, j->I, j->i0, j->i1, j->D, j->d0, j->d1, j->out);
     return NULL;
+}
+
+static void run_jobs(merge_job *jobs, int T) {
+    pthread_t *tid = (pthread_t *)calloc(T, sizeof(pthread_t));
+    for (int t = 0; t < T; ++t) pthread_create(&tid[t], NULL, merge_worker, &jobs[t]);
+    for (int t = 0; t < T; ++t) pthread_join(tid[t], NULL);
+    free(tid);
 }
 
 /*
@@ -159,9 +168,10 @@ size_t or_apply(const u64 *S, size_t ns, const u64 *I_in, size_t ni, const u64 *
     if (threads <= 1 || ns < (size_t)threads * 1024) {
         total = merge_range(S, 0, ns, I, 0, ni, D, 0, nd, out);
     } else {
+        /* key-range partition at equally spaced keys of S; each thread counts
+         * its range's output, then merges straight into its place in out */
         int T = threads;
         merge_job *jobs = (merge_job *)calloc(T, sizeof(merge_job));
-        pthread_t *tid = (pthread_t *)calloc(T, sizeof(pthread_t));
         size_t prev_s = 0, prev_i = 0, prev_d = 0;
         for (int t = 0; t < T; ++t) {
             size_t s1, i1, d1;
@@ -177,19 +187,17 @@ size_t or_apply(const u64 *S, size_t ns, const u64 *I_in, size_t ni, const u64 *
             jobs[t].s0 = prev_s; jobs[t].s1 = s1;
             jobs[t].i0 = prev_i; jobs[t].i1 = i1;
             jobs[t].d0 = prev_d; jobs[t].d1 = d1;
-            jobs[t].buf = (u64 *)malloc(((s1 - prev_s) + (i1 - prev_i) + 1) * sizeof(u64));
+            jobs[t].out = NULL;
             prev_s = s1; prev_i = i1; prev_d = d1;
         }
-        for (int t = 0; t < T; ++t) pthread_create(&tid[t], NULL, merge_worker, &jobs[t]);
-        for (int t = 0; t < T; ++t) pthread_join(tid[t], NULL);
+        run_jobs(jobs, T);
         total = 0;
         for (int t = 0; t < T; ++t) {
-            memcpy(out + total, jobs[t].buf, jobs[t].n * sizeof(u64));
+            jobs[t].out = out + total;
             total += jobs[t].n;
-            free(jobs[t].buf);
         }
+        run_jobs(jobs, T);
         free(jobs);
-        free(tid);
     }
     free(I);
     free(D);
@@ -211,6 +219,54 @@ int or_row_offsets(const u64 *keys, size_t n, u32 V, u32 *ro) {
         if (src >= V) return -1;
         ro[src + 1]++;
     }
+    for (u32 v = 0; v < V; ++v) ro[v + 1] += ro[v];
+    return 0;
+}
+
+typedef struct {
+    const u64 *keys;
+    size_t n;
+    u32 v0, v1;
+    u32 *ro;
+} ro_job;
+
+/* ro[v + 1] = edges of row v for v in [v0, v1) (rows are disjoint per job) */
+static void *ro_worker(void *p) {
+    ro_job *j = (ro_job *)p;
+    size_t i = lower_bound_u64(j->keys, j->n, (u64)j->v0 << 32);
+    const size_t e = lower_bound_u64(j->keys, j->n, (u64)j->v1 << 32);
+    for (u32 v = j->v0; v < j->v1; ++v) j->ro[v + 1] = 0;
+    for (; i < e; ++i) {
+        u64 k = j->keys[i];
+        if ((u32)k == 0xFFFFFFFFu) continue;
+        j->ro[(u32)(k >> 32) + 1]++;
+    }
+    return NULL;
+}
+
+/* or_row_offsets on `threads` threads: rows split into equal vertex ranges,
+ * each counted by one thread (binary search for its key range), then one
+ * prefix sum.  Same result and error as or_row_offsets. */
+int or_row_offsets_mt(const u64 *keys, size_t n, u32 V, u32 *ro, int threads) {
+    if (threads <= 1 || V < (u32)threads) return or_row_offsets(keys, n, V, ro);
+    size_t m = n; /* sorted: the last edge key (markers skipped) has the largest src */
+    while (m && (u32)keys[m - 1] == 0xFFFFFFFFu) --m;
+    if (m && (u32)(keys[m - 1] >> 32) >= V) return -1;
+    int T = threads;
+    ro_job *jobs = (ro_job *)calloc(T, sizeof(ro_job));
+    pthread_t *tid = (pthread_t *)calloc(T, sizeof(pthread_t));
+    for (int t = 0; t < T; ++t) {
+        jobs[t].keys = keys;
+        jobs[t].n = n;
+        jobs[t].v0 = (u32)((u64)V * t / T);
+        jobs[t].v1 = (u32)((u64)V * (t + 1) / T);
+        jobs[t].ro = ro;
+        pthread_create(&tid[t], NULL, ro_worker, &jobs[t]);
+    }
+    for (int t = 0; t < T; ++t) pthread_join(tid[t], NULL);
+    free(jobs);
+    free(tid);
+    ro[0] = 0;
     for (u32 v = 0; v < V; ++v) ro[v + 1] += ro[v];
     return 0;
 }

@@ -54,6 +54,14 @@ def test_apply_against_numpy_medium():
     want = np.union1d(np.setdiff1d(S, D), I)
     for threads in (1, 2, 4, 7):
         assert np.array_equal(oracle.apply(S, I, D, threads=threads), want)
+        # into a caller's buffer (reused, stale contents beyond the result)
+        buf = np.full(S.size + I.size + 5, 7, dtype=np.uint64)
+        assert np.array_equal(oracle.apply(S, I, D, threads=threads, out=buf), want)
+    with pytest.raises(ValueError):
+        oracle.apply(S, I, D, out=np.empty(S.size, dtype=np.uint64))  # too small
+    with pytest.raises(ValueError):
+        big = np.concatenate([S, np.zeros(I.size + 1, dtype=np.uint64)])
+        oracle.apply(big[:S.size], I, D, out=big)  # overlaps S
 
 
 def test_apply_edges_of_domain():
@@ -90,6 +98,17 @@ def test_row_offsets():
     assert np.array_equal(oracle.row_offsets(withm, V), ro)
     with pytest.raises(ValueError):
         oracle.row_offsets(keys_of([V], [0]), V)
+    # the threaded form (rows split over threads) gives the same offsets and error
+    for threads in (2, 3, 7, 16):
+        assert np.array_equal(oracle.row_offsets(S, V, threads=threads), ro)
+        assert np.array_equal(oracle.row_offsets(withm, V, threads=threads), ro)
+        with pytest.raises(ValueError):
+            oracle.row_offsets(keys_of([3, V], [0, 0]), V, threads=threads)
+    # empty rows at the ends and a single hub row
+    hub = keys_of(np.full(50, 40), np.arange(50))
+    for threads in (1, 4):
+        got = oracle.row_offsets(hub, V, threads=threads)
+        assert got[40] == 0 and got[41] == 50 and got[V] == 50
 
 
 def model_check(m, V, S):
