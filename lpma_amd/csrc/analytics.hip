@@ -408,6 +408,7 @@ static constexpr u32 PB_CHUNK = 4;                // pairs per staged chunk (32 
 static constexpr u32 PB_STAGE_MAX_BINS = 2048;    // staged bin pass LDS: 40 B per bin
 static constexpr u32 PBS_TPB = 512;               // staged bin pass: two workgroups per CU (80 KB LDS each)
 static constexpr u32 PBS_ITEMS = 8;
+static constexpr u32 PBA_TPB = 512;               // acc pass: fewer waves per LDS accumulator contend less
 
 // Lanes holding a live edge of the same bin form runs (adjacent lanes; keys
 // are sorted, so a row's edges to one bin are adjacent).  Returns the lane
@@ -697,7 +698,7 @@ __device__ __forceinline__ void lds_add2_retry(u64 *acc, u32 v, float x, u64 cur
 
 // chunk k of bin b (cb = exclusive scan of nch, cb[NB] = chunks) summed in
 // LDS into partial[k * PB_BIN + v] = (hi, lo)
-__global__ __launch_bounds__(PB_TPB) void k_pb_acc(const u64 *__restrict__ pairs, const u32 *__restrict__ off, u32 P,
+__global__ __launch_bounds__(PBA_TPB) void k_pb_acc(const u64 *__restrict__ pairs, const u32 *__restrict__ off, u32 P,
                                                    u32 NB, const u32 *__restrict__ cb, u32 chunk,
                                                    float2 *__restrict__ partial) {
     extern __shared__ u64 pb_acc[];  // PB_BIN words: (lo << 32) | hi
@@ -712,13 +713,13 @@ __global__ __launch_bounds__(PB_TPB) void k_pb_acc(const u64 *__restrict__ pairs
     const u64 e1 = off[(u64)(b + 1) * P];
     const u64 s0 = (u64)off[(u64)b * P] + (u64)(k - cb[b]) * chunk;
     const u64 s1 = min(e1, s0 + chunk);
-    for (u32 v = threadIdx.x; v < PB_BIN; v += PB_TPB) pb_acc[v] = 0ull;
+    for (u32 v = threadIdx.x; v < PB_BIN; v += PBA_TPB) pb_acc[v] = 0ull;
     __syncthreads();
-    for (u64 i0 = s0; i0 < s1; i0 += (u64)PB_TPB * PB_ITEMS) {
+    for (u64 i0 = s0; i0 < s1; i0 += (u64)PBA_TPB * PB_ITEMS) {
         u64 pr[PB_ITEMS];
 #pragma unroll
         for (u32 r = 0; r < PB_ITEMS; ++r) {
-            const u64 i = i0 + (u64)r * PB_TPB + threadIdx.x;
+            const u64 i = i0 + (u64)r * PBA_TPB + threadIdx.x;
             pr[r] = pairs[i < s1 ? i : s1 - 1];  // s1 > s0: a chunk is never empty
             if (i >= s1) pr[r] = 0ull;           // x = +0 adds nothing
         }
@@ -744,7 +745,7 @@ __global__ __launch_bounds__(PB_TPB) void k_pb_acc(const u64 *__restrict__ pairs
     }
     __syncthreads();
     float2 *out = partial + (u64)k * PB_BIN;
-    for (u32 v = threadIdx.x; v < PB_BIN; v += PB_TPB) {
+    for (u32 v = threadIdx.x; v < PB_BIN; v += PBA_TPB) {
         const u64 w = pb_acc[v];
         out[v] = make_float2(__uint_as_float((u32)w), __uint_as_float((u32)(w >> 32)));
     }
@@ -836,7 +837,7 @@ hipError_t pr_iterate_binned(const DevState &d, const PrBins &b, const float *pr
         k_pb_bin<<<b.parts, PB_TPB, b.bins * sizeof(u32), st>>>(d, b.part_log, b.parts, b.bins, off, c, pairs,
                                                                 b.pairs);
     if (b.max_chunks)
-        k_pb_acc<<<b.max_chunks, PB_TPB, PB_BIN * sizeof(u64), st>>>(pairs, off, b.parts, b.bins, cb, b.chunk,
+        k_pb_acc<<<b.max_chunks, PBA_TPB, PB_BIN * sizeof(u64), st>>>(pairs, off, b.parts, b.bins, cb, b.chunk,
                                                                           (float2 *)partial);
     k_pb_finish<<<grid_for(V, PR_TPB, PR_GRID), PR_TPB, 0, st>>>((const float2 *)partial, cb, pr, out, V, alpha, red,
                                                          red_next);
