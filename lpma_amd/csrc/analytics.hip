@@ -59,7 +59,7 @@ __global__ void k_bfs_len(DevState d, const u32 *__restrict__ q, u32 qn, u32 *__
     }
 }
 
-static constexpr u32 EXP_TPB = 256;
+static constexpr u32 EXP_TPB = 512;
 static constexpr u32 EXP_ITEMS = 4;
 static constexpr u32 EXP_RUN = EXP_TPB * EXP_ITEMS;  // slots per run
 static constexpr u32 EXP_STAGE = EXP_RUN + 1;        // staged entries: every entry of a run of non-empty

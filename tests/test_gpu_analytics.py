@@ -60,8 +60,8 @@ def test_pagerank_fp32_within_1e6_of_fp64(scale, E, deletes):
 
 def test_bfs_frontier_of_empty_rows():
     """Level 1 is 6000 rows of which all but every 97th are empty, so one run
-    of 1024 slots spans more frontier entries than the expansion stages in
-    LDS and the rest are found by the global search."""
+    of the expansion (2048 slots) spans more frontier entries than it stages
+    in LDS and the rest are found by the global search."""
     say("bfs empty-row frontier")
     from gpu_util import keys_of
     V = 1 << 14
