@@ -588,11 +588,13 @@ def run_analytics(args):
                "pagerank_ms": pr_s * 1e3, "pagerank_iterations": iters, "pagerank_last_l1_change": delta,
                "pagerank_edges_per_s": iters * E / pr_s, "pagerank_alg_GBps": pr_bytes / pr_s / 1e9,
                "bfs_ms": bfs_s * 1e3, "bfs_depth": depth, "bfs_reached": reached, "bfs_edges_scanned": scanned,
-               "bfs_edges_per_s": scanned / bfs_s}
+               "bfs_edges_per_s": scanned / bfs_s,
+               # per scanned edge its key (8 B) and a visited-word probe (4 B)
+               "bfs_alg_GBps": 12 * scanned / bfs_s / 1e9}
         rows.append(row)
         log(f"analytics batch {b} ({op}): update {upd_ms:.2f} ms, PageRank {iters} it in {pr_s * 1e3:.1f} ms "
             f"({row['pagerank_edges_per_s']:.3e} edges/s, {row['pagerank_alg_GBps']:.0f} GB/s), BFS depth {depth} "
-            f"{bfs_s * 1e3:.1f} ms ({row['bfs_edges_per_s']:.3e} edges/s)")
+            f"{bfs_s * 1e3:.1f} ms ({row['bfs_edges_per_s']:.3e} edges/s, {row['bfs_alg_GBps']:.0f} GB/s)")
     parity = None
     cpu = None
     if not args.no_cpu:
@@ -640,6 +642,8 @@ def run_analytics(args):
                    "parallelism": "single GPU"},
         "batches": rows,
         "bfs_edges_per_s": last["bfs_edges_per_s"],
+        "bfs_alg_GBps": last["bfs_alg_GBps"],
+        "pagerank_alg_GBps": last["pagerank_alg_GBps"],
         "parity": parity,
         "cpu_baseline": cpu,
     }
