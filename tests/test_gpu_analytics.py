@@ -129,3 +129,26 @@ def test_pagerank_more_bins_than_staged_pass():
     l1 = np.abs(got - want).sum()
     assert l1 < 1e-6, (l1, iters, delta)
     pma.close()
+
+
+def test_pagerank_atomic_push_above_binned_limit():
+    """V = 2^27 + 1: more destination bins than the binned push's LDS cursors
+    hold, so PageRank takes the per-edge compensated atomic push."""
+    say("pagerank V=2^27+1")
+    from gpu_util import keys_of
+    V = (1 << 27) + 1
+    rng = np.random.default_rng(13)
+    src = rng.integers(0, V, 100000)
+    dst = np.concatenate([rng.integers(0, 4096, 50000), rng.integers(V - 4096, V, 50000)])
+    keep = src != dst
+    keys = oracle.sort_unique(keys_of(src[keep], dst[keep]))
+    pma = L.PMA(V)
+    pma.insert(keys)
+    pr, iters, delta = pma.pagerank(0.85, 1e-6, 50)
+    got = pr.cpu().numpy().astype(np.float64)
+    del pr
+    want, _ = oracle.pagerank(keys, V, 0.85, iters)
+    assert abs(got.sum() - 1.0) < 1e-5
+    l1 = np.abs(got - want).sum()
+    assert l1 < 1e-6, (l1, iters, delta)
+    pma.close()
