@@ -206,7 +206,7 @@ int lpma_snapshot_edge_query(lpma_snapshot_t *s, const uint64_t *d_keys, uint64_
 int lpma_snapshot_bfs(lpma_snapshot_t *s, uint32_t src, uint32_t *d_levels, uint32_t *depth, uint64_t *edges_scanned,
                       void *stream);
 /* Workgroups a search on this snapshot expands its frontier with (0: the
- * default 2048, enough for the whole device).  Concurrent searches beside a
+ * default, as many as the device holds at once).  Concurrent searches beside a
  * latency-sensitive writer use fewer, so the writer's kernels find free
  * compute units. */
 int lpma_snapshot_set_search_width(lpma_snapshot_t *s, uint32_t workgroups);

@@ -4,6 +4,8 @@
 // :136-186 BFS; test.cu:170-182 converts with to_csr_rpma first).  Here both
 // read the PMA in place: row r's out-edges are the live slots between the
 // markers of rows r-1 and r, i.e. slots (row_off[r], row_off[r+1]).
+#include <atomic>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -193,13 +195,34 @@ hipError_t bfs_init(u32 *levels, u32 V, u32 src, u32 *queue, u32 *counters, u32 
 
 u64 bfs_run_words(u64 slots) { return slots / EXP_RUN + 2; }
 
+// Default expansion grid: every workgroup resident at once (compute units x
+// workgroups per CU that fit), so the runs are shared out in one round; a
+// grid one resident round larger leaves its last workgroups a tail of their
+// own (2048 workgroups of 512 threads: 9.1 ms per search, 1024: 11.6, one
+// round of 768: 8.1 on MI355X).
+static u32 bfs_expand_grid() {
+    static std::atomic<u32> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2048;
+    u32 g = cached[dev].load(std::memory_order_relaxed);
+    if (g) return g;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs_expand, EXP_TPB, 0) != hipSuccess || cus <= 0 ||
+        per_cu <= 0)
+        return 2048;
+    g = (u32)(cus * per_cu);
+    cached[dev].store(g, std::memory_order_relaxed);
+    return g;
+}
+
 hipError_t bfs_level(const DevState &d, const u32 *q, u32 qn, u32 *len, u64 *first, u32 *runs, u32 *scan_scratch,
                      u32 *levels, u32 *visited, u32 level, u32 *qnext, u32 *counters, u32 expand_grid,
                      hipStream_t st) {
     k_bfs_len<<<grid_for(qn, 256, 16384), 256, 0, st>>>(d, q, qn, len, first);
     (void)device_excl_scan(len, qn, len, scan_scratch, st);
     k_bfs_runs<<<grid_for(bfs_run_words(nseg_of(d) << d.log2S), 256, 16384), 256, 0, st>>>(len, qn, runs);
-    k_bfs_expand<<<expand_grid ? expand_grid : 2048, EXP_TPB, 0, st>>>(d, qn, len, first, runs, levels, visited,
+    k_bfs_expand<<<expand_grid ? expand_grid : bfs_expand_grid(), EXP_TPB, 0, st>>>(d, qn, len, first, runs, levels, visited,
                                                                       level, qnext, counters);
     return hipGetLastError();
 }
