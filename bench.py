@@ -680,7 +680,7 @@ def main():
     ap.add_argument("--segment-slots", type=int, default=32)
     ap.add_argument("--seg-slots-per-lane", type=int, default=0, help="single-segment rebalance shape (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-sweep", type=int, default=1, help="also run batch sizes 1e3..1e6 (config 2)")
+    ap.add_argument("--batch-sweep", type=int, default=1, help="also run batch sizes 1e3..1e6 of the R-MAT 2^20 stream")
     ap.add_argument("--cpu-batches", type=int, default=15)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -749,7 +749,7 @@ def main():
                                                     "rebuild_ms", "pyramid_ms")}
     windows = {str(l): int(st.windows_per_level[l]) for l in range(32) if st.windows_per_level[l]}
 
-    # config 2 reports every batch size: the same stream cut into batches of
+    # the R-MAT 2^20 workload at every batch size: the same stream cut into batches of
     # 1e3 (a fixed window of 1000 batches), 1e4, 1e5 and 1e6, each from a
     # fresh bulk load, with the rebalance kernels' algorithmic GB/s
     sweep = []

@@ -243,11 +243,11 @@ def test_batch_sizes_cover_sort_paths(n):
     t.close()
 
 
-def test_config1_full_size():
-    """Config 1 at full size: V = 2^16, 10^6 distinct uniform edges (seed 1,
+def test_uniform_2p16_one_million_edges():
+    """Uniform graph at full size: V = 2^16, 10^6 distinct uniform edges (seed 1,
     no self loops), then 10 insert batches of 10^4 absent edges and 10 delete
     batches of 10^4 live edges, compared with the oracle after every batch."""
-    say("config 1, full size")
+    say("uniform 2^16 V, 10^6 edges, full size")
     V = 1 << 16
     t = Triple(V, model=False)
     t.insert(L.gen_uniform(V, 1_000_000, 1))
