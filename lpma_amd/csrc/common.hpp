@@ -242,15 +242,24 @@ __device__ __forceinline__ u64 lower_bound_dev(const T *a, u64 lo, u64 hi, T x) 
     return lo;
 }
 
-// route a key: last segment s with sep[s] <= key (sep[0] = 0, nseg = 2^H)
+// route a key: last segment s with sep[s] <= key (sep[0] = 0, nseg = 2^H).
+// A 4-ary search: each step loads three pivots at once (sep is sorted, so
+// the number of pivots <= key picks the quarter), which halves the chain of
+// dependent loads; an odd H ends with one binary step.
 __device__ __forceinline__ u32 locate_segment(const u64 *__restrict__ sep, u32 H, u64 key) {
     u32 base = 0;
-    for (u32 step = (1u << H) >> 1; step; step >>= 1)
-        if (sep[base + step] <= key) base += step;
+    u32 q = 1u << H;
+    for (; q >= 4; q >>= 2) {
+        const u32 r = q >> 2;
+        const u64 a = sep[base + r], b = sep[base + 2 * r], c = sep[base + 3 * r];
+        base += (a <= key ? r : 0u) + (b <= key ? r : 0u) + (c <= key ? r : 0u);
+    }
+    if (q == 2 && sep[base + 1] <= key) base += 1;
     return base;
 }
 
-// is `key` among the live (front-packed) keys of segment s
+// is `key` among the live (front-packed) keys of segment s (a binary search:
+// a 4-ary one here, three probes per step, measured slower)
 __device__ __forceinline__ bool segment_holds(const DevState &d, u32 s, u64 key) {
     const u64 *seg = d.keys + ((u64)s << d.log2S);
     u32 c = d.cnt[s];
