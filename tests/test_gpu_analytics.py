@@ -152,3 +152,28 @@ def test_pagerank_atomic_push_above_binned_limit():
     l1 = np.abs(got - want).sum()
     assert l1 < 1e-6, (l1, iters, delta)
     pma.close()
+
+
+@pytest.mark.parametrize("slots", [4, 64, 256])
+def test_bfs_and_pagerank_other_segment_sizes(slots):
+    """The expansion and the bin passes read live lines through the segment
+    counts, whose geometry follows the segment size."""
+    say(f"analytics segment_slots={slots}")
+    V = 1 << 14
+    keys = L.shuffle(L.gen_rmat(14, 150000, seed=21), 22)
+    pma = L.PMA(V, segment_slots=slots)
+    pma.insert(keys[:100000])
+    pma.insert(keys[100000:])
+    ref = oracle.sort_unique(keys)
+    rng = np.random.default_rng(23)
+    d = ref[rng.choice(ref.size, 20000, replace=False)]
+    pma.delete(d)
+    ref = oracle.apply(ref, (), d)
+    for src in (0, 3):
+        lv, depth, scanned = pma.bfs(src)
+        assert np.array_equal(lv.cpu().numpy().view(np.uint32), oracle.bfs(ref, V, src))
+    pr, iters, delta = pma.pagerank(0.85, 1e-6, 50)
+    want, _ = oracle.pagerank(ref, V, 0.85, iters)
+    l1 = np.abs(pr.cpu().numpy().astype(np.float64) - want).sum()
+    assert l1 < 1e-6, (l1, iters, delta)
+    pma.close()
