@@ -686,9 +686,8 @@ def main():
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the store is "
-              "sharded by key range across GPUs only by design (no multi-GPU run is part of this build).",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: sharding the store by key range across GPUs is designed but not "
+              "built, so --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "served":
         return run_served(args)
