@@ -45,10 +45,13 @@ hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st);
-hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st);
+// with events (timing): the launch records its own start and end in them
+hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
 // grid sized for at most `bound` windows, the count read from the status on
 // the device (launches queued before the host knows it)
-hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st);
+hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0 = nullptr,
+                               hipEvent_t e1 = nullptr);
 u32 seg_lanes_log2(u32 log2S, int slots_log2);
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st);
 u32 pyramid_blocks(u32 H);

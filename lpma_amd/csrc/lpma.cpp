@@ -411,6 +411,24 @@ struct PhaseTimer {
     }
 };
 
+// Events that a launch records itself (hipExtLaunchKernelGGL): the phase
+// time is then the kernel's execution, without the ~1.7 us that a pair of
+// separate event records adds around it.  Null events when not profiling.
+struct LaunchTimer {
+    Pipeline &pl;
+    int phase;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(Pipeline &pl_, int p) : pl(pl_), phase(p) {
+        if (pl.profiling) {
+            a = take_event(pl);
+            b = take_event(pl);
+        }
+    }
+    ~LaunchTimer() {
+        if (a) pl.events.push_back({a, b, phase});
+    }
+};
+
 // Density table, rpma.cuh:136-168 (see lpma_config_t)
 void compute_density(lpma *h) {
     const lpma_config_t &c = h->cfg;
@@ -669,8 +687,8 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
 
     if (!windows_done) {
         if (s.nwork[CLASS_SEG] && !early) {
-            PhaseTimer t(pl, st, PH_SEG);
-            HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st));
+            LaunchTimer t(pl, PH_SEG);
+            HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st, t.a, t.b));
             pl.launches_seg++;
         }
         if (s.nwork[CLASS_WAVE] && !early) {
@@ -869,8 +887,8 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         if (!pl.ev_status) HIPCHECK(hipEventCreateWithFlags(&pl.ev_status, hipEventDisableTiming));
         HIPCHECK(hipEventRecord(pl.ev_status, st));
         {
-            PhaseTimer t(pl, st, PH_SEG);
-            HIPCHECK(rebalance_seg_bound(g, bb, n, st));
+            LaunchTimer t(pl, PH_SEG);
+            HIPCHECK(rebalance_seg_bound(g, bb, n, st, t.a, t.b));
             pl.launches_seg++;
         }
         {

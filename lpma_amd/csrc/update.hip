@@ -22,6 +22,8 @@
 // Every kernel works on a Group: one or many independent PMAs whose batch
 // elements carry their PMA id, so a round over many PMAs costs one launch
 // sequence.  A run (the updates of one segment) is keyed pid << 32 | seg.
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -839,9 +841,23 @@ u32 seg_lanes_log2(u32 log2S, int slots_log2) {
     return (u32)g;
 }
 
+// one single-segment launch; with events, the dispatch itself records its
+// start and end in them (the kernel's execution time, no separate packets)
+template <int SPL>
+static void seg_dispatch(u32 grid, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1, const Group &g,
+                         const BatchBufs &b, u32 log2G, u32 nw) {
+    if (e0)
+        hipExtLaunchKernelGGL(k_rebalance_seg<SPL>, dim3(grid), dim3(SEG_TPB), (u32)lds, st, e0, e1, 0u, g,
+                              (const Work *)b.work[CLASS_SEG], (const Status *)b.status, (const u64 *)b.ckey,
+                              (const i32 *)b.pd, log2G, nw);
+    else
+        k_rebalance_seg<SPL><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw);
+}
+
 // `windows`: the exact window count (eager rounds) or a bound that sizes the
 // grid, the kernel then reading the count from the status (graph replays)
-static hipError_t rebalance_seg_launch(const Group &g, const BatchBufs &b, u64 windows, bool exact, hipStream_t st) {
+static hipError_t rebalance_seg_launch(const Group &g, const BatchBufs &b, u64 windows, bool exact, hipStream_t st,
+                                       hipEvent_t e0, hipEvent_t e1) {
     u32 S = 1u << g.log2S;
     u32 log2G = g.seg_log2G;
     u32 groups_per_block = SEG_TPB >> log2G;
@@ -850,22 +866,24 @@ static hipError_t rebalance_seg_launch(const Group &g, const BatchBufs &b, u64 w
     u32 grid = (u32)(blocks < 1 ? 1 : (blocks > SEG_GRID_MAX ? SEG_GRID_MAX : blocks));
     u32 nw = exact ? (u32)windows : ~0u;
     switch (S >> log2G) {
-        case 1: k_rebalance_seg<1><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
-        case 2: k_rebalance_seg<2><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
-        case 4: k_rebalance_seg<4><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
-        case 8: k_rebalance_seg<8><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
-        case 16: k_rebalance_seg<16><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw); break;
+        case 1: seg_dispatch<1>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
+        case 2: seg_dispatch<2>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
+        case 4: seg_dispatch<4>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
+        case 8: seg_dispatch<8>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
+        case 16: seg_dispatch<16>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st) {
-    return rebalance_seg_launch(g, b, windows, true, st);
+hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st, hipEvent_t e0,
+                         hipEvent_t e1) {
+    return rebalance_seg_launch(g, b, windows, true, st, e0, e1);
 }
 
-hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st) {
-    return rebalance_seg_launch(g, b, bound, false, st);
+hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0,
+                               hipEvent_t e1) {
+    return rebalance_seg_launch(g, b, bound, false, st, e0, e1);
 }
 
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
@@ -881,7 +899,7 @@ hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, 
 // runs_cap bounds them.  merged must hold the live keys of every PMA of
 // the group.
 hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st) {
-    (void)rebalance_seg_launch(g, b, runs_cap < 65536u ? runs_cap : 65536u, false, st);
+    (void)rebalance_seg_launch(g, b, runs_cap < 65536u ? runs_cap : 65536u, false, st, nullptr, nullptr);
     const u32 gw = runs_cap < 4096 ? (runs_cap ? runs_cap : 1) : 4096;
     k_rebalance_wave<<<gw, 64, 0, st>>>(g, b.work[CLASS_WAVE], b.status, b.ckey, b.pd);
     const u32 gb = runs_cap < 1024 ? (runs_cap ? runs_cap : 1) : 1024;
