@@ -692,13 +692,13 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
             pl.launches_seg++;
         }
         if (s.nwork[CLASS_WAVE] && !early) {
-            PhaseTimer t(pl, st, PH_WAVE);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st));
+            LaunchTimer t(pl, PH_WAVE);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st, t.a, t.b));
             pl.launches_wave++;
         }
         if (s.nwork[CLASS_BLOCK]) {
-            PhaseTimer t(pl, st, PH_BLOCK);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st));
+            LaunchTimer t(pl, PH_BLOCK);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st, t.a, t.b));
             pl.launches_block++;
         }
         if (s.nwork[CLASS_LARGE]) {
@@ -892,8 +892,8 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
             pl.launches_seg++;
         }
         {
-            PhaseTimer t(pl, st, PH_WAVE);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, early_wave_grid(pl, n), st));
+            LaunchTimer t(pl, PH_WAVE);
+            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, early_wave_grid(pl, n), st, t.a, t.b));
             pl.launches_wave++;
         }
         HIPCHECK(hipEventSynchronize(pl.ev_status));

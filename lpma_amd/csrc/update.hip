@@ -886,11 +886,22 @@ hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hi
     return rebalance_seg_launch(g, b, bound, false, st, e0, e1);
 }
 
-hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st) {
-    if (cls == CLASS_WAVE)
-        k_rebalance_wave<<<grid, 64, 0, st>>>(g, b.work[CLASS_WAVE], b.status, b.ckey, b.pd);
-    else
-        k_rebalance_block<<<grid, 256, 0, st>>>(g, b.work[CLASS_BLOCK], b.status, b.ckey, b.pd);
+hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st, hipEvent_t e0,
+                         hipEvent_t e1) {
+    const Work *w = b.work[cls == CLASS_WAVE ? CLASS_WAVE : CLASS_BLOCK];
+    if (cls == CLASS_WAVE) {
+        if (e0)
+            hipExtLaunchKernelGGL(k_rebalance_wave, dim3(grid), dim3(64), 0u, st, e0, e1, 0u, g, w,
+                                  (const Status *)b.status, (const u64 *)b.ckey, (const i32 *)b.pd);
+        else
+            k_rebalance_wave<<<grid, 64, 0, st>>>(g, w, b.status, b.ckey, b.pd);
+    } else {
+        if (e0)
+            hipExtLaunchKernelGGL(k_rebalance_block, dim3(grid), dim3(256), 0u, st, e0, e1, 0u, g, w,
+                                  (const Status *)b.status, (const u64 *)b.ckey, (const i32 *)b.pd);
+        else
+            k_rebalance_block<<<grid, 256, 0, st>>>(g, w, b.status, b.ckey, b.pd);
+    }
     return hipGetLastError();
 }
 
