@@ -98,6 +98,21 @@ __device__ __forceinline__ void bfs_flush(u32 *s_q, u32 &s_n, u32 &s_base, u32 *
     __syncthreads();
 }
 
+// entries of a run's stage each thread loads (and later writes to LDS)
+static constexpr u32 EXP_SK = (EXP_STAGE + EXP_TPB - 1) / EXP_TPB;
+
+// the stage of the run starting at entry e0, into registers
+__device__ __forceinline__ void bfs_stage_load(const u32 *__restrict__ off, const u64 *__restrict__ first, u32 qn,
+                                               u32 e0, u32 (&p_off)[EXP_SK], u64 (&p_first)[EXP_SK]) {
+#pragma unroll
+    for (u32 k = 0; k < EXP_SK; ++k) {
+        const u32 kk = threadIdx.x + k * EXP_TPB;
+        const u64 idx = (u64)e0 + kk;
+        p_off[k] = (kk < EXP_STAGE && idx <= qn) ? off[idx] : 0xFFFFFFFFu;
+        p_first[k] = (kk < EXP_STAGE && idx < qn) ? first[idx] : 0ull;
+    }
+}
+
 __global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, u32 qn, const u32 *__restrict__ off,
                                                          const u64 *__restrict__ first,
                                                          const u32 *__restrict__ runs, u32 *__restrict__ levels,
@@ -114,15 +129,29 @@ __global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, u32 qn, cons
     const u32 Smask = (1u << d.log2S) - 1;
     const gu64 *keys = (const gu64 *)d.keys;
     u32 scanned = 0;
+    // a run's entry and stage are loaded one run ahead, into registers, and
+    // written to LDS once the previous run is done with the stage
+    u32 p_off[EXP_SK];
+    u64 p_first[EXP_SK];
+    u32 e0n = 0;
+    if (blockIdx.x < nruns) {
+        e0n = runs[blockIdx.x];
+        bfs_stage_load(off, first, qn, e0n, p_off, p_first);
+    }
     for (u64 run = blockIdx.x; run < nruns; run += gridDim.x) {
         const u64 t0 = run * EXP_RUN;
-        const u32 e0 = runs[run];
-        for (u32 k = threadIdx.x; k < EXP_STAGE; k += EXP_TPB) {
-            const u64 idx = (u64)e0 + k;
-            s_off[k] = idx <= qn ? off[idx] : 0xFFFFFFFFu;
-            s_first[k] = idx < qn ? first[idx] : 0ull;
+        const u32 e0 = e0n;
+#pragma unroll
+        for (u32 k = 0; k < EXP_SK; ++k) {
+            const u32 kk = threadIdx.x + k * EXP_TPB;
+            if (kk < EXP_STAGE) {
+                s_off[kk] = p_off[k];
+                s_first[kk] = p_first[k];
+            }
         }
         __syncthreads();
+        const bool more = run + gridDim.x < nruns;  // uniform
+        if (more) e0n = runs[run + gridDim.x];
         // slot of each item: the staged entry holding it, or the global search
         // past the stage (only when empty rows fill it)
         u64 slot[EXP_ITEMS];
@@ -146,6 +175,8 @@ __global__ __launch_bounds__(EXP_TPB) void k_bfs_expand(DevState d, u32 qn, cons
                 slot[i] = first[e] + (t - off[e]);
             }
         }
+        // the next run's stage: its entry has arrived during the searches
+        if (more) bfs_stage_load(off, first, qn, e0n, p_off, p_first);
         u32 live[EXP_ITEMS];
 #pragma unroll
         for (u32 i = 0; i < EXP_ITEMS; ++i) live[i] = d.cnt[slot[i] >> d.log2S];  // slot 0 when !ok: in bounds
