@@ -79,11 +79,15 @@ def step_list(ins_batches, del_batches, n):
     return out
 
 
-def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_lane=0, graph=False):
+def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_lane=0, graph=False, density=None):
     import torch
     import lpma_amd as L
     stream = torch.cuda.current_stream()
-    pma = L.PMA(V, segment_slots=seg)
+    if density is None:
+        pma = L.PMA(V, segment_slots=seg)
+    else:  # (lower leaf, lower root, upper leaf, upper root)
+        pma = L.PMA(V, segment_slots=seg, density_lower=density[1], density_upper=density[3],
+                    density_lower_leaf=density[0], density_upper_leaf=density[2])
     pma.set_launch(slots_per_lane)
     t = time.time()
     pma.bulk_load(L.to_device(bulk))
@@ -671,6 +675,9 @@ def main():
     ap.add_argument("--graph", type=int, default=0,
                     help="replay rounds as hipGraphs (headline: the timed pass; --forms batched: the rounds)")
     ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--density", default="",
+                    help="headline: density bounds 'lower_leaf,lower_root,upper_leaf,upper_root' "
+                         "(default: the reference's 0.42 / 1.0 at every level)")
     ap.add_argument("--tune", type=int, default=1, help="batched workload: autotune segment size / density / shape")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
@@ -713,11 +720,12 @@ def main():
     seq = step_list(ins_b, del_b, total)
 
     # pass 1: throughput and per-batch latency, no profiling events
+    dens = tuple(float(x) for x in args.density.split(",")) if args.density else None
     r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False,
-                slots_per_lane=args.seg_slots_per_lane, graph=bool(args.graph))
+                slots_per_lane=args.seg_slots_per_lane, graph=bool(args.graph), density=dens)
     # pass 2: same stream with per-launch HIP events for the kernel roofline
     rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True,
-                 slots_per_lane=args.seg_slots_per_lane)
+                 slots_per_lane=args.seg_slots_per_lane, density=dens)
 
     if args.copy_calib:
         src = torch.empty(args.copy_calib << 20, dtype=torch.uint8, device=dev)
@@ -817,7 +825,9 @@ def main():
         "data": "synthetic (seeded R-MAT generator)",
         "config": {"workload": f"rmat{args.scale}_{args.edges}E_half_bulk_stream_insert_then_delete_batch{B}",
                    "vertices": V, "edges": args.edges, "batch": B, "segment_slots": args.segment_slots,
-                   "density": "lower 0.42 / upper 1.0 (reference)", "parallelism": "single GPU"},
+                   "density": ("lower 0.42 / upper 1.0 (reference)" if not args.density else
+                               "lower leaf,root / upper leaf,root = " + args.density),
+                   "parallelism": "single GPU"},
         "p50_batch_ms": float(np.percentile(lat, 50)),
         "p99_batch_ms": float(np.percentile(lat, 99)),
         "mean_batch_ms": float(lat.mean()),
