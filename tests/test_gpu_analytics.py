@@ -177,3 +177,26 @@ def test_bfs_and_pagerank_other_segment_sizes(slots):
     l1 = np.abs(pr.cpu().numpy().astype(np.float64) - want).sum()
     assert l1 < 1e-6, (l1, iters, delta)
     pma.close()
+
+
+@pytest.mark.parametrize("V", [1, 2, 3])
+def test_tiny_and_empty_graphs(V):
+    """No edges: every vertex dangles, so PageRank is uniform and BFS reaches
+    only its root; then one edge 0 -> 1 where there is a vertex 1."""
+    say(f"analytics tiny V={V}")
+    from gpu_util import keys_of
+    pma = L.PMA(V)
+    for keys in ([], [(0, 1)] if V > 1 else []):
+        if keys:
+            pma.insert(keys_of([k[0] for k in keys], [k[1] for k in keys]))
+        ref = oracle.sort_unique(keys_of([k[0] for k in keys], [k[1] for k in keys])) if keys else \
+            np.empty(0, dtype=np.uint64)
+        lv, depth, scanned = pma.bfs(0)
+        want = oracle.bfs(ref, V, 0)
+        assert np.array_equal(lv.cpu().numpy().view(np.uint32), want)
+        assert scanned == len(keys)
+        pr, iters, delta = pma.pagerank(0.85, 1e-6, 50)
+        got = pr.cpu().numpy().astype(np.float64)
+        exp, _ = oracle.pagerank(ref, V, 0.85, iters)
+        assert np.abs(got - exp).sum() < 1e-6, (got, exp)
+    pma.close()
