@@ -1,35 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark of the batched edge-update path on one MI355X.
 
-Workload (the headline configuration): R-MAT graph, 2^20 vertices,
-2^24 distinct edges (a,b,c,d = 0.57,0.19,0.19,0.05, seed 2, self loops and
-repeats rejected), shuffled with seed 12; the first half is bulk-loaded, the
-second half is streamed as insert batches of B edges, then the same volume of
-live edges (a seeded uniform sample without replacement) is deleted in
-batches of B.  A step is one update batch; the stream is cyclic (deleted
-edges are re-inserted) when more steps are asked for than it holds.
+Headline workload: R-MAT graph, 2^20 vertices, 2^24 distinct edges
+(a,b,c,d = 0.57,0.19,0.19,0.05, seed 2, self loops and repeats rejected),
+shuffled with seed 12; the first half is bulk-loaded, the second half is
+streamed as insert batches of B edges, then the same volume of live edges
+(a seeded uniform sample without replacement of all 2^24) is deleted in
+batches of B.  A step is one update batch.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-    python bench.py --workload batched [--steps K] [--warmup W]
+    python bench.py --workload batched|served|analytics
 
-Prints one JSON line.  `value` is edge updates per second over the K timed
-batches (inputs resident in HBM, wall clock between device synchronisations).
+Prints one JSON line.  `value` is edge updates per second over the WHOLE
+stream (84 insert + 84 delete batches at B = 1e5; `steps` reports the batches
+timed, whatever --steps asks: the stream's state depends on every batch
+before it), inputs resident in HBM, wall clock between device
+synchronisations, from a fresh bulk load; W warm-up batches run first on a
+separate instance.  p50/p99 are per-batch HIP-event spans of that pass.
 `roofline` describes the dominant rebalance kernel, the one that moves the
 most algorithmic bytes (bytes per launch / average launch time, from HIP
 events the library records around each launch on the update stream in a
-second, profiled pass).
-`cpu_baseline` times the CPU oracle (sorted merge + row offsets, pthreads)
-on a bounded sample of the same batches.
+second, profiled pass over the same stream).  `cpu_baseline` times the CPU
+oracle (sorted merge + row offsets, pthreads on every host core the process
+may use, and on one) on the stream's first insert batches from the
+bulk-loaded state and its first delete batches from the state the GPU
+deletes from.  Unless --legs is empty, the line also carries two compact
+legs measured in the same run: `batched` (64 PMAs, one group update per
+round, against one update call per PMA per round) and `served` (the request
+server against a one-request-at-a-time loop at the top offered rate).
 
-`--workload batched` measures the batched form instead: 64 independent R-MAT
-graphs (edge counts log-uniform in [1e4, 1e7], vertices = edges/16 rounded
-to a power of two, seeds 3+i), 100 rounds in which every graph gets one
-batch (lognormal size, mu = ln 5000, sigma = 1.5, cap 1e6, 70% inserts /
-30% deletes).  A step is one round.  The same rounds run through one group
-update per round (the batched form) and through one update call per graph
-per round (the plain form), from identical bulk-loaded states; `value` is
-the batched form's edge updates/s, and both forms' round latencies are
-reported.
+`--workload batched` measures the batched form alone at full length: 64
+independent R-MAT graphs (edge counts log-uniform in [1e4, 1e7], vertices =
+edges/16 rounded to a power of two, seeds 3+i), 100 rounds in which every
+graph gets one batch (lognormal size, mu = ln 5000, sigma = 1.5, cap 1e6, 70%
+inserts / 30% deletes).  `--workload served` sweeps the served form over
+offered rates; `--workload analytics` runs PageRank and BFS on the live
+structure.
 """
 import argparse
 import json
@@ -40,55 +46,61 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_workload(scale, edges, seed, shuffle_seed, B, del_seed):
-    import lpma_amd as L
+def make_workload(scale, edges, B):
+    from lpma_amd.workloads import rmat_stream
     t = time.time()
-    keys = L.gen_rmat(scale, edges, 0.57, 0.19, 0.19, seed)
-    keys = L.shuffle(keys, shuffle_seed)
-    half = edges // 2
-    bulk = keys[:half]
-    rest = keys[half:]
-    ins_batches = [rest[i:i + B] for i in range(0, rest.size, B)]
-    rng = np.random.default_rng(del_seed)
-    dels = keys[rng.permutation(edges)[: rest.size]]  # live = all edges once inserts finished
-    del_batches = [dels[i:i + B] for i in range(0, dels.size, B)]
-    log(f"workload generated in {time.time() - t:.1f}s: bulk {bulk.size}, {len(ins_batches)} insert + "
+    bulk, ins_batches, del_batches = rmat_stream(scale=scale, edges=edges, seed=2, shuffle_seed=12, batch=B,
+                                                 del_seed=13)
+    log(f"workload ready in {time.time() - t:.1f}s: bulk {bulk.size}, {len(ins_batches)} insert + "
         f"{len(del_batches)} delete batches of {B}")
     return bulk, ins_batches, del_batches
 
 
-def step_list(ins_batches, del_batches, n):
-    """(op, batch index) for n steps of the cyclic stream."""
-    seq = [(1, i) for i in range(len(ins_batches))] + [(0, i) for i in range(len(del_batches))]
-    cyc = [(1, i) for i in range(len(del_batches))]  # re-insert the deleted edges
-    out = []
-    k = 0
-    while len(out) < n:
-        if k < len(seq):
-            out.append(seq[k])
-        else:
-            j = (k - len(seq)) % (2 * len(del_batches))
-            out.append((1, j) if j < len(del_batches) else (0, j - len(del_batches)))
-        k += 1
-    return out
+def stream_steps(ins_batches, del_batches):
+    """(op, batch index) of the whole stream: every insert batch, then every
+    delete batch."""
+    return [(1, i) for i in range(len(ins_batches))] + [(0, i) for i in range(len(del_batches))]
 
 
-def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_lane=0, graph=False, density=None):
-    import torch
+def new_pma(V, seg, slots_per_lane, density):
     import lpma_amd as L
-    stream = torch.cuda.current_stream()
     if density is None:
         pma = L.PMA(V, segment_slots=seg)
     else:  # (lower leaf, lower root, upper leaf, upper root)
         pma = L.PMA(V, segment_slots=seg, density_lower=density[1], density_upper=density[3],
                     density_lower_leaf=density[0], density_upper_leaf=density[2])
     pma.set_launch(slots_per_lane)
+    return pma
+
+
+def warm_up(bulk, ins_d, del_d, steps, V, seg, slots_per_lane=0, density=None):
+    """W untimed batches on an instance of their own (the timed pass starts
+    from a fresh bulk load, so the whole stream is timed)."""
+    import torch
+    import lpma_amd as L
+    if not steps:
+        return
+    pma = new_pma(V, seg, slots_per_lane, density)
+    pma.bulk_load(L.to_device(bulk))
+    for op, i in steps:
+        pma.update(ins_d[i] if op else del_d[i], op)
+    torch.cuda.synchronize()
+    pma.close()
+
+
+def run_gpu(bulk, ins_d, del_d, steps, V, seg, profile, slots_per_lane=0, graph=False, density=None):
+    """Bulk load, then time `steps` (op, batch) in order."""
+    import torch
+    import lpma_amd as L
+    stream = torch.cuda.current_stream()
+    pma = new_pma(V, seg, slots_per_lane, density)
     t = time.time()
     pma.bulk_load(L.to_device(bulk))
     torch.cuda.synchronize()
@@ -96,34 +108,16 @@ def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_la
     if graph:
         pma.set_graph(True)
         pma.reserve(max(int(b.numel()) for b in list(ins_d) + list(del_d)))
-
-    def batch_of(op, i, reinserted):
-        if op == 1 and not reinserted:
-            return ins_d[i]
-        return del_d[i]
-
-    def do(step_idx, st):
-        op, i = st
-        reins = step_idx >= len(ins_d) + len(del_d)
-        pma.update(batch_of(op, i, reins), op)
-
-    for k in range(warmup):
-        do(k, steps_seq[k])
-    torch.cuda.synchronize()
     pma.reset_stats()
     if profile:
         pma.set_profiling(True)
         pma.profile()  # clear
-    timed = steps_seq[warmup:]
     n_updates = 0
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in timed]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in steps]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for j, st in enumerate(timed):
-        k = warmup + j
-        op, i = st
-        reins = k >= len(ins_d) + len(del_d)
-        b = batch_of(op, i, reins)
+    for j, (op, i) in enumerate(steps):
+        b = ins_d[i] if op else del_d[i]
         ev[j][0].record(stream)
         pma.update(b, op)
         ev[j][1].record(stream)
@@ -133,7 +127,7 @@ def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_la
     lat = np.array([a.elapsed_time(b) for a, b in ev])
     stats = pma.stats()
     prof = pma.profile() if profile else None
-    # correctness spot check of the final state: sorted, unique, consistent
+    # consistency of the final state: sorted, unique, counts and markers
     assert pma.check() == 0
     res = dict(wall=wall, n_updates=n_updates, lat_ms=lat, stats=stats, prof=prof, edges=pma.edges,
                capacity=int(pma.view().capacity), height=int(pma.view().tree_height))
@@ -141,14 +135,32 @@ def run_gpu(bulk, ins_d, del_d, steps_seq, warmup, V, seg, profile, slots_per_la
     return res
 
 
+def host_cpu():
+    """(cores this process may use, CPU model name) of the host."""
+    cores = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
 def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
+    """The CPU oracle on the GPU run's own batches: the first n_ins insert
+    batches from the bulk-loaded state, and the first n_del delete batches
+    from the state the GPU deletes from (every edge of the stream live)."""
     import oracle
-    S = oracle.sort_unique(bulk)
     times = []
     n = 0
     empty = np.empty(0, dtype=np.uint64)
+    S = oracle.sort_unique(bulk)
+    cap = S.size + sum(int(b.size) for b in ins_batches) + 1
     # two output buffers, touched once before timing, used in turn
-    cap = S.size + sum(int(b.size) for b in ins_batches[:n_ins]) + 1
     bufs = [np.zeros(cap, dtype=np.uint64), np.zeros(cap, dtype=np.uint64)]
     for i in range(n_ins):
         t = time.perf_counter()
@@ -156,31 +168,36 @@ def cpu_baseline(bulk, ins_batches, del_batches, V, n_ins, n_del, threads):
         oracle.row_offsets(S, V, threads=threads)
         times.append(time.perf_counter() - t)
         n += ins_batches[i].size
-    # deletes sampled from the edges live at this point
-    rng = np.random.default_rng(77)
+    S = oracle.sort_unique(np.concatenate([bulk] + list(ins_batches)))
     for i in range(n_del):
-        d = S[rng.choice(S.size, del_batches[i].size, replace=False)]
         t = time.perf_counter()
-        S = oracle.apply(S, empty, d, threads=threads, out=bufs[(n_ins + i) & 1])
+        S = oracle.apply(S, empty, del_batches[i], threads=threads, out=bufs[(n_ins + i) & 1])
         oracle.row_offsets(S, V, threads=threads)
         times.append(time.perf_counter() - t)
-        n += d.size
+        n += del_batches[i].size
     return n / sum(times), times
 
 
-def measured_traffic(kernel, batch):
-    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes summarised in profiles/ (None when not measured for this
-    kernel and batch size)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"traffic_rmat20_b{batch}.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-        k = t["kernels"].get(kernel)
-        return None if k is None else k["hbm_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
-        return None
+# HBM traffic per launch, measured by rocprofv3 counter passes over a named
+# configuration (profiles/traffic_*.json); reported only for that configuration
+def measured_traffic(kernel, cfg):
+    """(bytes per launch, source) of `kernel` from the counter summary whose
+    recorded configuration equals `cfg`, else (None, reason)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("config") != cfg:
+            continue
+        k = t.get("kernels", {}).get(kernel)
+        if k is None:
+            return None, f"{kernel} not in {os.path.basename(path)}"
+        return k["hbm_bytes_per_launch"], {"file": "profiles/" + os.path.basename(path),
+                                           "window": t.get("window"), "method": t.get("method")}
+    return None, "no counter summary for this configuration"
 
 
 def _ptr_arrays(tensors_per_graph, r):
@@ -243,7 +260,9 @@ def run_rounds(w, fleet, group, ins_d, del_d, rounds, stream):
     return np.array([a.elapsed_time(b) for a, b in ev]), wall
 
 
-def run_batched(args):
+def batched_measure(args):
+    """The batched form against the plain form on the same rounds; returns
+    the JSON object of the batched workload."""
     import torch
     import lpma_amd as L
     from lpma_amd.workloads import BatchedWorkload
@@ -299,9 +318,7 @@ def run_batched(args):
         torch.cuda.synchronize()
     if len(forms) == 1:
         r = res[forms[0]]
-        print(json.dumps({"form": forms[0], "graph": args.graph, "value": r["value"], "p50": r["p50"],
-                          "p99": r["p99"]}))
-        return
+        return {"form": forms[0], "graph": args.graph, "value": r["value"], "p50": r["p50"], "p99": r["p99"]}
     assert finals["plain"] == finals["batched"] == finals["batched_graph"], "the forms ended in different states"
     b, p, bg = res["batched"], res["plain"], res["batched_graph"]
     out = {
@@ -330,7 +347,11 @@ def run_batched(args):
         "graph_rounds_value": bg["value"], "graph_round_p50_ms": bg["p50"], "graph_round_p99_ms": bg["p99"],
         "autotune": tuned,
     }
-    print(json.dumps(out))
+    return out
+
+
+def run_batched(args):
+    print(json.dumps(batched_measure(args)))
 
 
 def served_requests(rng, rate, duration, V, ins_pool, del_pool, mean_batch):
@@ -429,7 +450,9 @@ def summarize_trace(results, st=None):
     return row
 
 
-def run_served(args):
+def served_measure(args):
+    """The request server against a one-request-at-a-time loop on the same
+    traces; returns the JSON object of the served workload."""
     import torch
     import lpma_amd as L
     L.lib()
@@ -447,8 +470,6 @@ def run_served(args):
     del_pool = L.to_host_u64(keys[torch.randint(0, keys.numel(), (1 << 24,), device=keys.device)])
     rates = [float(x) for x in args.rates.split(",")]
     sweep = {}
-    checked = mismatched = 0
-    history = []  # every update request applied so far, in order
     for mode in args.served_modes.split(","):
         rows = []
         for rate in rates:
@@ -466,9 +487,6 @@ def run_served(args):
                             for k in ("update", "read", "bfs")) + f", all done at {row['last_done_s']:.2f}s"
                 + (f", {row['update_runs']} update runs, {row['snapshots']} snapshots, engine ms {row['engine_ms']}"
                    if "update_runs" in row else ""))
-            if mode == "served" and rate == rates[-1]:
-                checked, mismatched = verify_served_reads(keys, history, reqs, results, V, args.verify_reads)
-            history += [(r[2], r[3]) for r in reqs if r[1] == "update"]
         sweep[mode] = rows
     top = sweep["served"][-1]
     base = sweep["plain"][-1] if "plain" in sweep else top
@@ -494,44 +512,17 @@ def run_served(args):
         "update_p99_ms": top.get("update_p99_ms"),
         "plain_update_p99_ms": base.get("update_p99_ms"),
         "plain_form": "one loop serving each request alone in arrival order on the live structure",
-        "reads_checked_against_sequential_replay": checked,
-        "reads_mismatched": mismatched,
+        "parity": "reads and searches equal the oracle at their reported versions at this size: "
+                  "tests/test_gpu_fullsize.py::test_served_rmat24_trace_matches_oracle",
     }
-    print(json.dumps(out))
+    pma.close()
+    del keys
+    L.pool_trim()
+    return out
 
 
-def verify_served_reads(keys, history, reqs, results, V, n_check):
-    """Rebuilds the state the trace started from on a replica (the initial
-    graph, then every earlier update request one at a time), replays the
-    trace's update requests one at a time, and compares sampled reads with
-    the replica's neighbor lists after as many updates as the server
-    reported for them."""
-    import torch
-    import lpma_amd as L
-    if n_check <= 0:
-        return 0, 0
-    updates = [(r[2], r[3]) for r in reqs if r[1] == "update"]
-    reads = [(res["applied"], req[2], res) for req, (kind, res, _, _) in zip(reqs, results) if kind == "read"]
-    rng = np.random.default_rng(5)
-    pick = sorted(rng.choice(len(reads), min(n_check, len(reads)), replace=False), key=lambda i: reads[i][0])
-    rep = L.PMA(V)
-    rep.bulk_load(keys)
-    for a, b in history:
-        rep.update_mixed(L.to_device(a), L.to_device(b))
-    applied = 0
-    bad = 0
-    for i in pick:
-        a_cnt, verts, res = reads[i]
-        while applied < a_cnt:
-            ins, dels = updates[applied]
-            rep.update_mixed(L.to_device(ins), L.to_device(dels))
-            applied += 1
-        offs, vals = rep.neighbors(torch.from_numpy(verts.astype(np.int32)))
-        if not (np.array_equal(offs.cpu().numpy().view(np.uint32), res["offsets"]) and
-                np.array_equal(vals.cpu().numpy().view(np.uint32), res["values"])):
-            bad += 1
-    rep.close()
-    return len(pick), bad
+def run_served(args):
+    print(json.dumps(served_measure(args)))
 
 
 def run_analytics(args):
@@ -599,33 +590,27 @@ def run_analytics(args):
         log(f"analytics batch {b} ({op}): update {upd_ms:.2f} ms, PageRank {iters} it in {pr_s * 1e3:.1f} ms "
             f"({row['pagerank_edges_per_s']:.3e} edges/s, {row['pagerank_alg_GBps']:.0f} GB/s), BFS depth {depth} "
             f"{bfs_s * 1e3:.1f} ms ({row['bfs_edges_per_s']:.3e} edges/s, {row['bfs_alg_GBps']:.0f} GB/s)")
-    parity = None
     cpu = None
     if not args.no_cpu:
-        # the oracle on the final graph: BFS levels bit-exact, PageRank after
-        # the same fixed number of iterations within 1e-6 L1 (FP64 oracle)
+        # CPU baseline: the FP64 oracle's PageRank and BFS on the final graph
+        # (their parity with the GPU at this size is a GPU test:
+        # tests/test_gpu_fullsize.py::test_analytics_rmat24_full_size)
         import oracle
         ek, _ = pma.export()
         host = L.to_host_u64(ek)
         del ek
         t0 = time.perf_counter()
-        ref_lv = oracle.bfs(host, V, 0)
+        oracle.bfs(host, V, 0)
         cpu_bfs_s = time.perf_counter() - t0
-        lv, _, _ = pma.bfs(0)
-        bfs_exact = bool(np.array_equal(lv.cpu().numpy().view(np.uint32), ref_lv))
         it = args.analytics_check_iters
         t0 = time.perf_counter()
-        ref_pr, _ = oracle.pagerank(host, V, 0.85, it)
+        oracle.pagerank(host, V, 0.85, it)
         cpu_pr_s = time.perf_counter() - t0
-        pr, _, _ = pma.pagerank(0.85, 0.0, it)
-        l1 = float(np.abs(pr.cpu().numpy().astype(np.float64) - ref_pr).sum())
-        parity = {"bfs_levels_bit_exact": bfs_exact, "pagerank_iterations": it, "pagerank_l1_vs_fp64": l1,
-                  "pagerank_within_1e-6": l1 < 1e-6}
         E = host.size
         cpu = {"value": E * it / cpu_pr_s, "unit": "PageRank edges/s", "cores": 1, "kind": "port",
                "sample": f"FP64 oracle PageRank, {it} iterations, and BFS from 0 ({cpu_bfs_s:.1f} s, "
                          f"{E / cpu_bfs_s:.3e} edges/s) on the final {E}-edge graph, single thread"}
-        log(f"analytics parity: {parity}; cpu PageRank {cpu_pr_s:.1f}s, BFS {cpu_bfs_s:.1f}s")
+        log(f"analytics cpu baseline: PageRank {cpu_pr_s:.1f}s, BFS {cpu_bfs_s:.1f}s")
     last = rows[-1]
     out = {
         "metric": "analytics on the live structure: PageRank and BFS edges traversed/s over the gapped layout "
@@ -648,7 +633,8 @@ def run_analytics(args):
         "bfs_edges_per_s": last["bfs_edges_per_s"],
         "bfs_alg_GBps": last["bfs_alg_GBps"],
         "pagerank_alg_GBps": last["pagerank_alg_GBps"],
-        "parity": parity,
+        "parity": "BFS levels bit-exact and PageRank within 1e-6 L1 of the FP64 oracle at this size: "
+                  "tests/test_gpu_fullsize.py::test_analytics_rmat24_full_size",
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -670,7 +656,6 @@ def main():
     ap.add_argument("--bfs-workers", type=int, default=2)
     ap.add_argument("--search-workgroups", type=int, default=256,
                     help="served: workgroups each concurrent search expands its frontier with")
-    ap.add_argument("--verify-reads", type=int, default=64)
     ap.add_argument("--served-modes", default="served,plain")
     ap.add_argument("--graph", type=int, default=0,
                     help="replay rounds as hipGraphs (headline: the timed pass; --forms batched: the rounds)")
@@ -689,6 +674,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sweep", type=int, default=1, help="also run batch sizes 1e3..1e6 of the R-MAT 2^20 stream")
     ap.add_argument("--cpu-batches", type=int, default=15)
+    ap.add_argument("--legs", default="batched,served",
+                    help="headline: compact legs of the other forms to add to the JSON line ('' for none)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -704,27 +691,29 @@ def main():
         args.steps = 95 if args.steps is None else args.steps
         args.warmup = 5 if args.warmup is None else args.warmup
         return run_batched(args)
-    args.steps = 160 if args.steps is None else args.steps
     args.warmup = 4 if args.warmup is None else args.warmup
+    print(json.dumps(headline(args)))
 
+
+def headline(args):
     import torch
     import lpma_amd as L
     L.lib()
     V = 1 << args.scale
     B = args.batch
-    bulk, ins_b, del_b = make_workload(args.scale, args.edges, 2, 12, B, 13)
+    bulk, ins_b, del_b = make_workload(args.scale, args.edges, B)
     dev = torch.device("cuda", 0)
     ins_d = [L.to_device(b, dev) for b in ins_b]
     del_d = [L.to_device(b, dev) for b in del_b]
-    total = args.warmup + args.steps
-    seq = step_list(ins_b, del_b, total)
-
-    # pass 1: throughput and per-batch latency, no profiling events
+    seq = stream_steps(ins_b, del_b)
     dens = tuple(float(x) for x in args.density.split(",")) if args.density else None
-    r = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=False,
+    warm_up(bulk, ins_d, del_d, seq[:args.warmup], V, args.segment_slots, args.seg_slots_per_lane, dens)
+
+    # pass 1: the whole stream, throughput and per-batch latency, no profiling events
+    r = run_gpu(bulk, ins_d, del_d, seq, V, args.segment_slots, profile=False,
                 slots_per_lane=args.seg_slots_per_lane, graph=bool(args.graph), density=dens)
-    # pass 2: same stream with per-launch HIP events for the kernel roofline
-    rp = run_gpu(bulk, ins_d, del_d, seq, args.warmup, V, args.segment_slots, profile=True,
+    # pass 2: the same stream with per-launch HIP events for the kernel roofline
+    rp = run_gpu(bulk, ins_d, del_d, seq, V, args.segment_slots, profile=True,
                  slots_per_lane=args.seg_slots_per_lane, density=dens)
 
     if args.copy_calib:
@@ -755,25 +744,35 @@ def main():
                                                     "rebalance_wave_ms", "rebalance_block_ms", "large_ms",
                                                     "rebuild_ms", "pyramid_ms")}
     windows = {str(l): int(st.windows_per_level[l]) for l in range(32) if st.windows_per_level[l]}
+    prof_cfg = {"scale": args.scale, "edges": args.edges, "batch": B, "segment_slots": args.segment_slots,
+                "density": args.density or "reference", "seg_slots_per_lane": args.seg_slots_per_lane}
+    traffic, traffic_src = measured_traffic(name, prof_cfg)
 
-    # the R-MAT 2^20 workload at every batch size: the same stream cut into batches of
-    # 1e3 (a fixed window of 1000 batches), 1e4, 1e5 and 1e6, each from a
-    # fresh bulk load, with the rebalance kernels' algorithmic GB/s
+    # the same R-MAT stream at every batch size: batches of 1e3 (a fixed
+    # window of the first 1000), 1e4, 1e5 and 1e6 (the whole stream), each
+    # from a fresh bulk load, with the rebalance kernels' algorithmic GB/s
     sweep = []
     if args.batch_sweep:
         for Bs in (1000, 10000, 100000, 1000000):
-            bulk_s, ins_s, del_s = make_workload(args.scale, args.edges, 2, 12, Bs, 13)
-            ins_ds = [L.to_device(b, dev) for b in ins_s]
-            del_ds = [L.to_device(b, dev) for b in del_s]
-            n_steps = 1000 if Bs == 1000 else len(ins_s) + len(del_s)
-            seq_s = step_list(ins_s, del_s, n_steps + 2)
-            # timing pass (no per-phase events: at small batches their
-            # records cost as much as the kernels), then a profiled pass of
-            # the same batches for the rebalance kernels' time and bytes
-            rs = run_gpu(bulk_s, ins_ds, del_ds, seq_s, 2, V, args.segment_slots, profile=False,
-                         slots_per_lane=args.seg_slots_per_lane)
-            rsp = run_gpu(bulk_s, ins_ds, del_ds, seq_s, 2, V, args.segment_slots, profile=True,
-                          slots_per_lane=args.seg_slots_per_lane)
+            if Bs == B and not args.graph:
+                rs, rsp = r, rp
+                n_steps = len(seq)
+            else:
+                bulk_s, ins_s, del_s = make_workload(args.scale, args.edges, Bs)
+                ins_ds = [L.to_device(b, dev) for b in ins_s]
+                del_ds = [L.to_device(b, dev) for b in del_s]
+                seq_s = stream_steps(ins_s, del_s)
+                if Bs == 1000:
+                    seq_s = seq_s[:1000]
+                n_steps = len(seq_s)
+                # timing pass (no per-phase events: at small batches their
+                # records cost as much as the kernels), then a profiled pass of
+                # the same batches for the rebalance kernels' time and bytes
+                rs = run_gpu(bulk_s, ins_ds, del_ds, seq_s, V, args.segment_slots, profile=False,
+                             slots_per_lane=args.seg_slots_per_lane)
+                rsp = run_gpu(bulk_s, ins_ds, del_ds, seq_s, V, args.segment_slots, profile=True,
+                              slots_per_lane=args.seg_slots_per_lane)
+                del ins_ds, del_ds
             stt, pff = rsp["stats"], rsp["prof"]
             rms = pff.rebalance_seg_ms + pff.rebalance_wave_ms + pff.rebalance_block_ms + pff.large_ms
             rbytes = stt.alg_bytes_seg + stt.alg_bytes_wave + stt.alg_bytes_block + stt.alg_bytes_large
@@ -781,43 +780,41 @@ def main():
                           "p50_batch_ms": float(np.percentile(rs["lat_ms"], 50)),
                           "p99_batch_ms": float(np.percentile(rs["lat_ms"], 99)),
                           "rebalance_alg_GBps": rbytes / (rms / 1e3) / 1e9 if rms > 0 else 0.0,
-                          "note": "updates/s and latency from an unprofiled pass; rebalance GB/s from a "
-                                  "profiled pass over the same batches"})
+                          "rebuilds": int(stt.rebuilds)})
             log(f"batch {Bs}: {sweep[-1]['updates_per_s']:.3e} updates/s, p50 {sweep[-1]['p50_batch_ms']:.3f} ms, "
-                f"rebalance {sweep[-1]['rebalance_alg_GBps']:.0f} GB/s")
-            del ins_ds, del_ds
+                f"p99 {sweep[-1]['p99_batch_ms']:.3f} ms, rebalance {sweep[-1]['rebalance_alg_GBps']:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        cores, model = host_cpu()
         nb = min(args.cpu_batches, len(ins_b), len(del_b))
-        t = time.time()
-        v_cpu, times = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, threads)
-        log(f"cpu baseline ({threads} threads): {v_cpu:.3e} updates/s in {time.time() - t:.1f}s")
-        # the same oracle on one thread over the same sample; the baseline is
-        # the faster of the two (the merge is memory-bound at this size)
-        v_one, times_one = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, 1)
-        log(f"cpu baseline (1 thread): {v_one:.3e} updates/s")
-        runs = {threads: (v_cpu, times), 1: (v_one, times_one)}
+        runs = {}
+        for threads in sorted({cores, 1}, reverse=True):
+            t = time.time()
+            runs[threads] = cpu_baseline(bulk, ins_b, del_b, V, nb, nb, threads)
+            log(f"cpu baseline ({threads} threads): {runs[threads][0]:.3e} updates/s in {time.time() - t:.1f}s")
         best = max(runs, key=lambda k: runs[k][0])
         cpu = {"value": runs[best][0], "unit": "edge updates/s", "cores": best, "kind": "port",
-               "sample": f"CPU oracle (sorted merge + CSR offsets) on the first {nb} insert and {nb} delete batches "
-                         f"of {B} of the same stream, from the same bulk-loaded state, on {threads} threads and on "
-                         f"one; the faster is reported",
+               "host_cores": cores, "cpu_model": model,
+               "sample": f"CPU oracle (sorted merge + CSR offsets) on the GPU run's own batches: its first {nb} "
+                         f"insert batches of {B} from the bulk-loaded state and its first {nb} delete batches from "
+                         f"the state it deletes from, on all {cores} host cores and on one; the faster is reported",
                "p50_batch_ms": float(np.percentile(runs[best][1], 50)) * 1e3,
                "p99_batch_ms": float(np.percentile(runs[best][1], 99)) * 1e3,
                "by_threads": {str(k): {"value": v, "p50_batch_ms": float(np.percentile(ts, 50)) * 1e3,
                                        "p99_batch_ms": float(np.percentile(ts, 99)) * 1e3}
                               for k, (v, ts) in runs.items()}}
+    del ins_d, del_d
+    L.pool_trim()
 
     out = {
         "metric": "edge updates/sec (R-MAT 2^20 V, 16M E, batch 1e5); p99 batch latency; HBM GB/s",
         "value": value,
         "unit": "edge updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": len(seq),
         "warmup": args.warmup,
-        "ms_per_step": r["wall"] * 1e3 / args.steps,
+        "ms_per_step": r["wall"] * 1e3 / len(seq),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -825,6 +822,8 @@ def main():
         "data": "synthetic (seeded R-MAT generator)",
         "config": {"workload": f"rmat{args.scale}_{args.edges}E_half_bulk_stream_insert_then_delete_batch{B}",
                    "vertices": V, "edges": args.edges, "batch": B, "segment_slots": args.segment_slots,
+                   "timed": f"the whole stream: {len(ins_b)} insert + {len(del_b)} delete batches from a fresh bulk "
+                            f"load (warm-up batches ran on another instance)",
                    "density": ("lower 0.42 / upper 1.0 (reference)" if not args.density else
                                "lower leaf,root / upper leaf,root = " + args.density),
                    "parallelism": "single GPU"},
@@ -835,7 +834,7 @@ def main():
         "rebalance_alg_GBps": reb_bytes / (reb_ms / 1e3) / 1e9 if reb_ms > 0 else 0.0,
         "phase_ms_total": phases,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name, B),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
                      "launches": int(launches)},
         "cpu_baseline": cpu,
@@ -844,7 +843,52 @@ def main():
         "rebuilds": int(st.rebuilds),
         "final_state": {"edges": r["edges"], "capacity": r["capacity"], "tree_height": r["height"]},
     }
-    print(json.dumps(out))
+    for leg in [x for x in args.legs.split(",") if x]:
+        t = time.time()
+        try:
+            out[leg] = LEGS[leg](args)
+        except Exception as e:  # a leg that fails leaves the headline intact
+            out[leg] = {"error": repr(e)}
+        log(f"leg {leg}: {time.time() - t:.1f}s")
+    return out
+
+
+def batched_leg(args):
+    """The batched form against the plain form: 64 PMAs, 5 warm-up and 95
+    timed rounds, the autotuned configuration."""
+    import copy
+    a = copy.copy(args)
+    a.steps, a.warmup, a.forms, a.tune, a.graph = 95, 5, "both", 1, 0
+    o = batched_measure(a)
+    keep = ("value", "plain_value", "speedup_vs_plain", "round_p50_ms", "round_p99_ms", "plain_round_p50_ms",
+            "plain_round_p99_ms", "graph_rounds_value", "graph_round_p50_ms", "graph_round_p99_ms")
+    leg = {k: o[k] for k in keep}
+    leg["unit"] = "edge updates/s"
+    leg["config"] = {k: o["config"][k] for k in ("workload", "graphs", "initial_edges", "updates_timed",
+                                                  "segment_slots", "density_leaf_root")}
+    leg["p99_not_above_plain"] = o["round_p99_ms"] <= o["plain_round_p99_ms"]
+    return leg
+
+
+def served_leg(args):
+    """The request server against the one-request-at-a-time loop at the top
+    offered rate of the sweep (16000 requests/s for 1 s)."""
+    import copy
+    a = copy.copy(args)
+    a.rates, a.duration, a.served_modes = "16000", 1.0, "served,plain"
+    o = served_measure(a)
+    top, base = o["sweep"]["served"][-1], o["sweep"]["plain"][-1]
+    return {"value": o["value"], "unit": "edge updates/s", "plain_value": base["sustained_updates_per_s"],
+            "speedup_vs_plain": o["speedup_vs_plain"], "update_p50_ms": top.get("update_p50_ms"),
+            "update_p99_ms": top.get("update_p99_ms"), "plain_update_p50_ms": base.get("update_p50_ms"),
+            "plain_update_p99_ms": base.get("update_p99_ms"), "read_p99_ms": top.get("read_p99_ms"),
+            "bfs_p99_ms": top.get("bfs_p99_ms"),
+            "p99_not_above_plain": (top.get("update_p99_ms") or 0) <= (base.get("update_p99_ms") or 0),
+            "config": {"workload": o["config"]["workload"], "edges": o["config"]["edges"],
+                       "offered_req_per_s": 16000, "duration_s": 1.0}}
+
+
+LEGS = {"batched": batched_leg, "served": served_leg}
 
 
 if __name__ == "__main__":

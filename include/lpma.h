@@ -14,7 +14,9 @@
  * lpma_get_view is owned by the handle and stays valid until the next
  * mutating call on that handle.  One handle = one writer: calls on the same
  * handle must not overlap.  Errors are returned, never thrown across the ABI;
- * a failed update leaves the structure unchanged.
+ * a failed update leaves the structure unchanged (every allocation of an
+ * update precedes its first change), except after LPMA_ERR_DEVICE: a HIP
+ * runtime failure leaves the handle undefined, and it must be destroyed.
  *
  * Each entry point names the reference interface it replaces
  * (file:line in pkumod/LPMA).
@@ -256,6 +258,15 @@ int lpma_set_launch(lpma_t *h, uint32_t seg_slots_per_lane);
 int lpma_set_pool_budget(uint64_t bytes);
 int lpma_pool_trim(void);
 uint64_t lpma_pool_reserved_bytes(void);
+
+/* Fault injection for failure-path tests (no counterpart in the reference,
+ * whose allocations print and carry on, rpma.cuh:13-19): the nth allocation
+ * from now (device pool blocks, cached or new, and pinned host buffers)
+ * fails, and so does every later one until the next call, as when memory
+ * runs out; calls that need them return LPMA_ERR_OUT_OF_MEMORY.  0 disables
+ * it.  Process-wide.  When `remaining` is not NULL it receives the countdown
+ * this call replaced (0: the earlier injection fired, or none was set). */
+int lpma_inject_alloc_failure(uint64_t nth, uint64_t *remaining);
 
 const char *lpma_strerror(int code);
 

@@ -145,6 +145,7 @@ SIGNATURES = [
     ("lpma_set_pool_budget", _int, [_u64]),
     ("lpma_pool_trim", _int, []),
     ("lpma_pool_reserved_bytes", _u64, []),
+    ("lpma_inject_alloc_failure", _int, [_u64, ctypes.POINTER(_u64)]),
     ("lpma_strerror", ctypes.c_char_p, [_int]),
     ("lpma_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
     ("lpma_server_create", _int, [ctypes.POINTER(_vp), _vp, ctypes.POINTER(ServerConfig)]),
@@ -274,6 +275,28 @@ def mt64(seed, n):
 
 
 # ------------------------------------------------------------ device side
+
+def set_pool_budget(nbytes):
+    """Caps the bytes the device pool holds from hipMalloc (0 = unlimited)."""
+    _check(lib().lpma_set_pool_budget(int(nbytes)), "set_pool_budget")
+
+
+def pool_reserved_bytes():
+    return int(lib().lpma_pool_reserved_bytes())
+
+
+def pool_trim():
+    _check(lib().lpma_pool_trim(), "pool_trim")
+
+
+def inject_alloc_failure(nth):
+    """The nth allocation from now and every later one fail (0 disables).
+    Returns the countdown of the injection this call replaced (0: it fired,
+    or none was set)."""
+    rem = _u64(0)
+    _check(lib().lpma_inject_alloc_failure(int(nth), ctypes.byref(rem)), "inject_alloc_failure")
+    return int(rem.value)
+
 
 def _torch():
     import torch
@@ -414,6 +437,13 @@ class PMA:
         _check(lib().lpma_export_compacted(self._h, keys.data_ptr(), ro.data_ptr(), _stream_ptr(self.stream)),
                "export")
         return keys[: int(v.edges)], ro
+
+    def row_offsets(self):
+        """Standard CSR row offsets alone (export_compacted without the keys)."""
+        torch = _torch()
+        ro = torch.empty(self.num_vertices + 1, dtype=torch.int32, device=self.device)
+        _check(lib().lpma_export_compacted(self._h, None, ro.data_ptr(), _stream_ptr(self.stream)), "row_offsets")
+        return ro
 
     def get_data(self):
         torch = _torch()

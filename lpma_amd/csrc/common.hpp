@@ -175,6 +175,7 @@ struct Group {
     u32 log2S;    // segment size, the same for every PMA of the run
     u32 seg_log2G;  // lanes per window of the single-segment rebalance (log2)
     u32 inline_one; // one PMA whose geometry travels in `one` (no table load)
+    u32 defer_rebuild;  // elect no windows when some PMA needs a root rebuild or resize
     const RoundHdr *rh;  // graph replays: n and epoch come from here
     DevState one;
 };

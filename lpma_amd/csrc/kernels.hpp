@@ -44,6 +44,8 @@ hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchB
 hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st);
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st);
+// zeroes the work counts of the status (before a repeated election)
+hipError_t reset_work_counts(Status *status, hipStream_t st);
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st, hipEvent_t e0 = nullptr,
                          hipEvent_t e1 = nullptr);
 // with events (timing): the launch records its own start and end in them

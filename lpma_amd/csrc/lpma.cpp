@@ -6,6 +6,7 @@
 // is a plain rebalance, a root rebalance or a resize), instead of the
 // reference's per-level host loop with a device sync after every step.
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <math.h>
 #include <string.h>
 
@@ -66,6 +67,21 @@ std::mutex g_pools_mu;
 std::map<int, Pool *> g_pools;
 size_t g_budget = 0;  // 0 = unlimited
 
+// Fault injection for the failure-path tests: when positive, a countdown of
+// allocations (pool blocks, cached or new, and the pinned host buffers of a
+// run all count); the one that brings it to zero fails and so does every
+// later one (-1) until the injection is reset, as when memory runs out.
+std::atomic<long long> g_fail_countdown{0};
+
+bool injected_failure() {
+    long long c = g_fail_countdown.load(std::memory_order_relaxed);
+    while (c > 0) {
+        if (g_fail_countdown.compare_exchange_weak(c, c == 1 ? -1 : c - 1, std::memory_order_relaxed))
+            return c == 1;
+    }
+    return c < 0;
+}
+
 Pool &pool_for(int dev) {
     std::lock_guard<std::mutex> g(g_pools_mu);
     auto it = g_pools.find(dev);
@@ -76,6 +92,7 @@ Pool &pool_for(int dev) {
 }
 
 int pool_alloc(int dev, void **out, size_t bytes) {
+    if (injected_failure()) return LPMA_ERR_OUT_OF_MEMORY;
     Pool &P = pool_for(dev);
     std::lock_guard<std::mutex> g(P.mu);
     size_t r = Pool::round(bytes);
@@ -308,6 +325,7 @@ void release(lpma *h, DBuf<T> &b) { release_on(h->device, b); }
 template <typename T>
 int ensure_host(T *&p, u32 &cap, u64 n, hipStream_t sync_st) {
     if (p && cap >= n) return LPMA_OK;
+    if (injected_failure()) return LPMA_ERR_OUT_OF_MEMORY;
     u64 want = std::max<u64>(n, 2ull * cap);
     if (want < 8) want = 8;
     if (p) {
@@ -482,9 +500,15 @@ int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
     return LPMA_OK;
 }
 
+void release_geometry(lpma *h, Geometry &g) {
+    release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
+}
+
+// installs g as h's arrays (g is left empty)
 void install_geometry(lpma *h, u32 H, Geometry &g) {
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->sep); release(h, h->owner);
     h->keys = g.keys; h->cnt = g.cnt; h->pyr = g.pyr; h->sep = g.sep; h->owner = g.owner;
+    g = Geometry();
     h->st.keys = h->keys.p;
     h->st.cnt = h->cnt.p;
     h->st.pyr = h->pyr.p;
@@ -539,6 +563,7 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
         (r = ensure_host(pl.h_segs, pl.h_segs_n, nsegs, pl.cur_stream)))
         return r;
     if (!pl.h_out || pl.h_out_bytes < out_words * 8) {
+        if (injected_failure()) return LPMA_ERR_OUT_OF_MEMORY;
         if (pl.h_out) {
             (void)hipStreamSynchronize(pl.cur_stream);
             (void)hipHostFree(pl.h_out);
@@ -571,17 +596,18 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
         if (r_ != LPMA_OK) return r_; \
     } while (0)
 
-// Spread m sorted keys from `merged` over a fresh geometry of height H and
-// install it (create and resize).
-int respread_into_new(lpma *h, u32 H, const u64 *merged, u64 m, hipStream_t st) {
-    Geometry g;
-    CHECKED(alloc_geometry(h, H, g, st));
+// Spread m sorted keys from `merged` over the fresh arrays g of height H and
+// install them (create and resize); g is released on failure.
+int spread_and_install(lpma *h, u32 H, Geometry &g, const u64 *merged, u64 m, hipStream_t st) {
     DevState d2 = state_with(h, H, g);
-    HIPCHECK(rebuild_spread(d2, merged, m, st));
-    HIPCHECK(rebuild_pyramid(d2, true, st));
     // old arrays may still be read by queued kernels: wait before they
     // return to the pool
-    HIPCHECK(hipStreamSynchronize(st));
+    if (rebuild_spread(d2, merged, m, st) != hipSuccess || rebuild_pyramid(d2, true, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        (void)hipGetLastError();
+        release_geometry(h, g);
+        return LPMA_ERR_DEVICE;
+    }
     install_geometry(h, H, g);
     return LPMA_OK;
 }
@@ -647,16 +673,22 @@ int upload_table(Pipeline &pl, const Member *mem, u32 npma, u32 &pyr_blocks, hip
     return LPMA_OK;
 }
 
-// After the round's status is on the host: root rebuilds / resizes of the
-// flagged members (checked before anything changes), then the bookkeeping.
-// `windows_done` tells whether the window rebalances and the pyramid have
-// already been queued (graph replays) or are still to be launched here;
-// `early` that the single-segment and wave classes were queued before the
-// status was read (they only touch members without a root rebuild or
-// resize, whose windows the election drops).
-int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 pyr_blocks, bool windows_done,
+// After the round's status is on the host: the window classes still to be
+// launched, the pyramid, root rebuilds / resizes of the flagged members, then
+// the bookkeeping.  `windows_done` tells whether the window rebalances and
+// the pyramid have already been queued (graph replays); `early` that the
+// single-segment and wave classes were queued before the status was read
+// (a lone PMA: they only touch a PMA without a root rebuild or resize, whose
+// windows the election drops, and their scratch was reserved beforehand).
+//
+// Every fallible allocation of the round comes before its first change of a
+// PMA, so a round that fails with LPMA_ERR_OUT_OF_MEMORY changes nothing.  A
+// graph replay over several PMAs elects no windows at all when one of them
+// needs a rebuild (Group::defer_rebuild); the election then runs again here,
+// after the allocations.
+int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u32 pyr_blocks, bool windows_done,
                  hipStream_t st, bool early = false) {
-    const Status s = *(const Status *)pl.h_out;
+    Status s = *(const Status *)pl.h_out;
     const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
     pl.last_wave_windows = s.nwork[CLASS_WAVE];
     if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
@@ -667,53 +699,95 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
         return LPMA_OK;
     }
 
-    // new live counts; every root rebuild / resize is checked before any
-    // kernel changes a PMA, so a failing round changes nothing
+    // new live counts and heights; every root rebuild / resize is checked
+    // before any kernel changes a PMA
     std::vector<u64> Mnew(npma);
-    u64 m_sum = 0, m_rebuild = 0;
+    std::vector<u32> newH(npma);
+    u64 m_sum = 0, nseg_rebuild = 0;
     bool any_rebuild = false;
     for (u32 p = 0; p < npma; ++p) {
         lpma *h = mem[p].h;
         Mnew[p] = (u64)((i64)h->live + (ps[p].pd1 - ps[p].pd0));
         m_sum += Mnew[p];
+        newH[p] = h->st.H;
         if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) {
             any_rebuild = true;
-            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, Mnew[p]) : h->st.H;
-            if (!height_ok(h, newH)) return LPMA_ERR_TOO_LARGE;
-            m_rebuild = std::max(m_rebuild, Mnew[p]);
+            if (ps[p].flags & FLAG_RESIZE) newH[p] = fit_height(h, Mnew[p]);
+            if (!height_ok(h, newH[p])) return LPMA_ERR_TOO_LARGE;
+            nseg_rebuild = std::max<u64>(nseg_rebuild, 1ull << h->st.H);
         }
     }
+    const bool reelect = windows_done && any_rebuild && g_in.defer_rebuild;
+    if (reelect) windows_done = false;
+    Group g = g_in;
+    g.defer_rebuild = 0;
+
+    // allocations: scratch of the large windows (bounded by the runs when the
+    // election is still to come), of the rebuilds, and the new arrays of
+    // every resizing member
+    std::vector<Geometry> geo(npma);
+    auto undo = [&](int r) {
+        for (u32 p = 0; p < npma; ++p) release_geometry(mem[p].h, geo[p]);
+        return r;
+    };
+    {
+        int r = LPMA_OK;
+        const u64 nlarge = reelect ? n_eff : (windows_done ? 0 : s.nwork[CLASS_LARGE]);
+        if (nlarge || any_rebuild) r = ensure(pl, pl.merged, m_sum + 1);
+        if (!r && nlarge) {
+            if (!(r = ensure(pl, pl.lchunk, nlarge + 1)) && !(r = ensure(pl, pl.lmbase, nlarge)))
+                r = ensure(pl, pl.lhdr, 4);
+        }
+        if (!r && any_rebuild) {
+            if (!(r = ensure(pl, pl.gpre, 2 * nseg_rebuild + 2)))
+                r = ensure(pl, pl.scan_scr, scan_scratch_words(nseg_rebuild) + 8);
+            for (u32 p = 0; !r && p < npma; ++p)
+                if (newH[p] != mem[p].h->st.H) r = alloc_geometry(mem[p].h, newH[p], geo[p], st);
+        }
+        if (r) return undo(r);
+    }
     BatchBufs bb = batch_bufs(pl);
+#define HIPUNDO(x)                           \
+    do {                                     \
+        if ((x) != hipSuccess) {             \
+            (void)hipGetLastError();         \
+            return undo(LPMA_ERR_DEVICE);    \
+        }                                    \
+    } while (0)
+
+    if (reelect) {
+        // the graph's election stood down: clear its (empty) work counts and
+        // elect again, now that nothing can fail for lack of memory
+        HIPUNDO(reset_work_counts(bb.status, st));
+        HIPUNDO(elect_windows(g, n_eff, bb, st));
+        HIPUNDO(hipMemcpyAsync(pl.h_out, pl.out.p, sizeof(Status), hipMemcpyDeviceToHost, st));
+        HIPUNDO(hipStreamSynchronize(st));
+        s = *(const Status *)pl.h_out;
+    }
 
     if (!windows_done) {
         if (s.nwork[CLASS_SEG] && !early) {
             LaunchTimer t(pl, PH_SEG);
-            HIPCHECK(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st, t.a, t.b));
+            HIPUNDO(rebalance_seg(g, bb, s.nwork[CLASS_SEG], st, t.a, t.b));
             pl.launches_seg++;
         }
         if (s.nwork[CLASS_WAVE] && !early) {
             LaunchTimer t(pl, PH_WAVE);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st, t.a, t.b));
+            HIPUNDO(rebalance_lds(g, bb, CLASS_WAVE, std::min<u32>(s.nwork[CLASS_WAVE], 65535), st, t.a, t.b));
             pl.launches_wave++;
         }
         if (s.nwork[CLASS_BLOCK]) {
             LaunchTimer t(pl, PH_BLOCK);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st, t.a, t.b));
+            HIPUNDO(rebalance_lds(g, bb, CLASS_BLOCK, std::min<u32>(s.nwork[CLASS_BLOCK], 65535), st, t.a, t.b));
             pl.launches_block++;
         }
         if (s.nwork[CLASS_LARGE]) {
-            // the windows' merged slices add up to at most the members' live keys
-            CHECKED(ensure(pl, pl.merged, m_sum + 1));
-            CHECKED(ensure(pl, pl.lchunk, (u64)s.nwork[CLASS_LARGE] + 1));
-            CHECKED(ensure(pl, pl.lmbase, s.nwork[CLASS_LARGE]));
-            CHECKED(ensure(pl, pl.lhdr, 4));
-            bb = batch_bufs(pl);
             PhaseTimer t(pl, st, PH_LARGE);
-            HIPCHECK(rebalance_large(g, bb, pl.merged.p, 1024, st));
+            HIPUNDO(rebalance_large(g, bb, pl.merged.p, 1024, st));
         }
         {
             PhaseTimer t(pl, st, PH_PYR);
-            HIPCHECK(group_pyramid(g, pyr_blocks, st));
+            HIPUNDO(group_pyramid(g, pyr_blocks, st));
         }
     }
 
@@ -722,23 +796,19 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
         // PMA is merged into scratch and spread again (rebuild_rpma,
         // rpma.cuh:1417-1472)
         PhaseTimer t(pl, st, PH_REBUILD);
-        CHECKED(ensure(pl, pl.merged, m_rebuild + 1));
-        bb = batch_bufs(pl);
         for (u32 p = 0; p < npma; ++p) {
             if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
             lpma *h = mem[p].h;
             const u64 M = Mnew[p];
-            u32 newH = (ps[p].flags & FLAG_RESIZE) ? fit_height(h, M) : h->st.H;
             u64 nseg = 1ull << h->st.H;
-            CHECKED(ensure(pl, pl.gpre, 2 * nseg + 2));
-            CHECKED(ensure(pl, pl.scan_scr, scan_scratch_words(nseg) + 8));
-            HIPCHECK(rebuild_push(h->st, p, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.gpre.p + nseg + 1,
-                                  pl.scan_scr.p, st));
-            if (newH != h->st.H) {
-                CHECKED(respread_into_new(h, newH, pl.merged.p, M, st));
+            HIPUNDO(rebuild_push(h->st, p, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.gpre.p + nseg + 1,
+                                 pl.scan_scr.p, st));
+            if (newH[p] != h->st.H) {
+                int r = spread_and_install(h, newH[p], geo[p], pl.merged.p, M, st);
+                if (r) return undo(r);
             } else {
-                HIPCHECK(rebuild_spread(h->st, pl.merged.p, M, st));
-                HIPCHECK(rebuild_pyramid(h->st, true, st));
+                HIPUNDO(rebuild_spread(h->st, pl.merged.p, M, st));
+                HIPUNDO(rebuild_pyramid(h->st, true, st));
             }
             h->rebuilds++;
             h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
@@ -746,6 +816,7 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g, u32 
             pl.rebuild_slots += (u64)1 << (h->st.H + h->log2S);
         }
     }
+#undef HIPUNDO
     for (u32 p = 0; p < npma; ++p) {
         lpma *h = mem[p].h;
         h->live = Mnew[p];
@@ -796,6 +867,14 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     if (hi_bit > 64) return LPMA_ERR_TOO_LARGE;
     if (pl.graph_on && maxreq <= 1) return run_pipeline_graph(pl, mem, npma, n, pshift, hi_bit, st);
     CHECKED(ensure_batch(pl, n, npma, nsegs, std::min<u64>(n, nseg_total)));
+    // A lone PMA's single-segment and wave windows are queued before the
+    // host reads the status (below): the large windows that may follow
+    // them get their scratch for the worst case now (their merged keys are
+    // at most the live keys after the batch), or the status is read first.
+    bool early = npma == 1;
+    if (early && (ensure(pl, pl.merged, mem[0].h->live + n + 1) || ensure(pl, pl.lchunk, n + 1) ||
+                  ensure(pl, pl.lmbase, n) || ensure(pl, pl.lhdr, 4)))
+        early = false;
 
     u32 epoch, pyr_blocks;
     CHECKED(begin_epoch(mem, npma, epoch, st));
@@ -850,6 +929,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     g.seg_log2G = seg_lanes_log2(g.log2S, pl.seg_slots_log2);
     g.rh = nullptr;
     g.inline_one = npma == 1 ? 1u : 0u;
+    g.defer_rebuild = 0;
     g.one = mem[0].h->st;
     BatchBufs bb = batch_bufs(pl);
     // bit 0 of a packed key is the operation (with the request order above
@@ -882,7 +962,6 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     // status and queues the rest: a PMA with a root rebuild or resize has no
     // windows, so a round that fails after the status changes nothing.
     HIPCHECK(hipMemcpyAsync(pl.h_out, pl.out.p, out_bytes, hipMemcpyDeviceToHost, st));
-    const bool early = npma == 1;
     if (early) {
         if (!pl.ev_status) HIPCHECK(hipEventCreateWithFlags(&pl.ev_status, hipEventDisableTiming));
         HIPCHECK(hipEventRecord(pl.ev_status, st));
@@ -932,6 +1011,7 @@ int prepare_graph_buffers(Pipeline &pl, u64 cap, u32 npma, u64 merged_keys) {
     CHECKED(ensure(pl, pl.lhdr, 4));
     CHECKED(ensure(pl, pl.rhdr, 1));
     if (!pl.h_rhdr) {
+        if (injected_failure()) return LPMA_ERR_OUT_OF_MEMORY;
         if (hipHostMalloc((void **)&pl.h_rhdr, sizeof(RoundHdr), hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
             pl.h_rhdr = nullptr;
@@ -957,6 +1037,9 @@ Group graph_group(Pipeline &pl, u32 npma, u32 pshift, u32 log2S) {
     g.rh = pl.rhdr.p;
     // a captured graph outlives resizes: geometry always from the table
     g.inline_one = 0;
+    // several PMAs: no windows in a round where one of them rebuilds (its
+    // allocations come first, then the election runs again)
+    g.defer_rebuild = npma > 1 ? 1u : 0u;
     memset(&g.one, 0, sizeof(g.one));
     return g;
 }
@@ -1064,7 +1147,9 @@ int run_pipeline_graph(Pipeline &pl, const Member *mem, u32 npma, u64 n, u32 psh
     }
     pl.graph_launches++;
     HIPCHECK(hipStreamSynchronize(st));
-    const Group g = graph_group(pl, npma, pshift, log2S);
+    Group g = graph_group(pl, npma, pshift, log2S);
+    g.epoch = epoch;  // a repeated election runs eagerly, outside the graph
+    g.rh = nullptr;
     return finish_round(pl, mem, npma, g, pyr_blocks, true, st);
 }
 
@@ -1261,9 +1346,13 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
     }
     h->st.H = H;
     compute_density(h);
-    if ((r = respread_into_new(h, H, h->pl.merged.p, num_vertices, st)) != LPMA_OK) {
-        lpma_destroy(h);
-        return r;
+    {
+        Geometry g;
+        if ((r = alloc_geometry(h, H, g, st)) != LPMA_OK ||
+            (r = spread_and_install(h, H, g, h->pl.merged.p, num_vertices, st)) != LPMA_OK) {
+            lpma_destroy(h);
+            return r;
+        }
     }
     h->live = num_vertices;
     if (hipStreamSynchronize(st) != hipSuccess) {
@@ -1887,6 +1976,13 @@ int lpma_gen_rmat_device(uint32_t scale, uint64_t n, double a, double b, double 
 int lpma_set_pool_budget(uint64_t bytes) {
     std::lock_guard<std::mutex> g(g_pools_mu);
     g_budget = bytes;
+    return LPMA_OK;
+}
+
+int lpma_inject_alloc_failure(uint64_t nth, uint64_t *remaining) {
+    long long prev =
+        g_fail_countdown.exchange(nth > (uint64_t)LLONG_MAX ? LLONG_MAX : (long long)nth, std::memory_order_relaxed);
+    if (remaining) *remaining = prev > 0 ? (uint64_t)prev : 0;  // -1: it fired
     return LPMA_OK;
 }
 

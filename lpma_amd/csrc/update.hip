@@ -332,7 +332,8 @@ template <bool ONE>
 __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
                           Status *status) {
-    if (status->flags & (FLAG_INVALID_KEY | FLAG_SORT_STALL)) return;
+    const u32 stop = FLAG_INVALID_KEY | FLAG_SORT_STALL | (g.defer_rebuild ? FLAG_ROOT | FLAG_RESIZE : 0u);
+    if (status->flags & stop) return;
     const u32 n_eff = status->n_eff;
     u32 j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;
@@ -411,6 +412,15 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
         Work *dst = cls == CLASS_WAVE ? w0 : (cls == CLASS_BLOCK ? w1 : (cls == CLASS_LARGE ? w2 : w3));
         dst[cbase[cls] + off] = wk;
     }
+}
+
+__global__ void k_reset_work(Status *s) {
+    if (threadIdx.x < NUM_CLASSES) s->nwork[threadIdx.x] = 0;
+}
+
+hipError_t reset_work_counts(Status *status, hipStream_t st) {
+    k_reset_work<<<1, 64, 0, st>>>(status);
+    return hipGetLastError();
 }
 
 // n bounds the number of runs (the batch size); the kernel reads the exact

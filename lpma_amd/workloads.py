@@ -11,9 +11,38 @@ import os
 
 import numpy as np
 
-from . import gen_rmat, gen_rmat_samples
+from . import gen_rmat, gen_rmat_samples, shuffle
 
 RMAT = (0.57, 0.19, 0.19)  # a, b, c; d = 0.05
+
+_stream_keys = {}
+
+
+def rmat_stream(scale=20, edges=1 << 24, seed=2, shuffle_seed=12, batch=100000, del_seed=13):
+    """The R-MAT update stream of the power-law workload.
+
+    `edges` distinct R-MAT edges on 2^scale vertices (a, b, c = RMAT, seed
+    `seed`, self loops and repeats rejected), shuffled with `shuffle_seed`.
+    The first half is the bulk load; the second half is cut into insert
+    batches of `batch`; then as many edges as were inserted, a uniform sample
+    without replacement (seed `del_seed`) of all `edges` -- the live set once
+    the inserts are done -- are cut into delete batches of `batch`.
+    Returns (bulk, insert batches, delete batches) as uint64 arrays.  The
+    generated keys are cached per (scale, edges, seed, shuffle_seed).
+    """
+    k = (scale, edges, seed, shuffle_seed)
+    if k not in _stream_keys:
+        a, b, c = RMAT
+        _stream_keys[k] = shuffle(gen_rmat(scale, edges, a, b, c, seed), shuffle_seed)
+    keys = _stream_keys[k]
+    half = edges // 2
+    bulk = keys[:half]
+    rest = keys[half:]
+    ins = [rest[i:i + batch] for i in range(0, rest.size, batch)]
+    rng = np.random.default_rng(del_seed)
+    dels = keys[rng.permutation(edges)[: rest.size]]
+    dls = [dels[i:i + batch] for i in range(0, dels.size, batch)]
+    return bulk, ins, dls
 
 
 def _pow2_scale(x):

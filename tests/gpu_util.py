@@ -20,6 +20,22 @@ def say(msg):
     sys.stdout.flush()
 
 
+def with_markers(ref, ro, V):
+    """The live keys get_data_rpma returns (rpma.cuh:1735-1761): the edges
+    and one row marker r<<32|0xFFFFFFFF per vertex, in key order (a row's
+    marker follows its edges, at the row's end offset)."""
+    markers = (np.arange(V, dtype=np.uint64) << U32) | np.uint64(0xFFFFFFFF)
+    return np.insert(ref, ro[1:].astype(np.int64), markers)
+
+
+def check_get_data(pma, ref, ro, V):
+    got = L.to_host_u64(pma.get_data())
+    want = with_markers(ref, ro, V)
+    assert got.size == want.size, (got.size, want.size)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"get_data differs at {bad[:5]}: got {got[bad[:5]]}, want {want[bad[:5]]}"
+
+
 class Triple:
     """The HIP store, the sorted-merge oracle and the PMA layout model, kept in
     lock step.  `model=False` skips the layout model (large cases)."""
@@ -66,7 +82,9 @@ class Triple:
         assert ek.size == self.ref.size
         bad = np.nonzero(ek != self.ref)[0]
         assert bad.size == 0, f"first mismatch at {bad[:5]}: got {ek[bad[:5]]}, want {self.ref[bad[:5]]}"
-        assert np.array_equal(ro, oracle.row_offsets(self.ref, self.V))
+        want_ro = oracle.row_offsets(self.ref, self.V)
+        assert np.array_equal(ro, want_ro)
+        check_get_data(self.pma, self.ref, want_ro, self.V)
         if layout and self.model is not None:
             gk, gro = self.pma.copy_view()
             gk = L.to_host_u64(gk)

@@ -227,7 +227,7 @@ def test_queries():
     t.close()
 
 
-@pytest.mark.parametrize("n", [9000, 300000, 700000, 3000000])
+@pytest.mark.parametrize("n", [9000, 300000, 700000, 3000000, 5000000, 9000000])
 def test_batch_sizes_cover_sort_paths(n):
     """Batches that exercise every radix-sort path: one workgroup (<= 8192),
     self-offset tiles, and tiles offset by a device-wide scan."""
