@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pma {
 
 typedef uint64_t u64;
@@ -272,22 +274,74 @@ __device__ __forceinline__ bool segment_holds(const DevState &d, u32 s, u64 key)
 
 __device__ __forceinline__ u32 lane_id() { return __lane_id(); }
 
-template <typename T>
-__device__ __forceinline__ T wave_incl_scan(T v) {
-    const int lane = (int)lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        T t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
+// Inclusive prefix sum of 32-bit lanes by DPP, with no LDS round trip:
+// row shifts by 1, 2, 4, 8 scan each row of 16 lanes (a lane whose source
+// lies outside its row adds 0), then lane 15's total is broadcast into rows 1
+// and 3 and lane 31's into rows 2 and 3 (row_bcast:15 / row_bcast:31).
+// Every lane of the wave must be active.
+__device__ __forceinline__ u32 dpp_incl_scan32(u32 v) {
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
+// AND / OR of a 64-bit value over the wave, the same in every lane (the
+// DPP scan pattern with the operation's identity for lanes out of a row;
+// lane 63 ends with the whole wave's result)
+template <bool AND>
+__device__ __forceinline__ u32 dpp_reduce32(u32 v) {
+    const int id = AND ? -1 : 0;
+#define LPMA_DPP_STEP(ctrl, rows)                                                           \
+    {                                                                                       \
+        const u32 t = (u32)__builtin_amdgcn_update_dpp(id, (int)v, ctrl, rows, 0xF, false); \
+        v = AND ? (v & t) : (v | t);                                                        \
+    }
+    LPMA_DPP_STEP(0x111, 0xF)
+    LPMA_DPP_STEP(0x112, 0xF)
+    LPMA_DPP_STEP(0x114, 0xF)
+    LPMA_DPP_STEP(0x118, 0xF)
+    LPMA_DPP_STEP(0x142, 0xA)
+    LPMA_DPP_STEP(0x143, 0xC)
+#undef LPMA_DPP_STEP
+    return (u32)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ u64 wave_and64(u64 v) {
+    return ((u64)dpp_reduce32<true>((u32)(v >> 32)) << 32) | dpp_reduce32<true>((u32)v);
+}
+__device__ __forceinline__ u64 wave_or64(u64 v) {
+    return ((u64)dpp_reduce32<false>((u32)(v >> 32)) << 32) | dpp_reduce32<false>((u32)v);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    if constexpr (sizeof(T) == 4 && std::is_integral<T>::value) {
+        return (T)dpp_incl_scan32((u32)v);
+    } else {
+        const int lane = (int)lane_id();
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            T t = __shfl_up(v, o, 64);
+            if (lane >= o) v += t;
+        }
+        return v;
+    }
+}
+
+// sum over the wave, the same in every lane (every lane of the wave active)
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
+    if constexpr (sizeof(T) == 4 && std::is_integral<T>::value) {
+        return (T)__builtin_amdgcn_readlane((int)dpp_incl_scan32((u32)v), 63);
+    } else {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    }
 }
 
 // exclusive scan across a block of NT threads (NT multiple of 64).
@@ -305,16 +359,17 @@ __device__ __forceinline__ T block_excl_scan(T v, T &total, T *tmp) {
     }
     if ((tid & 63) == 63) tmp[w] = inc;
     __syncthreads();
-    if (tid == 0) {
-        T run = 0;
-        for (int i = 0; i < NW; ++i) { T t = tmp[i]; tmp[i] = run; run += t; }
-        tmp[NW] = run;
+    // every thread adds up the wave totals (the same LDS words for all lanes)
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const T t = tmp[i];
+        tot += t;
+        pre += i < w ? t : (T)0;
     }
-    __syncthreads();
-    T res = tmp[w] + inc - v;
-    total = tmp[NW];
-    __syncthreads();
-    return res;
+    total = tot;
+    __syncthreads();  // tmp is free for the next call
+    return pre + inc - v;
 }
 
 // in-place exclusive scan of an LDS array of n entries (n may exceed NT);

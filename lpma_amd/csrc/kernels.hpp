@@ -9,8 +9,10 @@ u64 scan_scratch_words(u64 n);
 hipError_t device_excl_scan(const u32 *in, u64 n, u32 *out, u32 *scratch, hipStream_t st);
 u64 sort_scratch_words(u64 n);
 u32 sort_zero_words();
+// k0, k1, k2: n keys each (k2 only for batches of up to 2^17 keys, as the
+// second buffer of an oversized bucket)
 u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
-                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st);
+                   u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, u32 *flags, hipStream_t st);
 
 // update.hip
 struct BatchBufs {

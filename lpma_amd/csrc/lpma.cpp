@@ -940,7 +940,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         PhaseTimer t(pl, st, PH_SORT);
         HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), st));
         pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
-                           pl.sort_scr.p, &bb.status->flags, st);
+                           pl.ukey.p, pl.sort_scr.p, &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
     }
     {
@@ -1077,7 +1077,7 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
                   batch_begin(g, bb.status, none, pl.segs.p, pl.sort_scr.p, sort_zero_words(), cs) == hipSuccess;
         if (ok) {
             u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, 0, hi_bit, pl.pk0.p, pl.pk1.p,
-                                    pl.sort_scr.p, &bb.status->flags, cs);
+                                    pl.ukey.p, pl.sort_scr.p, &bb.status->flags, cs);
             ok = hipGetLastError() == hipSuccess && locate_and_compact(g, pk, cap, bb, cs) == hipSuccess &&
                  density_check(g, cap, bb, cs) == hipSuccess && elect_windows(g, cap, bb, cs) == hipSuccess &&
                  rebalance_fixed(g, bb, (u32)cap, nullptr, cs) == hipSuccess &&

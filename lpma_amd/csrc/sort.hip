@@ -1,13 +1,18 @@
-// sort.hip -- batch packing, LDS-staged wave64 LSD radix sort, and the
-// device-wide u32 exclusive scan used across the library.
+// sort.hip -- batch packing and sorting on gfx950, and the device-wide u32
+// exclusive scan used across the library.
 //
 // Replaces thrust::sort_by_key over the batch (rpma.cuh:1557,
 // gpma.cuh:751).  The batch key src<<32|dst and its operation are packed
-// into ((src << vbits | dst) << 1 | op), vbits = ceil(log2 V), so only
-// 2*vbits+1 bits are sorted (41 bits, 6 passes, for V = 2^20), and equal
-// keys end up with the delete before the insert; the dedup that follows
-// keeps the last of each run, which makes a batch "deletes, then inserts".
-// A group of PMAs sorts as one batch: the PMA id sits above the packed key.
+// into ((src << vbits | dst) << 1 | op), vbits = ceil(log2 V), with the PMA id
+// above it for a group and the request order between key and operation for a
+// merged sequence, so equal keys end up ordered delete before insert, earlier
+// request before later; the dedup that follows keeps the last of each run.
+// Three paths by batch size:
+//   <= 2^17 keys   sample-split sort: two kernels, buckets chosen from a
+//                  sample, each sorted in LDS on the bits that vary in it;
+//   <= 2^22 keys   LSD radix, one kernel per 10/11-bit digit (decoupled
+//                  look-back), the digit counts of every pass taken at pack;
+//   larger         LSD radix, 8-bit digits, histogram + scan + scatter.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -18,9 +23,6 @@ static constexpr int RS_RADIX = 1 << RS_BITS;
 static constexpr int RS_TPB = 256;
 static constexpr int RS_ITEMS_BIG = 16;   // keys per thread per tile, large inputs
 static constexpr int RS_ITEMS_SMALL = 4;  // smaller tiles for more workgroups on mid-size batches
-static constexpr int RS_SMALL_TPB = 1024;
-static constexpr int RS_SMALL_CAP = 8192;  // single-workgroup sort (2 x 64 KiB of LDS)
-static constexpr u32 RS_PREFIX_MAX_TILES = 512;  // scatter derives its offsets itself up to here
 
 // Batch element i comes from the source whose [off, off + n) holds it.
 // Sources are few and sorted by off; each thread finds its first source by
@@ -35,19 +37,16 @@ __device__ __forceinline__ u32 source_of(const BatchSeg *__restrict__ segs, u32 
 }
 
 // Packs a group batch into sort keys pid << pshift | (src << vbits | dst) << 1 | op
-// (keys with src or dst >= V raise FLAG_INVALID_KEY) and, with HIST, counts
-// the first radix digit of every tile.
-template <bool HIST>
+// (keys with src or dst >= V raise FLAG_INVALID_KEY) and counts the first
+// radix digit of every tile (digit-major: hist[d * nblocks + tile]).
 __global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
                                                         const u64 *n_dev, u32 pshift, u32 sbits, u64 *__restrict__ out,
                                                         u32 *__restrict__ flags, u32 shift, u32 items,
-                                                        u32 *__restrict__ hist, u32 nblocks, int block_major) {
+                                                        u32 *__restrict__ hist, u32 nblocks) {
     __shared__ u32 h[RS_RADIX];
     if (n_dev) n = *n_dev;
-    if (HIST) {
-        h[threadIdx.x] = 0;
-        __syncthreads();
-    }
+    h[threadIdx.x] = 0;
+    __syncthreads();
     const u64 base = (u64)blockIdx.x * items * RS_TPB + threadIdx.x;
     bool bad = false;
     if (base < n) {
@@ -62,15 +61,12 @@ __global__ __launch_bounds__(RS_TPB) void k_pack_group(const BatchSeg *__restric
             if (src >= sg.V || dst >= sg.V) { bad = true; src = 0; dst = 0; }
             u64 p = ((u64)sg.pid << pshift) | ((((((u64)src) << sg.vbits) | dst) << sbits | sg.seq) << 1) | sg.op;
             out[i] = p;
-            if (HIST) atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
+            atomicAdd(&h[(p >> shift) & (RS_RADIX - 1)], 1u);
         }
     }
     if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
-    if (HIST) {
-        __syncthreads();
-        if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
-        else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-    }
+    __syncthreads();
+    hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
 // ------------------------------------------------------------ stable ranking
@@ -122,10 +118,9 @@ __device__ __forceinline__ void rank_round(bool valid, u32 digit, RankLds<NT> &s
     __syncthreads();
 }
 
-// per-tile digit histogram; layout block-major (hist[b*RADIX + d]) when
-// `block_major`, digit-major (hist[d*nb + b]) otherwise
+// per-tile digit histogram, digit-major (hist[d * nblocks + tile])
 __global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys, u64 n, const u64 *n_dev, u32 shift,
-                                                     u32 items, u32 *__restrict__ hist, u32 nblocks, int block_major) {
+                                                     u32 items, u32 *__restrict__ hist, u32 nblocks) {
     __shared__ u32 h[RS_RADIX];
     if (n_dev) n = *n_dev;
     h[threadIdx.x] = 0;
@@ -136,36 +131,18 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_hist(const u64 *__restrict__ keys
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_RADIX - 1)], 1u);
     }
     __syncthreads();
-    if (block_major) hist[(u64)blockIdx.x * RS_RADIX + threadIdx.x] = h[threadIdx.x];
-    else hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[(u64)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Scatter of one pass.  With block-major histograms (few tiles) every
-// workgroup derives its digit offsets itself: digit totals and the prefix
-// over earlier tiles from the histogram rows, then a scan over the digits;
-// otherwise hist_scan holds the device-wide exclusive scan (digit-major).
+// Scatter of one pass; hist holds the device-wide exclusive scan of the
+// digit-major histograms (each tile's offset of each digit).
 __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ in, u64 *__restrict__ out, u64 n,
                                                         const u64 *n_dev, u32 shift, u32 items,
-                                                        const u32 *__restrict__ hist, u32 nblocks, int block_major) {
+                                                        const u32 *__restrict__ hist, u32 nblocks) {
     __shared__ RankLds<RS_TPB> s;
     __shared__ u32 gofs[RS_RADIX];
-    __shared__ u32 stmp[RS_TPB / 64 + 1];
     if (n_dev) n = *n_dev;
-    const u32 d = threadIdx.x;
-    if (block_major) {
-        u32 tot = 0, pre = 0;
-#pragma unroll 16
-        for (u32 b = 0; b < nblocks; ++b) {
-            u32 v = hist[(u64)b * RS_RADIX + d];
-            tot += v;
-            pre += b < blockIdx.x ? v : 0u;
-        }
-        u32 all;
-        u32 dbase = block_excl_scan<RS_TPB, u32>(tot, all, stmp);
-        gofs[d] = dbase + pre;
-    } else {
-        gofs[d] = hist[(u64)d * nblocks + blockIdx.x];
-    }
+    gofs[threadIdx.x] = hist[(u64)threadIdx.x * nblocks + blockIdx.x];
     rank_init<RS_TPB>(s);
     __syncthreads();
     u64 base = (u64)blockIdx.x * items * RS_TPB;
@@ -178,48 +155,6 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const u64 *__restrict__ i
         rank_round<RS_TPB>(valid, digit, s, pos);
         if (valid) out[gofs[digit] + pos] = key;
     }
-}
-
-// Whole sort inside one workgroup's LDS for n <= RS_SMALL_CAP.
-__global__ __launch_bounds__(RS_SMALL_TPB) void k_rs_small(u64 *__restrict__ keys, u64 n, const u64 *n_dev,
-                                                             u32 lo_bit, u32 hi_bit) {
-    __shared__ u64 buf[2][RS_SMALL_CAP];
-    __shared__ RankLds<RS_SMALL_TPB> s;
-    __shared__ u32 h[RS_RADIX];
-    if (n_dev) n = *n_dev;
-    for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) buf[0][i] = keys[i];
-    int cur = 0;
-    for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
-        for (u32 i = threadIdx.x; i < RS_RADIX; i += RS_SMALL_TPB) h[i] = 0;
-        rank_init<RS_SMALL_TPB>(s);
-        __syncthreads();
-        for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB)
-            atomicAdd(&h[(buf[cur][i] >> shift) & (RS_RADIX - 1)], 1u);
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            // one wave scans the 256 digit counts
-            u32 v[RS_RADIX / 64];
-            u32 sum = 0;
-#pragma unroll
-            for (int r = 0; r < RS_RADIX / 64; ++r) { v[r] = h[threadIdx.x * (RS_RADIX / 64) + r]; sum += v[r]; }
-            u32 ex = wave_incl_scan(sum) - sum;
-#pragma unroll
-            for (int r = 0; r < RS_RADIX / 64; ++r) { h[threadIdx.x * (RS_RADIX / 64) + r] = ex; ex += v[r]; }
-        }
-        __syncthreads();
-        for (u32 r = 0; r * RS_SMALL_TPB < n; ++r) {
-            u32 i = r * RS_SMALL_TPB + threadIdx.x;
-            bool valid = i < n;
-            u64 key = valid ? buf[cur][i] : 0;
-            u32 digit = (u32)(key >> shift) & (RS_RADIX - 1);
-            u32 pos = 0;
-            rank_round<RS_SMALL_TPB>(valid, digit, s, pos);
-            if (valid) buf[cur ^ 1][h[digit] + pos] = key;
-        }
-        __syncthreads();
-        cur ^= 1;
-    }
-    for (u32 i = threadIdx.x; i < n; i += RS_SMALL_TPB) keys[i] = buf[cur][i];
 }
 
 // ------------------------------------------------- one kernel per pass
@@ -455,6 +390,396 @@ __global__ __launch_bounds__(OS_TPB) void k_os_pass(const u64 *__restrict__ in, 
     }
 }
 
+// ------------------------------------------------------- sample-split sort
+//
+// Batches of up to SS_MAX_N keys sort in two kernels, whatever their key
+// width.  k_ss_split: every workgroup draws the same regular sample of the
+// batch (SS_SAMPLE packed keys), sorts it in LDS by counting, and takes its
+// quantiles as the splitters of nb buckets (rounded down to a key boundary,
+// so that equal keys of any request order and operation share a bucket);
+// then it packs its tile of SS_TILE keys, assigns each to its bucket, and
+// writes the tile back grouped by bucket (tile-major: no global offsets are
+// needed), with its per-bucket counts.  k_ss_bucket: one workgroup per
+// bucket gathers the bucket's keys from every tile into registers, sorts
+// them with a bitonic network (shuffles inside a wave, LDS across waves) and
+// writes them at the bucket's offset.  A sample-chosen split keeps buckets
+// even under R-MAT skew, where a split on the top key bits puts a hub's keys
+// in one bucket.  A bucket beyond the workgroup's SB_CAP keys (many equal
+// keys) is LSD-radix-sorted through global memory, chunk by chunk, on the
+// bits on which its keys differ (AND / OR of the keys), with wave64 ranking
+// (the k_os_pass scheme).
+
+static constexpr int SS_TPB = 256;
+static constexpr int SS_ITEMS = 4;
+static constexpr u32 SS_TILE = SS_TPB * SS_ITEMS;  // keys per tile
+static constexpr u32 SS_SAMPLE = SS_TPB;            // sample keys (one per thread)
+static constexpr u32 SS_MAX_BUCKETS = 128;
+static constexpr u64 SS_MAX_N = 1ull << 17;
+static constexpr u32 SS_MAX_TILES = (u32)(SS_MAX_N / SS_TILE);
+static constexpr int SB_TPB = 1024;                 // bucket sort workgroup
+static constexpr int SB_ITEMS = 4;
+static constexpr u32 SB_CAP = SB_TPB * SB_ITEMS;    // keys sorted in LDS
+static constexpr u32 SB_NW = SB_TPB / 64;
+
+// buckets of a batch of n keys: about a thousand keys each, a power of two
+static u32 ss_buckets(u64 n) {
+    u32 nb = 1;
+    while (nb < SS_MAX_BUCKETS && (u64)nb * SS_TILE < n) nb <<= 1;
+    return nb;
+}
+
+__device__ __forceinline__ u64 pack_element(const BatchSeg &sg, u64 i, u32 pshift, u32 sbits, bool &bad) {
+    u64 k = sg.keys[i - sg.off];
+    u32 src = (u32)(k >> 32), dst = (u32)k;
+    if (src >= sg.V || dst >= sg.V) { bad = true; src = 0; dst = 0; }
+    return ((u64)sg.pid << pshift) | ((((((u64)src) << sg.vbits) | dst) << sbits | sg.seq) << 1) | sg.op;
+}
+
+// Splitters and bucket choice work on 32-bit projections of the packed
+// keys, x >> psh with psh >= the bits below the key (request order,
+// operation): equal keys project equally, so they share a bucket, and any
+// non-decreasing splitters give a correct split.
+__global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
+                                                      const u64 *n_dev, u32 pshift, u32 sbits, u32 psh, u32 nb,
+                                                      u64 *__restrict__ tmp, u32 *__restrict__ flags,
+                                                      u32 *__restrict__ bc) {
+    __shared__ u64 smp[SS_SAMPLE];  // projection << 32 | sample index: distinct
+    __shared__ u32 spl[SS_MAX_BUCKETS];
+    __shared__ u32 cnt[SS_MAX_BUCKETS];
+    __shared__ u32 base[SS_MAX_BUCKETS];
+    if (n_dev) n = *n_dev;
+    const u32 t = blockIdx.x;
+    const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
+    if (t >= ntiles) return;  // whole workgroup: grids sized for a capacity
+    const u32 tid = threadIdx.x;
+
+    // the tile's keys and the sample key are loaded first, all in flight
+    // while the sample is ranked
+    bool bad = false;
+    const u64 tbase = (u64)t * SS_TILE;
+    u64 key[SS_ITEMS];
+    bool valid[SS_ITEMS];
+    {
+        const u64 i0 = tbase + tid;
+        u32 s = source_of(segs, nsegs, i0 < n ? i0 : n - 1);
+        BatchSeg sg = segs[s];
+#pragma unroll
+        for (int r = 0; r < SS_ITEMS; ++r) {
+            const u64 i = i0 + (u64)r * SS_TPB;
+            valid[r] = i < n;
+            key[r] = 0;
+            if (valid[r]) {
+                while (i >= sg.off + sg.n) sg = segs[++s];
+                key[r] = pack_element(sg, i, pshift, sbits, bad);
+            }
+        }
+    }
+    {
+        const u64 p = ((2ull * tid + 1) * n) / (2ull * SS_SAMPLE);
+        bool ignore = false;
+        const u64 x = pack_element(segs[source_of(segs, nsegs, p)], p, pshift, sbits, ignore);
+        smp[tid] = ((u64)(u32)(x >> psh) << 32) | tid;
+    }
+    if (tid < SS_MAX_BUCKETS) cnt[tid] = 0;
+    __syncthreads();
+    // rank of each (distinct) sample entry by counting, two entries per LDS read
+    {
+        const u64 x = smp[tid];
+        const ulonglong2 *s2 = (const ulonglong2 *)smp;
+        u32 r = 0;
+#pragma unroll 16
+        for (u32 q = 0; q < SS_SAMPLE / 2; ++q) {
+            const ulonglong2 y = s2[q];
+            r += (y.x < x ? 1u : 0u) + (y.y < x ? 1u : 0u);
+        }
+        __syncthreads();
+        smp[r] = x;
+    }
+    __syncthreads();
+    if (tid + 1 < nb) spl[tid] = (u32)(smp[(tid + 1) * (SS_SAMPLE / nb)] >> 32);
+    __syncthreads();
+
+    // bucket = number of splitters <= the key's projection
+    u32 bk[SS_ITEMS], loc[SS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SS_ITEMS; ++r) {
+        const u32 x = (u32)(key[r] >> psh);
+        u32 lo = 0, hi = nb - 1;
+        while (lo < hi) {
+            const u32 mid = (lo + hi) >> 1;
+            if (spl[mid] <= x) lo = mid + 1; else hi = mid;
+        }
+        bk[r] = lo;
+        loc[r] = valid[r] ? atomicAdd(&cnt[lo], 1u) : 0u;
+    }
+    if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
+    __syncthreads();
+    if (tid < 64) {
+        // exclusive scan of the bucket counts (nb <= 128: two per lane)
+        const u32 c0 = 2 * tid < nb ? cnt[2 * tid] : 0u, c1 = 2 * tid + 1 < nb ? cnt[2 * tid + 1] : 0u;
+        const u32 ex = wave_incl_scan(c0 + c1) - (c0 + c1);
+        if (2 * tid < nb) base[2 * tid] = ex;
+        if (2 * tid + 1 < nb) base[2 * tid + 1] = ex + c0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SS_ITEMS; ++r)
+        if (valid[r]) tmp[tbase + base[bk[r]] + loc[r]] = key[r];
+    if (tid < nb) bc[(u64)tid * ntiles + t] = (base[tid] << 16) | cnt[tid];
+}
+
+// One LSD pass over the (up to SB_CAP) keys x[] of the workgroup, digit
+// (x >> shift) & 255: returns each key's rank inside the chunk (stable:
+// wave w holds keys w*256 + r*64 + lane) in rk[] and leaves in dofs[d] the
+// chunk's exclusive count of smaller digits, in whist[w][d] the keys of digit
+// d in waves before w.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 shift, u32 (*whist)[256], u32 *dofs,
+                                        u32 *stmp, u32 (&rk)[SB_ITEMS]) {
+    const u32 lane = lane_id();
+    const u32 w = threadIdx.x >> 6;
+    const u64 lt = (1ull << lane) - 1ull;
+    for (u32 i = threadIdx.x; i < SB_NW * 256; i += SB_TPB) (&whist[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SB_ITEMS; ++r) {
+        if (w * (64 * SB_ITEMS) + r * 64 >= m) break;  // the rest of this wave's rounds are empty
+        const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+        const bool valid = j < m;
+        const u32 dg = (u32)(x[r] >> shift) & 255u;
+        u64 peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const u64 bb = __ballot((dg >> b) & 1u);
+            peers &= ((dg >> b) & 1u) ? bb : ~bb;
+        }
+        const u32 below = __popcll(peers & lt);
+        const u32 before = whist[w][dg];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) whist[w][dg] = before + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rk[r] = before + below;
+    }
+    __syncthreads();
+    // the first four waves: digit d = thread; its count per wave turns into
+    // the prefix of the waves before, its total into a wave-wide DPP scan
+    u32 total = 0, inc = 0;
+    if (threadIdx.x < 256) {
+        const u32 d = threadIdx.x;
+        u32 c[SB_NW];
+#pragma unroll
+        for (u32 v = 0; v < SB_NW; ++v) c[v] = whist[v][d];
+#pragma unroll
+        for (u32 v = 0; v < SB_NW; ++v) {
+            whist[v][d] = total;
+            total += c[v];
+        }
+        inc = dpp_incl_scan32(total);
+        if (lane == 63) stmp[w] = inc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        u32 pre = 0;
+        for (u32 v = 0; v < w; ++v) pre += stmp[v];
+        dofs[threadIdx.x] = pre + inc - total;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tmp, u64 *__restrict__ out,
+                                                       u64 *__restrict__ aux, u64 n, const u64 *n_dev, u32 nb,
+                                                       const u32 *__restrict__ bc) {
+    __shared__ u64 kb[1][SB_CAP];
+    __shared__ u32 whist[SB_NW][256];
+    __shared__ u32 dofs[256];
+    __shared__ u32 run[256];
+    __shared__ u32 toff[SS_MAX_TILES + 1], tb[SS_MAX_TILES];
+    __shared__ u32 stmp[SB_NW + 1];
+    __shared__ u64 red[2][SB_NW];
+    (void)kb;
+    if (n_dev) n = *n_dev;
+    const u32 b = blockIdx.x;
+    const u32 tid = threadIdx.x;
+    const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
+    if (b >= nb || ntiles == 0) return;
+    // the bucket's offset: the keys of the buckets before it, whose
+    // per-tile counts are the first b rows of bc
+    u64 obase;
+    {
+        // at most SS_MAX_BUCKETS * SS_MAX_TILES / SB_TPB = 16 words a thread,
+        // all loads in flight at once
+        constexpr u32 PER = SS_MAX_BUCKETS * SS_MAX_TILES / SB_TPB;
+        const u32 lim = b * ntiles;
+        u32 part[PER];
+#pragma unroll
+        for (u32 q = 0; q < PER; ++q) {
+            const u32 i = q * SB_TPB + tid;
+            part[q] = i < lim ? bc[i] & 0xFFFFu : 0u;
+        }
+        u32 sum = 0;
+#pragma unroll
+        for (u32 q = 0; q < PER; ++q) sum += part[q];
+        u32 tot;
+        (void)block_excl_scan<SB_TPB, u32>(sum, tot, stmp);
+        obase = tot;
+    }
+    // where the bucket lies in each tile: offsets inside the bucket
+    u32 m;
+    {
+        const u32 e = tid < ntiles ? bc[(u64)b * ntiles + tid] : 0u;
+        u32 tot;
+        const u32 ex = block_excl_scan<SB_TPB, u32>(e & 0xFFFFu, tot, stmp);
+        if (tid < ntiles) {
+            toff[tid] = ex;
+            tb[tid] = e >> 16;
+        }
+        if (tid == 0) toff[ntiles] = tot;
+        m = tot;
+    }
+    if (m == 0) return;
+    __syncthreads();
+    const u32 lane = lane_id();
+    const u32 w = tid >> 6;
+    auto addr_of = [&](u32 j) -> u64 {
+        u32 lo = 0, hi = ntiles;  // last tile with toff <= j
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if (toff[mid] <= j) lo = mid; else hi = mid;
+        }
+        return (u64)lo * SS_TILE + tb[lo] + (j - toff[lo]);
+    };
+
+    if (m <= SB_CAP) {
+        // gather: element j = w * 256 + r * 64 + lane into x[r] (addresses
+        // first, then the loads), and the bits on which the keys differ
+        u64 x[SB_ITEMS];
+        u32 rk[SB_ITEMS];
+        {
+            u64 ad[SB_ITEMS];
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) {
+                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                ad[r] = j < m ? addr_of(j) : ~0ull;
+            }
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) x[r] = ad[r] != ~0ull ? tmp[ad[r]] : 0ull;
+        }
+        u64 andv = ~0ull, orv = 0;
+#pragma unroll
+        for (int r = 0; r < SB_ITEMS; ++r) {
+            const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+            if (j < m) {
+                andv &= x[r];
+                orv |= x[r];
+            }
+        }
+        andv = wave_and64(andv);
+        orv = wave_or64(orv);
+        if (lane == 0) { red[0][w] = andv; red[1][w] = orv; }
+        __syncthreads();
+        andv = ~0ull;
+        orv = 0;
+#pragma unroll
+        for (u32 q = 0; q < SB_NW; ++q) { andv &= red[0][q]; orv |= red[1][q]; }
+        const u64 diff = andv ^ orv;
+        const u32 lo_bit = diff ? (u32)__builtin_ctzll(diff) : 64u;
+        const u32 hi_bit = diff ? 64u - (u32)__builtin_clzll(diff) : 0u;
+        u64 *kbuf = kb[0];
+        for (u32 shift = lo_bit; shift < hi_bit; shift += 8) {
+            sb_rank(x, m, shift, whist, dofs, stmp, rk);
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) {
+                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                if (j < m) {
+                    const u32 dg = (u32)(x[r] >> shift) & 255u;
+                    kbuf[dofs[dg] + whist[w][dg] + rk[r]] = x[r];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) {
+                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                if (j < m) x[r] = kbuf[j];
+            }
+            // (the next pass's first barrier keeps kbuf until every thread has read it)
+        }
+#pragma unroll
+        for (int r = 0; r < SB_ITEMS; ++r) {
+            const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+            if (j < m) out[obase + j] = x[r];
+        }
+        return;
+    }
+
+    // beyond the LDS (many equal keys): gathered into the output range, then
+    // LSD passes over the bits that vary, chunk by chunk through global
+    // memory (from the output range to aux and back); each digit's running
+    // offset carries across the chunks, which are taken in order (stable)
+    u64 andv = ~0ull, orv = 0;
+    for (u32 j = tid; j < m; j += SB_TPB) {
+        const u64 x = tmp[addr_of(j)];
+        andv &= x;
+        orv |= x;
+        out[obase + j] = x;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        andv &= __shfl_xor(andv, o, 64);
+        orv |= __shfl_xor(orv, o, 64);
+    }
+    if (lane == 0) { red[0][w] = andv; red[1][w] = orv; }
+    __syncthreads();  // also makes the gather's stores visible to the workgroup
+    andv = ~0ull;
+    orv = 0;
+#pragma unroll
+    for (u32 q = 0; q < SB_NW; ++q) { andv &= red[0][q]; orv |= red[1][q]; }
+    const u64 diff = andv ^ orv;
+    const u32 lo_bit = diff ? (u32)__builtin_ctzll(diff) : 64u;
+    const u32 hi_bit = diff ? 64u - (u32)__builtin_clzll(diff) : 0u;
+    u64 *src = out + obase, *dst = aux + obase;
+    for (u32 shift = lo_bit; shift < hi_bit; shift += 8) {
+        // digit totals of the bucket, then their exclusive scan as the start
+        for (u32 i = tid; i < SB_NW * 256; i += SB_TPB) (&whist[0][0])[i] = 0;
+        __syncthreads();
+        for (u32 j = tid; j < m; j += SB_TPB) atomicAdd(&whist[0][(u32)(src[j] >> shift) & 255u], 1u);
+        __syncthreads();
+        {
+            const u32 c = tid < 256 ? whist[0][tid] : 0u;
+            u32 tot;
+            const u32 ex = block_excl_scan<SB_TPB, u32>(c, tot, stmp);
+            if (tid < 256) run[tid] = ex;
+        }
+        __syncthreads();
+        for (u32 c0 = 0; c0 < m; c0 += SB_CAP) {
+            const u32 mc = m - c0 < SB_CAP ? m - c0 : SB_CAP;
+            u64 x[SB_ITEMS];
+            u32 rk[SB_ITEMS];
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) {
+                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                x[r] = j < mc ? src[c0 + j] : 0ull;
+            }
+            sb_rank(x, mc, shift, whist, dofs, stmp, rk);
+#pragma unroll
+            for (int r = 0; r < SB_ITEMS; ++r) {
+                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                if (j < mc) {
+                    const u32 dg = (u32)(x[r] >> shift) & 255u;
+                    dst[run[dg] + whist[w][dg] + rk[r]] = x[r];
+                }
+            }
+            __syncthreads();
+            // the chunk's count of digit d: the next digit's chunk offset
+            // minus this one's
+            if (tid < 256) run[tid] += (tid + 1 < 256 ? dofs[tid + 1] : mc) - dofs[tid];
+            __syncthreads();
+        }
+        u64 *t2 = src; src = dst; dst = t2;
+        __syncthreads();
+    }
+    if (src != out + obase)
+        for (u32 j = tid; j < m; j += SB_TPB) out[obase + j] = src[j];
+}
+
 // ------------------------------------------------------ device-wide scan
 
 static constexpr int SC_TPB = 256;
@@ -581,13 +906,19 @@ static void os_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u3
 // sort_zero_words() words of `scratch` must be zero on entry (the batch's
 // first kernel clears them).
 u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32 pshift, u32 sbits, u32 lo_bit,
-                   u32 hi_bit, u64 *k0, u64 *k1, u32 *scratch, u32 *flags, hipStream_t st) {
+                   u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, u32 *flags, hipStream_t st) {
     if (n == 0) return k0;
-    if (n <= (u64)RS_SMALL_CAP) {
-        const u32 items = 4;
-        k_pack_group<false><<<(u32)((n + items * RS_TPB - 1) / (items * RS_TPB)), RS_TPB, 0, st>>>(
-            segs, nsegs, n, n_dev, pshift, sbits, k0, flags, 0, items, nullptr, 0, 0);
-        if (n > 1 || n_dev) k_rs_small<<<1, RS_SMALL_TPB, 0, st>>>(k0, n, n_dev, lo_bit, hi_bit);
+    if (n <= SS_MAX_N) {
+        // sample-split sort: k1 holds the tiles grouped by bucket, k0 the
+        // result, k2 the second buffer of oversized buckets; the tiles'
+        // bucket counts go to the scratch after its cleared words
+        const u32 nb = ss_buckets(n);
+        const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
+        u32 *bc = scratch + OS_GH_WORDS;
+        const u32 ks = 1 + sbits;
+        const u32 psh = hi_bit > 32 + ks ? hi_bit - 32 : ks;
+        k_ss_split<<<ntiles, SS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, psh, nb, k1, flags, bc);
+        k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, n_dev, nb, bc);
         return k0;
     }
     if (n <= OS_MAX_N) {
@@ -605,18 +936,17 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
     const u32 items = rs_items(n);
     const u64 tile = (u64)items * RS_TPB;
     const u32 nblk = (u32)((n + tile - 1) / tile);
-    const int block_major = nblk <= RS_PREFIX_MAX_TILES;
     u32 *hist = scratch;
     u32 *sscr = scratch + (u64)RS_RADIX * nblk + 1;
     u64 *src = k0, *dst = k1;
     for (u32 shift = lo_bit; shift < hi_bit; shift += RS_BITS) {
         if (shift == lo_bit)
-            k_pack_group<true><<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, k0, flags, shift, items,
-                                                       hist, nblk, block_major);
+            k_pack_group<<<nblk, RS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, k0, flags, shift, items,
+                                                  hist, nblk);
         else
-            k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, n_dev, shift, items, hist, nblk, block_major);
-        if (!block_major) (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);
-        k_rs_scatter<<<nblk, RS_TPB, 0, st>>>(src, dst, n, n_dev, shift, items, hist, nblk, block_major);
+            k_rs_hist<<<nblk, RS_TPB, 0, st>>>(src, n, n_dev, shift, items, hist, nblk);
+        (void)device_excl_scan(hist, (u64)RS_RADIX * nblk, hist, sscr, st);
+        k_rs_scatter<<<nblk, RS_TPB, 0, st>>>(src, dst, n, n_dev, shift, items, hist, nblk);
         u64 *t = src; src = dst; dst = t;
     }
     return src;

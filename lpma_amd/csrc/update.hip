@@ -240,10 +240,18 @@ __device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start,
     return l;
 }
 
-// Every run leader (first update of a run) picks the lowest level whose
-// window fits after the batch, walking up the count pyramid; updates of the
-// window are counted by galloping over the sorted run keys.  The first and
-// last update of each PMA record its compacted range.
+// Every run leader (first update of a run) climbs the count pyramid from its
+// segment until a window fits after the batch; the window's updates are
+// counted by galloping over the sorted run keys.  Windows are evaluated
+// once: arriving at a window of level >= 1, a run claims it (atomic max of
+// epoch << 32 | ~run); only the first run to arrive climbs on, a later one
+// stops there (its updates lie in that window, which the first run either
+// takes or leaves for a larger window that contains it).  The smallest run
+// index among the arrivals owns the claim; k_leaders keeps a claimed window
+// only if no claimed window of a higher level contains it, so a window that
+// was passed through never survives.  A lone PMA that would hold more keys
+// than it has slots (live + the batch's net delta) is resized without any
+// climb.  The first and last update of each PMA record its compacted range.
 template <bool ONE>
 __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                           const Status *__restrict__ status_in, u32 *__restrict__ rlevel, Status *status) {
@@ -271,33 +279,47 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             const DevState d = geom<ONE>(g, pid);
             const u64 pb = (u64)pid << 32;
             const u32 s = (u32)rk;
+            const u64 nseg = nseg_of(d);
+            const u64 tag = (u64)group_epoch(g) << 32;
             u32 u0 = j, u1 = j + 1;
             u32 l = 0;
-            u64 C = 0;
+            bool stopped = false;
+            if (g.npma == 1) {
+                // more keys than slots: along some run's path every window
+                // overflows its capacity (a full window has a child over its
+                // own), so that run reaches the root and the PMA grows
+                const i64 mroot = (i64)*count_ptr(d, d.H, 0) + (i64)status_in->delta;
+                if (mroot > (i64)(nseg << d.log2S)) l = d.H + 1;
+            }
             for (; l <= d.H; ++l) {
-                u64 w = (u64)s >> l;
-                u64 a = w << l;
+                const u64 w = (u64)s >> l;
+                const u64 a = w << l;
+                if (l > 0) {
+                    const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
+                                              (unsigned long long)(tag | (u64)(~j)));
+                    if ((old & ~0xFFFFFFFFull) == tag) {
+                        stopped = true;
+                        break;
+                    }
+                }
                 // the window grows by its sibling, which holds about as many
                 // runs as the window had: stride guess for the one boundary
                 // that moves
                 const u32 guess = max(1u, u1 - u0);
-                C = *count_ptr(d, l, w);  // independent of the searches: in flight meanwhile
+                const u64 C = *count_ptr(d, l, w);  // independent of the searches: in flight meanwhile
                 u0 = gallop_back(cseg, u0, pb + a, guess);
                 u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l), guess);
-                i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
+                const i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
                 if (window_fits(dn, l, d.H, (u64)m)) break;
             }
             rlevel[j] = l;
-            if (l >= d.H) {
-                local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
-                atomicOr(&g.pstat[pid].flags, local_flags);
-            } else if (l > 0) {
-                // claim the window; the smallest run index wins (atomic max of
-                // ~j).  A single-segment window has one candidate, its own run,
-                // so level 0 needs no claim.
-                atomicMax((unsigned long long *)&d.owner[owner_idx(nseg_of(d), l, (u64)s >> l)],
-                          (unsigned long long)(((u64)group_epoch(g) << 32) | (u64)(~j)));
-                lvbit = 1u << l;
+            if (!stopped) {
+                if (l >= d.H) {
+                    local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
+                    atomicOr(&g.pstat[pid].flags, local_flags);
+                } else if (l > 0) {
+                    lvbit = 1u << l;
+                }
             }
         }
     }

@@ -229,8 +229,10 @@ def test_queries():
 
 @pytest.mark.parametrize("n", [9000, 300000, 700000, 3000000, 5000000, 9000000])
 def test_batch_sizes_cover_sort_paths(n):
-    """Batches that exercise every radix-sort path: one workgroup (<= 8192),
-    self-offset tiles, and tiles offset by a device-wide scan."""
+    """Batches that exercise every sort path: the sample-split sort (up to
+    2^17 keys), the one-kernel-per-digit LSD sort (up to 2^22), and the
+    histogram / scan / scatter passes above (8.4M keys take their 16-key
+    tiles)."""
     say(f"batch of {n}")
     rng = np.random.default_rng(n)
     V = 1 << 16
@@ -310,4 +312,50 @@ def test_update_sequence_equals_sequential_application(nreq):
         t.compare()
         assert np.array_equal(L.to_host_u64(twin.export()[0]), t.ref)
     twin.close()
+    t.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 4097, 65536, 131072, 131073])
+def test_sample_split_sort_sizes(n):
+    """Batch sizes at the sample-split sort's tile and bucket boundaries and
+    just past its limit (2^17 + 1 takes the LSD path)."""
+    say(f"sample-split sizes n={n}")
+    rng = np.random.default_rng(1000 + n)
+    V = 1 << 14
+    t = Triple(V, model=n <= 65536)
+    t.insert(rand_keys(rng, V, 50000))
+    t.insert(rand_keys(rng, V, n))
+    t.compare()
+    t.mixed(rand_keys(rng, V, n), t.ref[rng.integers(0, t.ref.size, n)])
+    t.compare()
+    t.close()
+
+
+def test_sample_split_sort_skew():
+    """Skewed batches: most keys on one hub (a sample split still cuts the
+    hub's keys into buckets), one key repeated 10^5 times (one bucket, no
+    bits vary), and 3 * 10^4 inserts with 3 * 10^4 deletes of one key (an
+    oversized bucket whose keys differ in the operation bit only: the
+    passes through global memory)."""
+    say("sample-split skew")
+    rng = np.random.default_rng(77)
+    V = 1 << 15
+    t = Triple(V)
+    t.insert(rand_keys(rng, V, 40000))
+    t.compare()
+    hub = keys_of(np.full(70000, 123), rng.integers(0, V, 70000))
+    t.insert(np.concatenate([hub, rand_keys(rng, V, 30000)]))
+    t.compare()
+    k = keys_of([777], [4242])
+    t.insert(np.repeat(k, 100000))
+    t.compare()
+    t.mixed(np.repeat(k, 30000), np.repeat(k, 30000))
+    t.compare()
+    t.mixed(np.repeat(t.ref[:5], 8000), np.repeat(t.ref[5:10], 8000))
+    t.compare()
+    # an oversized bucket over many bits: 6000 copies each of 3 keys of
+    # one row beside 50000 spread keys
+    few = keys_of([9, 9, 9], [1, 1 << 14, (1 << 15) - 1])
+    t.mixed(np.concatenate([np.repeat(few, 6000), rand_keys(rng, V, 50000)]), np.repeat(few[:1], 7000))
+    t.compare()
     t.close()
