@@ -156,6 +156,15 @@ int lpma_copy_view(lpma_t *h, uint64_t *d_keys, uint32_t *d_row_offsets, void *s
  * view.edges keys. */
 int lpma_export_compacted(lpma_t *h, uint64_t *d_edge_keys, uint32_t *d_row_offsets, void *stream);
 
+/* lpma_export_compacted without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is a static
+ * device array of about 8 MiB, held on each device the library runs on until
+ * the process ends, so it handles structures of up to 2^21 segments
+ * (LPMA_ERR_TOO_LARGE beyond).  Calls on one device share that array:
+ * calls that overlap on one device race. */
+int lpma_export_compacted_noalloc(lpma_t *h, uint64_t *d_edge_keys, uint32_t *d_row_offsets, void *stream);
+
 /* get_data_rpma (rpma.cuh:1735-1761): all live keys including markers,
  * view.live_keys entries. */
 int lpma_export_data(lpma_t *h, uint64_t *d_keys, void *stream);

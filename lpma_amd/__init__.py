@@ -126,6 +126,7 @@ SIGNATURES = [
     ("lpma_get_view", _int, [_vp, ctypes.POINTER(View)]),
     ("lpma_copy_view", _int, [_vp, _vp, _vp, _vp]),
     ("lpma_export_compacted", _int, [_vp, _vp, _vp, _vp]),
+    ("lpma_export_compacted_noalloc", _int, [_vp, _vp, _vp, _vp]),
     ("lpma_export_data", _int, [_vp, _vp, _vp]),
     ("lpma_to_csr", _int, [_vp, _vp, _vp, _vp]),
     ("lpma_edge_query", _int, [_vp, _vp, _u64, _vp, _vp]),

@@ -81,6 +81,11 @@ hipError_t gen_rmat_device(u32 scale, u64 n, double a, double b, double c, u64 s
 // views.hip
 hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64 *edge_keys, u32 *row_std,
                             u64 *all_keys, u32 *csr_row, u32 *csr_col, hipStream_t st);
+// the export with its scratch in a static device array (no allocation);
+// setup finds the array's address on the current device
+hipError_t export_scratch_setup(int device);
+u64 export_static_max_segments();
+hipError_t export_compacted_static(int device, const DevState &d, u64 *edge_keys, u32 *row_std, hipStream_t st);
 hipError_t edge_query(const DevState &d, const u64 *q, u64 n, u32 *res, hipStream_t st);
 hipError_t neighbor_count(const DevState &d, const u32 *srcs, u64 n, u32 *len, hipStream_t st);
 hipError_t neighbor_gather(const DevState &d, const u32 *srcs, u64 n, const u32 *offs, u32 *out, hipStream_t st);

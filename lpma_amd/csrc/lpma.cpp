@@ -1307,6 +1307,12 @@ int lpma_create(lpma_t **out, uint32_t num_vertices, const lpma_config_t *cfg_in
     h->st.vbits = h->vbits;
 
     int r = LPMA_OK;
+    // the allocation-free export's static scratch on this device
+    if (export_scratch_setup(dev) != hipSuccess) {
+        (void)hipGetLastError();
+        lpma_destroy(h);
+        return LPMA_ERR_DEVICE;
+    }
     if (hipMalloc(&h->d_err, 16) != hipSuccess) {
         (void)hipGetLastError();
         h->d_err = nullptr;
@@ -1453,6 +1459,14 @@ static int export_common(lpma_t *h, u64 *edges, u32 *row_std, u64 *all, u32 *csr
 
 int lpma_export_compacted(lpma_t *h, uint64_t *d_edge_keys, uint32_t *d_row_offsets, void *stream) {
     return export_common(h, d_edge_keys, d_row_offsets, nullptr, nullptr, nullptr, stream);
+}
+
+int lpma_export_compacted_noalloc(lpma_t *h, uint64_t *d_edge_keys, uint32_t *d_row_offsets, void *stream) {
+    if (!h) return LPMA_ERR_INVALID_ARGUMENT;
+    if ((1ull << h->st.H) > export_static_max_segments()) return LPMA_ERR_TOO_LARGE;
+    DevGuard guard(h->device);
+    HIPCHECK(export_compacted_static(h->device, h->st, d_edge_keys, d_row_offsets, (hipStream_t)stream));
+    return LPMA_OK;
 }
 
 int lpma_export_data(lpma_t *h, uint64_t *d_keys, void *stream) {

@@ -1,6 +1,8 @@
 // views.hip -- read paths over the gapped layout: compacted export (the
 // parity surface), the reference's CSR convention, edge-membership queries,
 // neighbor-list reads, and an on-device invariant checker.
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -49,6 +51,47 @@ hipError_t export_compacted(const DevState &d, u32 *gpre, u32 *scan_scratch, u64
     k_export<<<grid_for(nseg << d.log2S, 256, 16384), 256, 0, st>>>(d, gpre, edge_keys, row_std, all_keys, csr_row,
                                                                     csr_col);
     return hipGetLastError();
+}
+
+// ------------------------------------------- export without allocation
+//
+// The compacted export's scratch (the segment-count prefix, nseg + 1 words,
+// and the scan's partials) in a static device array, for callers that may
+// not allocate (graph capture, serving loops).  Every call on a device
+// shares that device's copy: calls that overlap on one device race.  Nothing
+// in it needs clearing: the scan writes every word before it is read.
+
+static constexpr u64 EXPORT_STATIC_MAX_SEGS = 1ull << 21;
+static constexpr u64 EXPORT_STATIC_WORDS =
+    (EXPORT_STATIC_MAX_SEGS + 1 + (EXPORT_STATIC_MAX_SEGS + 2047) / 2048 + 2 + 8 + 63) / 64 * 64;
+static __device__ __align__(256) u32 scratch[EXPORT_STATIC_WORDS];
+
+static constexpr int EXPORT_STATIC_MAX_DEVICES = 64;
+static u32 *g_scratch_ptr[EXPORT_STATIC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `device` (the current device), found
+// once per device; the host-side name of the array is not a device address.
+hipError_t export_scratch_setup(int device) {
+    if (device < 0 || device >= EXPORT_STATIC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = (u32 *)p;
+    return hipSuccess;
+}
+
+u64 export_static_max_segments() { return EXPORT_STATIC_MAX_SEGS; }
+
+hipError_t export_compacted_static(int device, const DevState &d, u64 *edge_keys, u32 *row_std, hipStream_t st) {
+    if (nseg_of(d) > EXPORT_STATIC_MAX_SEGS) return hipErrorInvalidValue;
+    hipError_t e = export_scratch_setup(device);
+    if (e != hipSuccess) return e;
+    u32 *gpre = g_scratch_ptr[device];
+    u32 *scan = gpre + EXPORT_STATIC_MAX_SEGS + 1;
+    return export_compacted(d, gpre, scan, edge_keys, row_std, nullptr, nullptr, nullptr, st);
 }
 
 // Exact edge membership (edge_query_rpma / edge_query_kernel,
