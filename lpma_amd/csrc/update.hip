@@ -240,16 +240,21 @@ __device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start,
     return l;
 }
 
+// levels climbed with galloping searches before the lock-step searches
+static constexpr u32 DENSITY_GALLOP_LEVELS = 8;
+// updates (of the window below) from which a window is claimed on arrival
+static constexpr u32 DENSITY_CLAIM_ON_ARRIVAL = 64;
+
 // Every run leader (first update of a run) climbs the count pyramid from its
-// segment until a window fits after the batch; the window's updates are
-// counted by galloping over the sorted run keys.  Windows are evaluated
-// once: arriving at a window of level >= 1, a run claims it (atomic max of
-// epoch << 32 | ~run); only the first run to arrive climbs on, a later one
-// stops there (its updates lie in that window, which the first run either
-// takes or leaves for a larger window that contains it).  The smallest run
-// index among the arrivals owns the claim; k_leaders keeps a claimed window
-// only if no claimed window of a higher level contains it, so a window that
-// was passed through never survives.  A lone PMA that would hold more keys
+// segment until a window fits after the batch, and claims it (atomic max of
+// epoch << 32 | ~run: the smallest run index wins); the window's updates are
+// counted by galloping over the sorted run keys.  Large windows (many
+// updates below them) are evaluated once: a run arriving there claims it
+// first, and only the first run to arrive climbs on; a later one stops there
+// (its updates lie in that window, which the first run either takes or
+// leaves for a larger window that contains it).  k_leaders keeps a claimed
+// window only if no claimed window of a higher level contains it, so a
+// window that was passed through never survives.  A lone PMA that would hold more keys
 // than it has slots (live + the batch's net delta) is resized without any
 // climb.  The first and last update of each PMA record its compacted range.
 template <bool ONE>
@@ -291,10 +296,15 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
                 const i64 mroot = (i64)*count_ptr(d, d.H, 0) + (i64)status_in->delta;
                 if (mroot > (i64)(nseg << d.log2S)) l = d.H + 1;
             }
-            for (; l <= d.H; ++l) {
+            bool fit = false;
+            for (; l <= d.H && l <= DENSITY_GALLOP_LEVELS; ++l) {
                 const u64 w = (u64)s >> l;
                 const u64 a = w << l;
-                if (l > 0) {
+                // windows with many updates are claimed on arrival (only the
+                // first run to arrive climbs on); small ones are evaluated by
+                // every run that reaches them and claimed where they fit
+                const bool on_arrival = l > 0 && u1 - u0 >= DENSITY_CLAIM_ON_ARRIVAL;
+                if (on_arrival) {
                     const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
                                               (unsigned long long)(tag | (u64)(~j)));
                     if ((old & ~0xFFFFFFFFull) == tag) {
@@ -310,33 +320,105 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
                 u0 = gallop_back(cseg, u0, pb + a, guess);
                 u1 = gallop_fwd(cseg, u1, n_eff, pb + a + (1ull << l), guess);
                 const i64 m = (i64)C + (i64)(pd[u1] - pd[u0]);
-                if (window_fits(dn, l, d.H, (u64)m)) break;
+                if (window_fits(dn, l, d.H, (u64)m)) {
+                    if (l > 0 && l < d.H && !on_arrival)
+                        atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
+                                  (unsigned long long)(tag | (u64)(~j)));
+                    fit = true;
+                    break;
+                }
+            }
+            // Higher up (rare: a run still climbing there leads a large
+            // region that is full or sparse), the bounds of several levels'
+            // windows are found together: their binary searches step in lock
+            // step, so the dependent loads are one search deep, not one per
+            // level; the claims still go level by level.
+            while (!stopped && !fit && l <= d.H) {
+                constexpr u32 G = 8;
+                u32 lo0[G], hi0[G], lo1[G], hi1[G];
+                u64 t0[G], t1[G], Cl[G];
+                const u32 ng = min(G, d.H + 1 - l);
+#pragma unroll
+                for (u32 q = 0; q < G; ++q) {
+                    const u32 lq = l + q;
+                    const u64 a = ((u64)s >> lq) << lq;
+                    t0[q] = pb + a;
+                    t1[q] = pb + a + (1ull << lq);
+                    lo0[q] = 0; hi0[q] = q < ng ? u0 : 0u;      // the start lies at or before u0
+                    lo1[q] = u1; hi1[q] = q < ng ? n_eff : u1;  // the end at or after u1
+                    Cl[q] = q < ng ? *count_ptr(d, lq, (u64)s >> lq) : 0ull;
+                }
+                bool more = true;
+                while (more) {
+                    more = false;
+                    u64 v0[G], v1[G];
+#pragma unroll
+                    for (u32 q = 0; q < G; ++q) {
+                        v0[q] = lo0[q] < hi0[q] ? cseg[(lo0[q] + hi0[q]) >> 1] : 0ull;
+                        v1[q] = lo1[q] < hi1[q] ? cseg[(lo1[q] + hi1[q]) >> 1] : 0ull;
+                    }
+#pragma unroll
+                    for (u32 q = 0; q < G; ++q) {
+                        if (lo0[q] < hi0[q]) {
+                            const u32 mid = (lo0[q] + hi0[q]) >> 1;
+                            if (v0[q] < t0[q]) lo0[q] = mid + 1; else hi0[q] = mid;
+                        }
+                        if (lo1[q] < hi1[q]) {
+                            const u32 mid = (lo1[q] + hi1[q]) >> 1;
+                            if (v1[q] < t1[q]) lo1[q] = mid + 1; else hi1[q] = mid;
+                        }
+                        more |= lo0[q] < hi0[q] || lo1[q] < hi1[q];
+                    }
+                }
+                i32 p0[G], p1[G];
+#pragma unroll
+                for (u32 q = 0; q < G; ++q) {
+                    p0[q] = q < ng ? pd[lo0[q]] : 0;
+                    p1[q] = q < ng ? pd[lo1[q]] : 0;
+                }
+                for (u32 q = 0; q < ng; ++q, ++l) {
+                    const u64 w = (u64)s >> l;
+                    const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
+                                              (unsigned long long)(tag | (u64)(~j)));
+                    if ((old & ~0xFFFFFFFFull) == tag) {
+                        stopped = true;
+                        break;
+                    }
+                    u0 = lo0[q];
+                    u1 = lo1[q];
+                    const i64 m = (i64)Cl[q] + (i64)(p1[q] - p0[q]);
+                    if (window_fits(dn, l, d.H, (u64)m)) {
+                        fit = true;
+                        break;
+                    }
+                }
             }
             rlevel[j] = l;
             if (!stopped) {
-                if (l >= d.H) {
-                    local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
-                    atomicOr(&g.pstat[pid].flags, local_flags);
-                } else if (l > 0) {
-                    lvbit = 1u << l;
-                }
+                if (l >= d.H) local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
+                else if (l > 0) lvbit = 1u << l;
             }
         }
     }
-    // one atomic per wave for the flags; the claimed levels per PMA: lanes of
-    // the first PMA in the wave combine theirs, the others (rare) go alone
-    u32 f = local_flags;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
+    // one atomic per wave for the flags (a batch that resizes raises them in
+    // every run: atomics on one word from each run would serialise); per PMA
+    // the flags and the claimed levels: lanes of the first PMA in the wave
+    // combine theirs, the others (rare) go alone (the levels into the
+    // workgroup's replica of the PMA's mask)
+    const u32 f = dpp_reduce32<false>(local_flags);
     if (lane_id() == 0 && f) atomicOr(&status->flags, f);
-    // (into the workgroup's replica of the PMA's mask)
     const u32 pid0 = __shfl(pid, 0, 64);
     const u32 rep = blockIdx.x & (LV_REPL - 1);
-    u32 lv = pid == pid0 ? lvbit : 0u;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) lv |= __shfl_xor(lv, o, 64);
-    if (lane_id() == 0 && lv) atomicOr(&g.lvmask[(u64)pid0 * LV_REPL + rep], lv);
-    if (pid != pid0 && lvbit) atomicOr(&g.lvmask[(u64)pid * LV_REPL + rep], lvbit);
+    const u32 lv = dpp_reduce32<false>(pid == pid0 ? lvbit : 0u);
+    const u32 pf = dpp_reduce32<false>(pid == pid0 ? local_flags : 0u);
+    if (lane_id() == 0 && pid0 != 0xFFFFFFFFu) {
+        if (lv) atomicOr(&g.lvmask[(u64)pid0 * LV_REPL + rep], lv);
+        if (pf) atomicOr(&g.pstat[pid0].flags, pf);
+    }
+    if (pid != pid0 && pid != 0xFFFFFFFFu) {
+        if (lvbit) atomicOr(&g.lvmask[(u64)pid * LV_REPL + rep], lvbit);
+        if (local_flags) atomicOr(&g.pstat[pid].flags, local_flags);
+    }
 }
 
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
