@@ -1196,63 +1196,157 @@ __device__ __forceinline__ u32 window_of_chunk(const u32 *chunk_base, u32 n, u32
     return lo;
 }
 
-static constexpr int LG_TPB = 256;
-static constexpr u32 LG_SEG_MAX = 1u << LARGE_CHUNK_LOG2_SLOTS;  // segments per chunk at S = 1
+static constexpr int LG_TPB = 256;     // pull: threads per block
+static constexpr int LGP_TPB = 512;    // push: threads per block
+static constexpr u32 LG_SLOTS = 1u << LARGE_CHUNK_LOG2_SLOTS;
+static constexpr u32 LG_SEG_MAX = LG_SLOTS / 2;  // segments per chunk at S = 2
+static constexpr int LGP_PER = (int)((LG_SLOTS + 2 + LGP_TPB - 1) / LGP_TPB);  // odd: 9
+static_assert(LGP_PER & 1, "an odd run length keeps the scan's LDS accesses off shared banks");
 
-__global__ __launch_bounds__(LG_TPB) void k_large_push(Group g, const Work *__restrict__ work,
-                                                        const u32 *__restrict__ chunk_base, const u64 *__restrict__ mbase,
-                                                        const u32 *__restrict__ hdr, const u64 *__restrict__ ckey,
-                                                        const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
-                                                        u64 *__restrict__ merged) {
+// In-place exclusive scan of arr[0, n) (arr[n] receives the total), n <
+// NT * PER: each thread sums PER consecutive entries, one block scan joins
+// the runs.  Every thread of the block must call it.
+template <int NT, int PER>
+__device__ __forceinline__ void block_scan_runs(u32 *arr, u32 n, u32 *tmp) {
+    const u32 b = threadIdx.x * PER;
+    u32 v[PER];
+    u32 s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = b + k < n ? arr[b + k] : 0u;
+        s += v[k];
+    }
+    u32 tot;
+    u32 ex = block_excl_scan<NT, u32>(s, tot, tmp);  // its barrier orders the reads before the writes
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (b + k < n) arr[b + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) arr[n] = tot;
+    __syncthreads();
+}
+
+// First index in [lo, hi) with a[idx] >= x (hi if none), found by one whole
+// wave: each step tests 64 evenly spaced pivots at once, so the chain of
+// dependent loads is log64 of the range, not log2.
+__device__ __forceinline__ u32 wave_lower_bound(const u64 *__restrict__ a, u32 lo, u32 hi, u64 x) {
+    const u32 lane = lane_id();
+    while (lo < hi) {
+        const u32 step = (hi - lo + 63) / 64;
+        const u32 p = lo + lane * step;
+        const u64 below = __ballot(p < hi && a[p] < x);  // a prefix of the lanes: a is sorted
+        const u32 c = (u32)__popcll(below);
+        if (c == 0) return lo;
+        // a[lo + (c-1)*step] < x, and a[lo + c*step] >= x unless that is past hi
+        const u32 nlo = lo + (c - 1) * step + 1;
+        hi = min(lo + c * step, hi);
+        lo = nlo;
+    }
+    return lo;
+}
+
+// Push of one chunk (2^cl segments, at most 4096 slots) into the window's
+// merged slice.  The chunk's slots are staged in LDS; each update of the
+// chunk finds its rank among the chunk's live keys there and records what it
+// does to the keys after it (+1 at its rank for an insert, -1 after the key
+// it deletes, which is flagged).  One scan over the ranks then gives every
+// live key its shift, so each key costs a coalesced load and store and no
+// search.  Inserted keys are stored by their updates.
+__global__ __launch_bounds__(LGP_TPB) void k_large_push(Group g, const Work *__restrict__ work,
+                                                         const u32 *__restrict__ chunk_base,
+                                                         const u64 *__restrict__ mbase, const u32 *__restrict__ hdr,
+                                                         const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
+                                                         const i32 *__restrict__ pd, u64 *__restrict__ merged) {
     if (g.rh) merged = g.rh->merged;
-    __shared__ u32 pre[LG_SEG_MAX + 1];
-    __shared__ u32 stmp[LG_TPB / 64 + 1];
+    __shared__ u64 ks[LG_SLOTS];           // the chunk's slots
+    __shared__ u32 pre[LG_SEG_MAX + 1];    // live keys per segment, then their exclusive scan
+    __shared__ u32 sh[LG_SLOTS + 2];       // per rank: net updates that first move it, then their scan
+    __shared__ u32 del[LG_SLOTS / 32 + 1];  // ranks whose key is deleted (rank <= LG_SLOTS)
+    __shared__ u32 stmp[LGP_TPB / 64 + 1];
     __shared__ u32 qr[2];
+    __shared__ u64 ob;
     const u32 nwin = hdr[0], nch = hdr[1];
     const u32 log2S = g.log2S;
     const u32 Smask = (1u << log2S) - 1;
+    const u32 wv = threadIdx.x >> 6;
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
         const DevState d = pma_of(g, wk.pid);
         const u64 pb = (u64)wk.pid << 32;
         const u32 cl = chunk_level(log2S, wk.L);
-        const u64 j = c - chunk_base[i];
-        const u64 s0 = (u64)wk.a + (j << cl);
+        const u64 s0 = (u64)wk.a + ((u64)(c - chunk_base[i]) << cl);
         const u32 nsc = 1u << cl;
-        const u64 obase = rank_in_window(d, wk.a, wk.L, s0, cl);
-        const u64 mb = mbase[i];
-        const i32 pbase = pd[wk.u0];
-        for (u32 x = threadIdx.x; x < nsc; x += LG_TPB) pre[x] = d.cnt[s0 + x];
-        if (threadIdx.x == 0) {
-            qr[0] = (u32)lower_bound_dev<u64>(cseg, wk.u0, wk.u1, pb + s0);
-            qr[1] = (u32)lower_bound_dev<u64>(cseg, wk.u0, wk.u1, pb + s0 + nsc);
-        }
-        __syncthreads();
-        block_scan_array<LG_TPB>(pre, nsc, stmp);
+        const u32 W = nsc << log2S;  // <= LG_SLOTS (S <= 1024)
         const u64 slot0 = s0 << log2S;
-        const u32 W = nsc << log2S;
-        for (u32 t = threadIdx.x; t < W; t += LG_TPB) {
-            u32 seg = t >> log2S, lane = t & Smask;
-            u32 p0 = pre[seg];
-            if (lane >= pre[seg + 1] - p0) continue;
-            u64 x = d.keys[slot0 + t];
-            // updates outside the chunk's segments are all below or all
-            // above its keys: search only the chunk's
-            u64 u = lower_bound_dev<u64>(ckey, qr[0], qr[1], x);
-            if (u < qr[1] && ckey[u] == x) continue;  // deleted
-            u64 o = obase + p0 + lane + (i64)(pd[u] - pbase);
-            if (o < wk.M) merged[mb + o] = x;
+        const u64 mb = mbase[i];
+        // the loads of this phase are independent of each other
+        u64 kv[LG_SLOTS / LGP_TPB];
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+            const u32 t = threadIdx.x + k * LGP_TPB;
+            kv[k] = t < W ? d.keys[slot0 + t] : KEY_NONE;
         }
-        for (u32 q = qr[0] + threadIdx.x; q < qr[1]; q += LG_TPB) {
-            if (pd[q + 1] - pd[q] != 1) continue;
-            u64 y = ckey[q];
-            u64 sg = (u32)cseg[q];
-            u64 o = obase + pre[sg - s0] + lower_bound_dev<u64>(d.keys + (sg << log2S), 0, d.cnt[sg], y) +
-                    (i64)(pd[q] - pbase);
-            if (o < wk.M) merged[mb + o] = y;
+        for (u32 x = threadIdx.x; x < nsc; x += LGP_TPB) pre[x] = d.cnt[s0 + x];
+        if (wv < 2) {
+            // the chunk's updates: [first >= its first segment, first >= the next chunk's)
+            const u32 q = wave_lower_bound(cseg, wk.u0, wk.u1, pb + s0 + (wv ? nsc : 0u));
+            if (lane_id() == 0) qr[wv] = q;
+        } else if (wv == 2) {
+            // rank of the chunk's first key in its window: the live counts of
+            // the aligned blocks left of it, one level per lane
+            const u64 rel = s0 - wk.a;
+            const u32 l = cl + lane_id();
+            u64 v = 0;
+            if (l < wk.L && ((rel >> l) & 1ull))
+                v = *count_ptr(d, l, (wk.a + ((rel >> (l + 1)) << (l + 1))) >> l);
+            v = wave_sum(v);
+            if (lane_id() == 0) ob = v;
+        }
+        for (u32 t = threadIdx.x; t < W + 2; t += LGP_TPB) sh[t] = 0;
+        for (u32 t = threadIdx.x; t < LG_SLOTS / 32 + 1; t += LGP_TPB) del[t] = 0;
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+            const u32 t = threadIdx.x + k * LGP_TPB;
+            if (t < W) ks[t] = kv[k];
         }
         __syncthreads();
+        block_scan_runs<LGP_TPB, LGP_PER>(pre, nsc, stmp);  // pre[nsc] = live keys of the chunk
+        const u32 q0 = qr[0], q1 = qr[1];
+        const u64 obase = ob;
+        const i32 pbase = pd[wk.u0];
+        for (u32 q = q0 + threadIdx.x; q < q1; q += LGP_TPB) {
+            const u64 y = ckey[q];
+            const u32 ls = (u32)cseg[q] - (u32)s0;
+            const i32 dl = pd[q + 1] - pd[q];  // +1 insert (y absent), -1 delete (y present)
+            const u32 p0 = pre[ls];
+            const u32 r = p0 + (u32)lower_bound_dev<u64>(ks + (ls << log2S), 0, pre[ls + 1] - p0, y);
+            if (dl > 0) {
+                atomicAdd(&sh[r], 1u);
+                const u64 o = obase + r + (u64)(i64)(pd[q] - pbase);
+                if (o < wk.M) merged[mb + o] = y;
+            } else {
+                atomicAdd(&sh[r + 1], 0xFFFFFFFFu);
+                atomicOr(&del[r >> 5], 1u << (r & 31));
+            }
+        }
+        __syncthreads();
+        block_scan_runs<LGP_TPB, LGP_PER>(sh, pre[nsc] + 1, stmp);  // sh[r + 1]: net updates below rank r's key
+        const i64 cbase = (i64)(pd[q0] - pbase);                   // the window's updates before the chunk
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+            const u32 t = threadIdx.x + k * LGP_TPB;
+            if (t >= W) break;
+            const u32 seg = t >> log2S, lane = t & Smask;
+            const u32 p0 = pre[seg];
+            if (lane >= pre[seg + 1] - p0) continue;
+            const u32 r = p0 + lane;
+            if ((del[r >> 5] >> (r & 31)) & 1u) continue;
+            const u64 o = obase + r + (u64)(cbase + (i64)(i32)sh[r + 1]);
+            if (o < wk.M) merged[mb + o] = kv[k];
+        }
+        __syncthreads();  // the LDS arrays are reused by the next chunk
     }
 }
 
@@ -1279,13 +1373,23 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__re
         const u64 mb = mbase[i];
         const u64 M = wk.M;
         const u64 slot0 = ((u64)wk.a + rel0) << log2S;
-        const u32 W = nsc << log2S;
-        for (u32 t = threadIdx.x; t < W; t += LG_TPB) {
-            u64 seg = rel0 + (t >> log2S);
-            u32 lane = t & Smask;
-            u64 f = spread_first(seg, M, wk.L);
-            u64 g = spread_first(seg + 1, M, wk.L);
-            u64 key = (lane < g - f) ? merged[mb + f + lane] : KEY_NONE;
+        const u32 W = nsc << log2S;  // <= LG_SLOTS
+        // every load of the chunk is issued before the first store
+        u64 kv[LG_SLOTS / LG_TPB];
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LG_TPB; ++k) {
+            const u32 t = threadIdx.x + k * LG_TPB;
+            const u64 seg = rel0 + (t >> log2S);
+            const u32 lane = t & Smask;
+            const u64 f = spread_first(seg, M, wk.L);
+            const u64 g = spread_first(seg + 1, M, wk.L);
+            kv[k] = (t < W && lane < g - f) ? merged[mb + f + lane] : KEY_NONE;
+        }
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LG_TPB; ++k) {
+            const u32 t = threadIdx.x + k * LG_TPB;
+            if (t >= W) break;
+            const u64 key = kv[k];
             d.keys[slot0 + t] = key;
             if (is_marker(key) && (u32)(key >> 32) < d.V) {
                 d.row_off[(u32)(key >> 32) + 1] = (u32)(slot0 + t);
@@ -1326,7 +1430,8 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__re
 
 hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st) {
     k_large_prep<<<1, 1024, 0, st>>>(g, b.work[CLASS_LARGE], b.status, b.lchunk, b.lmbase, b.lhdr);
-    k_large_push<<<grid, LG_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
+    // push: ~58 KB of LDS per block, two blocks per CU
+    k_large_push<<<min(grid, 512u), LGP_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
                                           merged);
     k_large_pull<<<grid, LG_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, merged);
     return hipGetLastError();
