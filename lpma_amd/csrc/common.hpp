@@ -64,6 +64,12 @@ static constexpr u32 BLOCK_E_CAP = 2048;
 static constexpr u32 BLOCK_SEG_CAP = 1024;
 // the multi-workgroup path cuts windows into chunks of this many slots
 static constexpr u32 LARGE_CHUNK_LOG2_SLOTS = 12;
+// Update index of a lone PMA, rebuilt every batch: for each window of the
+// index level (at most WFIRST_N windows), the first compacted update inside
+// it.  It turns a window's update range above that level into two lookups.
+static constexpr u32 WFIRST_LOG2 = 12;
+static constexpr u32 WFIRST_N = 1u << WFIRST_LOG2;
+__host__ __device__ __forceinline__ unsigned index_level(unsigned H) { return H > WFIRST_LOG2 + 8 ? H - WFIRST_LOG2 : 8; }
 
 // device statistics slots (u64 counters)
 enum StatSlot {

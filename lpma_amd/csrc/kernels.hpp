@@ -27,6 +27,7 @@ struct BatchBufs {
     u32 *lchunk;   // n+1: chunk prefix over large windows
     u64 *lmbase;   // n: merged-scratch base per large window
     u32 *lhdr;     // 4: large windows, chunks
+    u32 *wfirst;   // WFIRST_N + 1: first compacted update per index-level window (lone PMA)
     Status *status;
 };
 
@@ -39,8 +40,9 @@ struct InlineSegs {
 
 u32 locate_blocks(u64 n);
 // also clears zero[0, nzero) (the sort's digit totals)
+// and resets the update index wfirst
 hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, u32 *zero, u32 nzero,
-                       hipStream_t st);
+                       u32 *wfirst, hipStream_t st);
 hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st);
 // fixed grids for graph replays (work counts read on the device)
 hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64 *merged, hipStream_t st);

@@ -155,7 +155,7 @@ struct Pipeline {
     hipStream_t cur_stream = nullptr;  // stream of the call in progress
 
     DBuf<u64> pk0, pk1, ukey, useg, ckey, cseg, merged, lmbase;
-    DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr, lvmask;
+    DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr, lvmask, wfirst;
     DBuf<i32> pd, part;
     DBuf<Work> work[NUM_CLASSES];
     DBuf<PmaEntry> tab;
@@ -381,7 +381,7 @@ void pipeline_free(Pipeline &pl) {
     release_on(d, pl.ckey); release_on(d, pl.cseg); release_on(d, pl.merged); release_on(d, pl.lmbase);
     release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
     release_on(d, pl.lchunk); release_on(d, pl.lhdr); release_on(d, pl.pd); release_on(d, pl.part);
-    release_on(d, pl.lvmask);
+    release_on(d, pl.lvmask); release_on(d, pl.wfirst);
     for (int c = 0; c < NUM_CLASSES; ++c) release_on(d, pl.work[c]);
     release_on(d, pl.tab); release_on(d, pl.segs); release_on(d, pl.out); release_on(d, pl.rhdr);
     for (auto &kv : pl.graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -541,6 +541,7 @@ BatchBufs batch_bufs(Pipeline &pl) {
     b.lchunk = pl.lchunk.p;
     b.lmbase = pl.lmbase.p;
     b.lhdr = pl.lhdr.p;
+    b.wfirst = pl.wfirst.p;
     b.status = status_of(pl);
     return b;
 }
@@ -554,7 +555,7 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
         (r = ensure(pl, pl.useg, n)) || (r = ensure(pl, pl.ckey, n)) || (r = ensure(pl, pl.cseg, n)) ||
         (r = ensure(pl, pl.pd, n + 1)) || (r = ensure(pl, pl.rlevel, n)) || (r = ensure(pl, pl.part, 2 * nb + 2)) ||
         (r = ensure(pl, pl.tab, npma)) || (r = ensure(pl, pl.segs, std::max<u32>(nsegs, INLINE_SEGS))) ||
-        (r = ensure(pl, pl.lvmask, (u64)npma * LV_REPL)) ||
+        (r = ensure(pl, pl.lvmask, (u64)npma * LV_REPL)) || (r = ensure(pl, pl.wfirst, WFIRST_N + 1)) ||
         (r = ensure(pl, pl.out, out_words)))
         return r;
     for (int c = 0; c < NUM_CLASSES; ++c)
@@ -938,7 +939,7 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     u64 *pk;
     {
         PhaseTimer t(pl, st, PH_SORT);
-        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), st));
+        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), bb.wfirst, st));
         pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
                            pl.ukey.p, pl.sort_scr.p, &bb.status->flags, st);
         HIPCHECK(hipGetLastError());
@@ -1074,7 +1075,7 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
         bool ok = hipMemcpyAsync(pl.rhdr.p, pl.h_rhdr, sizeof(RoundHdr), hipMemcpyHostToDevice, cs) == hipSuccess &&
                   hipMemcpyAsync(pl.segs.p, pl.h_segs, nsegs * sizeof(BatchSeg), hipMemcpyHostToDevice, cs) ==
                       hipSuccess &&
-                  batch_begin(g, bb.status, none, pl.segs.p, pl.sort_scr.p, sort_zero_words(), cs) == hipSuccess;
+                  batch_begin(g, bb.status, none, pl.segs.p, pl.sort_scr.p, sort_zero_words(), bb.wfirst, cs) == hipSuccess;
         if (ok) {
             u64 *pk = pack_and_sort(pl.segs.p, nsegs, cap, &pl.rhdr.p->n, pshift, 0, 0, hi_bit, pl.pk0.p, pl.pk1.p,
                                     pl.ukey.p, pl.sort_scr.p, &bb.status->flags, cs);

@@ -40,11 +40,11 @@ u32 locate_blocks(u64 n) { return (u32)((n + LOC_TILE - 1) / LOC_TILE); }
 // k_pyr_chunk (nseg >> 11 entries and fewer), so no separate launch is
 // needed for them.
 __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs, u32 *zero,
-                                                      u32 nzero);
+                                                      u32 nzero, u32 *wfirst);
 
 hipError_t batch_begin(const Group &g, Status *status, const InlineSegs &in, BatchSeg *segs, u32 *zero, u32 nzero,
-                       hipStream_t st) {
-    k_batch_begin<<<g.npma, 1024, 0, st>>>(g, status, in, segs, zero, nzero);
+                       u32 *wfirst, hipStream_t st) {
+    k_batch_begin<<<g.npma, 1024, 0, st>>>(g, status, in, segs, zero, nzero, wfirst);
     return hipGetLastError();
 }
 
@@ -133,11 +133,13 @@ static constexpr u32 COMPACT_SELF_PREFIX_MAX = 2048;  // blocks
 // Stream compaction of the effective updates.  With few blocks every block
 // derives its prefix (and the totals) from the per-block partials itself;
 // otherwise k_scan_parts has already scanned them.
-__global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ ukey, const u64 *__restrict__ useg, u64 n,
-                                                     const RoundHdr *rh, const i32 *__restrict__ pc,
+__global__ __launch_bounds__(LOC_TPB) void k_compact(const Group g, const u64 *__restrict__ ukey,
+                                                     const u64 *__restrict__ useg, u64 n, const i32 *__restrict__ pc,
                                                      const i32 *__restrict__ pdl, u32 nb,
                                                      int self_prefix, u64 *__restrict__ ckey, u64 *__restrict__ cseg,
-                                                     i32 *__restrict__ pd, Status *__restrict__ status) {
+                                                     i32 *__restrict__ pd, u32 *__restrict__ wfirst,
+                                                     Status *__restrict__ status) {
+    const RoundHdr *rh = g.rh;
     __shared__ i32 tmp[LOC_TPB / 64 + 1];
     __shared__ i32 red[4][LOC_TPB / 64];
     i32 base_c, base_d;
@@ -181,6 +183,20 @@ __global__ __launch_bounds__(LOC_TPB) void k_compact(const u64 *__restrict__ uke
         cseg[ex] = sg & ~RK_OP;
         pd[ex] = dx;
     }
+    if (g.npma == 1) {
+        // update index: the first update of each index-level window.  The
+        // wave's updates are sorted, so only the first of each window in
+        // the wave competes (atomic min) with the other waves'.
+        const u32 H = pma_of(g, 0).H, lb = index_level(H);
+        if (lb <= H) {
+            const bool e = sg != RK_NONE;
+            const u32 w = e ? (u32)sg >> lb : 0u;  // low word of a run key: the segment
+            const u64 below = __ballot(e) & ((1ull << lane_id()) - 1ull);
+            const int p = below ? 63 - __clzll((long long)below) : (int)lane_id();
+            const u32 wp = (u32)__shfl((int)w, p, 64);
+            if (e && (below == 0 || wp != w)) atomicMin(&wfirst[w], (u32)ex);
+        }
+    }
 }
 
 hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchBufs &b, hipStream_t st) {
@@ -190,8 +206,8 @@ hipError_t locate_and_compact(const Group &g, const u64 *pk, u64 n, const BatchB
     else k_locate<false><<<nb, LOC_TPB, 0, st>>>(g, pk, n, b.ukey, b.useg, pc, pdl);
     int self_prefix = nb <= COMPACT_SELF_PREFIX_MAX;
     if (!self_prefix) k_scan_parts<<<1, 1024, 0, st>>>(pc, pdl, nb, b.status, b.pd);
-    k_compact<<<nb, LOC_TPB, 0, st>>>(b.ukey, b.useg, n, g.rh, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd,
-                                      b.status);
+    k_compact<<<nb, LOC_TPB, 0, st>>>(g, b.ukey, b.useg, n, pc, pdl, nb, self_prefix, b.ckey, b.cseg, b.pd,
+                                      b.wfirst, b.status);
     return hipGetLastError();
 }
 
@@ -240,10 +256,48 @@ __device__ __forceinline__ u32 gallop_back(const u64 *__restrict__ a, u32 start,
     return l;
 }
 
-// levels climbed with galloping searches before the lock-step searches
+// levels climbed with galloping searches before the lock-step searches (a
+// group of PMAs; a lone PMA switches to its update index at index_level(H))
 static constexpr u32 DENSITY_GALLOP_LEVELS = 8;
 // updates (of the window below) from which a window is claimed on arrival
 static constexpr u32 DENSITY_CLAIM_ON_ARRIVAL = 64;
+// levels evaluated together above the gallops
+static constexpr u32 DENSITY_LOCKSTEP = 8;
+
+// Update-index lookup table of a block: F[x] = first compacted update at or
+// after index-level window x (n_eff past the last), from the per-window
+// firsts of k_compact by a suffix minimum.  nW <= WFIRST_N.
+static constexpr u32 WF_PER = (WFIRST_N + 1 + 255) / 256 | 1;  // odd: LDS banks apart
+__device__ __forceinline__ void fill_window_firsts(const u32 *__restrict__ wfirst, u32 nW, u32 n_eff, u32 *F,
+                                                   u32 *wtot) {
+    const u32 b0 = threadIdx.x * WF_PER;
+    u32 v[WF_PER];
+#pragma unroll
+    for (u32 k = 0; k < WF_PER; ++k) v[k] = b0 + k < nW ? min(wfirst[b0 + k], n_eff) : n_eff;
+    u32 m = n_eff;
+#pragma unroll
+    for (int k = WF_PER - 1; k >= 0; --k) {
+        m = min(m, v[k]);
+        v[k] = m;  // suffix minimum inside the thread's run
+    }
+    // suffix minimum over the later threads
+    const u32 lane = lane_id(), wv = threadIdx.x >> 6;
+    u32 x = m;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 y = (u32)__shfl_down((int)x, o, 64);
+        if (lane + o < 64) x = min(x, y);
+    }
+    if (lane == 0) wtot[wv] = x;
+    const u32 nxt = (u32)__shfl_down((int)x, 1, 64);
+    __syncthreads();
+    u32 after = lane < 63 ? nxt : n_eff;
+    for (u32 w = wv + 1; w < blockDim.x / 64; ++w) after = min(after, wtot[w]);
+#pragma unroll
+    for (u32 k = 0; k < WF_PER; ++k)
+        if (b0 + k <= nW) F[b0 + k] = min(v[k], after);
+    __syncthreads();
+}
 
 // Every run leader (first update of a run) climbs the count pyramid from its
 // segment until a window fits after the batch, and claims it (atomic max of
@@ -254,16 +308,28 @@ static constexpr u32 DENSITY_CLAIM_ON_ARRIVAL = 64;
 // (its updates lie in that window, which the first run either takes or
 // leaves for a larger window that contains it).  k_leaders keeps a claimed
 // window only if no claimed window of a higher level contains it, so a
-// window that was passed through never survives.  A lone PMA that would hold more keys
-// than it has slots (live + the batch's net delta) is resized without any
-// climb.  The first and last update of each PMA record its compacted range.
+// window that was passed through never survives.
+// Above the gallops, a lone PMA reads its windows' update ranges off the
+// update index (two LDS lookups per level, every level's loads independent);
+// one run per index-level window climbs on (the first in its wave, then the
+// first to arrive).  A group searches the bounds of several levels in lock
+// step.  A lone PMA that would hold more keys than it has slots (live + the
+// batch's net delta) is resized without any climb.  The first and last
+// update of each PMA record its compacted range.
 template <bool ONE>
-__global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
-                          const Status *__restrict__ status_in, u32 *__restrict__ rlevel, Status *status) {
+__global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
+                                                 const u32 *__restrict__ wfirst, const Status *__restrict__ status_in,
+                                                 u32 *__restrict__ rlevel, Status *status) {
+    __shared__ u32 F[WFIRST_N + 1];
+    __shared__ u32 wtot[256 / 64];
     const u32 n_eff = status_in->n_eff;
     const u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool lone = g.npma == 1;
     u32 local_flags = 0;
     u32 pid = 0xFFFFFFFFu, lvbit = 0;
+    bool leader = false, stopped = false, fit = false;
+    u32 l = 0, u0 = 0, u1 = 0, s = 0;
+    u64 pb = 0;
     if (j < n_eff) {
         const u64 rk = cseg[j];
         pid = (u32)(rk >> 32);
@@ -276,28 +342,28 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
             g.pstat[pid].u1 = j + 1;
             g.pstat[pid].pd1 = pd[j + 1];
         }
-        if (j == 0 || prev != rk) {
+        leader = j == 0 || prev != rk;
+        if (leader) {
             // density bounds from the device table (a lone PMA's entry at a
             // fixed address): they stay out of the kernel arguments, which
             // every launch of the pipeline copies
             const Density &dn = g.tab[ONE ? 0 : pid].dn;
             const DevState d = geom<ONE>(g, pid);
-            const u64 pb = (u64)pid << 32;
-            const u32 s = (u32)rk;
+            pb = (u64)pid << 32;
+            s = (u32)rk;
             const u64 nseg = nseg_of(d);
             const u64 tag = (u64)group_epoch(g) << 32;
-            u32 u0 = j, u1 = j + 1;
-            u32 l = 0;
-            bool stopped = false;
-            if (g.npma == 1) {
+            u0 = j;
+            u1 = j + 1;
+            if (lone) {
                 // more keys than slots: along some run's path every window
                 // overflows its capacity (a full window has a child over its
                 // own), so that run reaches the root and the PMA grows
                 const i64 mroot = (i64)*count_ptr(d, d.H, 0) + (i64)status_in->delta;
                 if (mroot > (i64)(nseg << d.log2S)) l = d.H + 1;
             }
-            bool fit = false;
-            for (; l <= d.H && l <= DENSITY_GALLOP_LEVELS; ++l) {
+            const u32 lg = lone ? index_level(d.H) : DENSITY_GALLOP_LEVELS + 1;
+            for (; l <= d.H && l < lg; ++l) {
                 const u64 w = (u64)s >> l;
                 const u64 a = w << l;
                 // windows with many updates are claimed on arrival (only the
@@ -328,76 +394,138 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
                     break;
                 }
             }
-            // Higher up (rare: a run still climbing there leads a large
-            // region that is full or sparse), the bounds of several levels'
-            // windows are found together: their binary searches step in lock
-            // step, so the dependent loads are one search deep, not one per
-            // level; the claims still go level by level.
-            while (!stopped && !fit && l <= d.H) {
-                constexpr u32 G = 8;
-                u32 lo0[G], hi0[G], lo1[G], hi1[G];
-                u64 t0[G], t1[G], Cl[G];
-                const u32 ng = min(G, d.H + 1 - l);
+        }
+    }
+    bool climbing = leader && !stopped && !fit;
+    if (lone) {
+        const DevState d = geom<ONE>(g, 0);
+        const u32 lb = index_level(d.H);
+        bool high = climbing && l == lb && lb <= d.H;
+        // one run per index-level window climbs on: the first in its wave
+        // (the wave's runs are sorted), then the first to arrive (claim)
+        const u32 wl = (u32)((u64)s >> lb);
+        const u64 below = __ballot(high) & ((1ull << lane_id()) - 1ull);
+        const int pl = below ? 63 - __clzll((long long)below) : (int)lane_id();
+        const u32 wpl = (u32)__shfl((int)wl, pl, 64);
+        if (high && below != 0 && wpl == wl) {
+            high = false;
+            stopped = true;
+        }
+        const u64 nseg = nseg_of(d);
+        const u64 tag = (u64)group_epoch(g) << 32;
+        if (high && lb > 0 && lb < d.H) {
+            const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, lb, wl)],
+                                      (unsigned long long)(tag | (u64)(~j)));
+            if ((old & ~0xFFFFFFFFull) == tag) {
+                high = false;
+                stopped = true;
+            }
+        }
+        if (__syncthreads_or(high)) fill_window_firsts(wfirst, 1u << (d.H - lb), n_eff, F, wtot);
+        if (high) {
+            const Density &dn = g.tab[0].dn;
+            while (!fit && l <= d.H) {
+                u64 Cl[DENSITY_LOCKSTEP];
+                i32 p0[DENSITY_LOCKSTEP], p1[DENSITY_LOCKSTEP];
+                const u32 ng = min(DENSITY_LOCKSTEP, d.H + 1 - l);
 #pragma unroll
-                for (u32 q = 0; q < G; ++q) {
+                for (u32 q = 0; q < DENSITY_LOCKSTEP; ++q) {
                     const u32 lq = l + q;
-                    const u64 a = ((u64)s >> lq) << lq;
-                    t0[q] = pb + a;
-                    t1[q] = pb + a + (1ull << lq);
-                    lo0[q] = 0; hi0[q] = q < ng ? u0 : 0u;      // the start lies at or before u0
-                    lo1[q] = u1; hi1[q] = q < ng ? n_eff : u1;  // the end at or after u1
-                    Cl[q] = q < ng ? *count_ptr(d, lq, (u64)s >> lq) : 0ull;
-                }
-                bool more = true;
-                while (more) {
-                    more = false;
-                    u64 v0[G], v1[G];
-#pragma unroll
-                    for (u32 q = 0; q < G; ++q) {
-                        v0[q] = lo0[q] < hi0[q] ? cseg[(lo0[q] + hi0[q]) >> 1] : 0ull;
-                        v1[q] = lo1[q] < hi1[q] ? cseg[(lo1[q] + hi1[q]) >> 1] : 0ull;
+                    if (q < ng) {
+                        const u64 w = (u64)s >> lq;
+                        Cl[q] = *count_ptr(d, lq, w);
+                        p0[q] = pd[F[(u32)(w << (lq - lb))]];
+                        p1[q] = pd[F[(u32)((w + 1) << (lq - lb))]];
                     }
-#pragma unroll
-                    for (u32 q = 0; q < G; ++q) {
-                        if (lo0[q] < hi0[q]) {
-                            const u32 mid = (lo0[q] + hi0[q]) >> 1;
-                            if (v0[q] < t0[q]) lo0[q] = mid + 1; else hi0[q] = mid;
-                        }
-                        if (lo1[q] < hi1[q]) {
-                            const u32 mid = (lo1[q] + hi1[q]) >> 1;
-                            if (v1[q] < t1[q]) lo1[q] = mid + 1; else hi1[q] = mid;
-                        }
-                        more |= lo0[q] < hi0[q] || lo1[q] < hi1[q];
-                    }
-                }
-                i32 p0[G], p1[G];
-#pragma unroll
-                for (u32 q = 0; q < G; ++q) {
-                    p0[q] = q < ng ? pd[lo0[q]] : 0;
-                    p1[q] = q < ng ? pd[lo1[q]] : 0;
                 }
                 for (u32 q = 0; q < ng; ++q, ++l) {
-                    const u64 w = (u64)s >> l;
-                    const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
-                                              (unsigned long long)(tag | (u64)(~j)));
-                    if ((old & ~0xFFFFFFFFull) == tag) {
-                        stopped = true;
-                        break;
-                    }
-                    u0 = lo0[q];
-                    u1 = lo1[q];
-                    const i64 m = (i64)Cl[q] + (i64)(p1[q] - p0[q]);
-                    if (window_fits(dn, l, d.H, (u64)m)) {
+                    if (window_fits(dn, l, d.H, (u64)((i64)Cl[q] + (i64)(p1[q] - p0[q])))) {
+                        if (l > lb && l < d.H)
+                            atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, (u64)s >> l)],
+                                      (unsigned long long)(tag | (u64)(~j)));
                         fit = true;
                         break;
                     }
                 }
             }
-            rlevel[j] = l;
-            if (!stopped) {
-                if (l >= d.H) local_flags = (l == d.H) ? FLAG_ROOT : FLAG_RESIZE;
-                else if (l > 0) lvbit = 1u << l;
+        }
+    } else if (climbing) {
+        // Higher up (rare: a run still climbing there leads a large region
+        // that is full or sparse), the bounds of several levels' windows are
+        // found together: their binary searches step in lock step, so the
+        // dependent loads are one search deep, not one per level; the
+        // claims still go level by level.
+        const DevState d = geom<ONE>(g, pid);
+        const Density &dn = g.tab[ONE ? 0 : pid].dn;
+        const u64 nseg = nseg_of(d);
+        const u64 tag = (u64)group_epoch(g) << 32;
+        while (!stopped && !fit && l <= d.H) {
+            constexpr u32 G = DENSITY_LOCKSTEP;
+            u32 lo0[G], hi0[G], lo1[G], hi1[G];
+            u64 t0[G], t1[G], Cl[G];
+            const u32 ng = min(G, d.H + 1 - l);
+#pragma unroll
+            for (u32 q = 0; q < G; ++q) {
+                const u32 lq = l + q;
+                const u64 a = ((u64)s >> lq) << lq;
+                t0[q] = pb + a;
+                t1[q] = pb + a + (1ull << lq);
+                lo0[q] = 0; hi0[q] = q < ng ? u0 : 0u;      // the start lies at or before u0
+                lo1[q] = u1; hi1[q] = q < ng ? n_eff : u1;  // the end at or after u1
+                Cl[q] = q < ng ? *count_ptr(d, lq, (u64)s >> lq) : 0ull;
             }
+            bool more = true;
+            while (more) {
+                more = false;
+                u64 v0[G], v1[G];
+#pragma unroll
+                for (u32 q = 0; q < G; ++q) {
+                    v0[q] = lo0[q] < hi0[q] ? cseg[(lo0[q] + hi0[q]) >> 1] : 0ull;
+                    v1[q] = lo1[q] < hi1[q] ? cseg[(lo1[q] + hi1[q]) >> 1] : 0ull;
+                }
+#pragma unroll
+                for (u32 q = 0; q < G; ++q) {
+                    if (lo0[q] < hi0[q]) {
+                        const u32 mid = (lo0[q] + hi0[q]) >> 1;
+                        if (v0[q] < t0[q]) lo0[q] = mid + 1; else hi0[q] = mid;
+                    }
+                    if (lo1[q] < hi1[q]) {
+                        const u32 mid = (lo1[q] + hi1[q]) >> 1;
+                        if (v1[q] < t1[q]) lo1[q] = mid + 1; else hi1[q] = mid;
+                    }
+                    more |= lo0[q] < hi0[q] || lo1[q] < hi1[q];
+                }
+            }
+            i32 p0[G], p1[G];
+#pragma unroll
+            for (u32 q = 0; q < G; ++q) {
+                p0[q] = q < ng ? pd[lo0[q]] : 0;
+                p1[q] = q < ng ? pd[lo1[q]] : 0;
+            }
+            for (u32 q = 0; q < ng; ++q, ++l) {
+                const u64 w = (u64)s >> l;
+                const u64 old = atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
+                                          (unsigned long long)(tag | (u64)(~j)));
+                if ((old & ~0xFFFFFFFFull) == tag) {
+                    stopped = true;
+                    break;
+                }
+                u0 = lo0[q];
+                u1 = lo1[q];
+                const i64 m = (i64)Cl[q] + (i64)(p1[q] - p0[q]);
+                if (window_fits(dn, l, d.H, (u64)m)) {
+                    fit = true;
+                    break;
+                }
+            }
+        }
+    }
+    if (leader) {
+        const u32 H = geom<ONE>(g, pid).H;
+        rlevel[j] = l;
+        if (!stopped) {
+            if (l >= H) local_flags = (l == H) ? FLAG_ROOT : FLAG_RESIZE;
+            else if (l > 0) lvbit = 1u << l;
         }
     }
     // one atomic per wave for the flags (a batch that resizes raises them in
@@ -424,9 +552,11 @@ __global__ void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__re
 hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
     // one thread per update (no grid-stride loop): the grid covers n
     if (g.inline_one)
-        k_density<true><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
+        k_density<true><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.wfirst, b.status, b.rlevel,
+                                                                    b.status);
     else
-        k_density<false><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.status, b.rlevel, b.status);
+        k_density<false><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.wfirst, b.status, b.rlevel,
+                                                                     b.status);
     return hipGetLastError();
 }
 
@@ -1540,9 +1670,11 @@ __device__ void pyr_top_levels(const DevState &d) {
 __global__ __launch_bounds__(1024) void k_pyr_top(DevState d) { pyr_top_levels(d); }
 
 __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs, u32 *zero,
-                                                      u32 nzero) {
+                                                      u32 nzero, u32 *wfirst) {
     if (blockIdx.x == 0) {
         for (u32 i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
+        if (g.npma == 1)
+            for (u32 i = threadIdx.x; i <= WFIRST_N; i += blockDim.x) wfirst[i] = 0xFFFFFFFFu;
         if (threadIdx.x == 0) {
             s->flags = 0;
             s->n_eff = 0;
