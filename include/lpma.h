@@ -254,6 +254,17 @@ int lpma_set_graph(lpma_t *h, int on);
  * later rounds of that size capture nothing. */
 int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream);
 
+/* Makes growth up to max_edges live edges free of device allocation: the
+ * arrays of every height that resizes up to that size pass through are
+ * allocated now and held by the handle, and the rebuild scratch is sized
+ * for the largest.  From then on a resize also keeps the arrays it leaves,
+ * so shrinking back to a height held before allocates nothing either.  The
+ * memory comes from the device pool and counts against its budget;
+ * LPMA_ERR_OUT_OF_MEMORY leaves the structure unchanged.  No counterpart in the reference, whose resize
+ * allocates its new levels when it happens (rpma.cuh:1396-1415); for
+ * callers whose latency bound covers resize batches. */
+int lpma_reserve_capacity(lpma_t *h, uint64_t max_edges, void *stream);
+
 /* Workgroup shape of the single-segment window rebalance: slots handled by
  * each lane (a power of two; 0 restores the default, 4 slots per lane up to
  * 256-slot segments, a full wave per segment above).  The lanes of one

@@ -142,6 +142,7 @@ SIGNATURES = [
     ("lpma_get_profile", _int, [_vp, ctypes.POINTER(Profile)]),
     ("lpma_set_graph", _int, [_vp, _int]),
     ("lpma_reserve", _int, [_vp, _u64, _vp]),
+    ("lpma_reserve_capacity", _int, [_vp, _u64, _vp]),
     ("lpma_set_launch", _int, [_vp, _u32]),
     ("lpma_set_pool_budget", _int, [_u64]),
     ("lpma_pool_trim", _int, []),
@@ -533,6 +534,12 @@ class PMA:
 
     def reserve(self, max_batch):
         _check(lib().lpma_reserve(self._h, int(max_batch), _stream_ptr(self.stream)), "reserve")
+
+    def reserve_capacity(self, max_edges):
+        """Allocate now what resizes up to max_edges live edges will need
+        (cached in the device pool), so those resize batches make no device
+        allocation."""
+        _check(lib().lpma_reserve_capacity(self._h, int(max_edges), _stream_ptr(self.stream)), "reserve_capacity")
 
     def set_launch(self, seg_slots_per_lane=0):
         _check(lib().lpma_set_launch(self._h, int(seg_slots_per_lane)), "set_launch")

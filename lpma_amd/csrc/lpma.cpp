@@ -216,6 +216,12 @@ static u32 early_wave_grid(const Pipeline &pl, u64 n) {
 
 // ------------------------------------------------------------------ handle
 
+// the arrays of one height
+struct Geometry {
+    DBuf<u64> keys, sep, owner;
+    DBuf<u32> cnt, pyr;
+};
+
 struct lpma {
     int device = 0;
     hipStream_t cur_stream = nullptr;  // stream of the call in progress
@@ -231,6 +237,11 @@ struct lpma {
     DBuf<u32> cnt, pyr, row_off;
     DBuf<u64> owner;
     DBuf<u64> sep;
+    // arrays of other heights held for resizes (lpma_reserve_capacity): a
+    // resize takes its new arrays from here and, while keep_spares is set,
+    // leaves its old ones here
+    std::map<u32, Geometry> spares;
+    bool keep_spares = false;
 
     Pipeline pl;  // plain updates, views and analytics scratch
     // analytics scratch
@@ -474,18 +485,17 @@ u32 fit_height(const lpma *h, u64 m) {
 
 bool height_ok(const lpma *h, u32 H) { return H <= 31 && ((u64)h->cfg.segment_slots << H) <= (1ull << 32); }
 
-struct Geometry {
-    DBuf<u64> keys, sep, owner;
-    DBuf<u32> cnt, pyr;
-};
-
 // arrays for height H; the caller spreads keys into every slot and segment
 int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
     u64 nseg = 1ull << H;
     u64 cap = nseg << h->log2S;
     int r;
-    if ((r = ensure(h, g.keys, cap)) || (r = ensure(h, g.cnt, nseg)) || (r = ensure(h, g.pyr, nseg)) ||
-        (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
+    auto spare = h->spares.find(H);
+    if (spare != h->spares.end()) {
+        g = spare->second;
+        h->spares.erase(spare);
+    } else if ((r = ensure(h, g.keys, cap)) || (r = ensure(h, g.cnt, nseg)) || (r = ensure(h, g.pyr, nseg)) ||
+               (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
         release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
         return r;
     }
@@ -506,7 +516,12 @@ void release_geometry(lpma *h, Geometry &g) {
 
 // installs g as h's arrays (g is left empty)
 void install_geometry(lpma *h, u32 H, Geometry &g) {
-    release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->sep); release(h, h->owner);
+    if (h->keep_spares && h->keys.p && !h->spares.count(h->st.H)) {
+        Geometry &old = h->spares[h->st.H];
+        old.keys = h->keys; old.cnt = h->cnt; old.pyr = h->pyr; old.sep = h->sep; old.owner = h->owner;
+    } else {
+        release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->sep); release(h, h->owner);
+    }
     h->keys = g.keys; h->cnt = g.cnt; h->pyr = g.pyr; h->sep = g.sep; h->owner = g.owner;
     g = Geometry();
     h->st.keys = h->keys.p;
@@ -1170,7 +1185,14 @@ int reserve_pipeline(Pipeline &pl, lpma *const *members, u32 npma, u64 max_batch
     const u32 pshift = 2 * vmax + 1;
     const u32 hi_bit = pshift + bits_for(npma);
     if (hi_bit > 64) return LPMA_ERR_INVALID_ARGUMENT;
-    if (!pl.graph_on) return ensure_batch(pl, max_batch, npma, 2 * npma, std::min<u64>(max_batch, nseg_total));
+    if (!pl.graph_on) {
+        // work lists for as many runs as keys (the segments may multiply
+        // first), and the large-window scratch of a batch that size
+        int r = ensure_batch(pl, max_batch, npma, 2 * npma, max_batch);
+        if (!r && !(r = ensure(pl, pl.lchunk, max_batch + 1)) && !(r = ensure(pl, pl.lmbase, max_batch)))
+            r = ensure(pl, pl.lhdr, 4);
+        return r;
+    }
     const u64 top = graph_capacity(max_batch);
     // headroom for the live keys to grow before the scratch is reallocated
     CHECKED(prepare_graph_buffers(pl, top, npma, live_sum + live_sum / 4 + max_batch));
@@ -1376,6 +1398,7 @@ int lpma_destroy(lpma_t *h) {
     DevGuard guard(h->device);
     (void)hipDeviceSynchronize();
     pipeline_free(h->pl);
+    for (auto &kv : h->spares) release_geometry(h, kv.second);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
     release(h, h->aq0); release(h, h->apoff); release(h, h->apcb);
@@ -1821,6 +1844,50 @@ int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream) {
     h->cur_stream = (hipStream_t)stream;
     lpma *one[1] = {h};
     return reserve_pipeline(h->pl, one, 1, max_batch, (hipStream_t)stream);
+}
+
+// Growth up to m live keys without device allocation: the arrays of every
+// height a resize may pass through are allocated now and held by the handle
+// (resizes take them and leave their old arrays in their place), and the
+// pipeline's rebuild scratch is sized for the largest.
+int reserve_growth(lpma *h, u64 m, hipStream_t st) {
+    const u32 Ht = fit_height(h, m);
+    if (!height_ok(h, Ht)) return LPMA_ERR_TOO_LARGE;
+    std::vector<u32> added;
+    int r = LPMA_OK;
+    for (u32 H = h->st.H + 1; H <= Ht && r == LPMA_OK; ++H) {
+        if (h->spares.count(H)) continue;
+        const u64 nseg = 1ull << H;
+        Geometry g;
+        if ((r = ensure(h, g.keys, nseg << h->log2S)) || (r = ensure(h, g.cnt, nseg)) ||
+            (r = ensure(h, g.pyr, nseg)) || (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
+            release_geometry(h, g);
+            break;
+        }
+        h->spares[H] = g;
+        added.push_back(H);
+    }
+    h->pl.cur_stream = st;
+    const u64 nseg = 1ull << std::max(Ht, h->st.H);
+    if (!r && !(r = ensure(h->pl, h->pl.merged, m + 1)) && !(r = ensure(h->pl, h->pl.gpre, 2 * nseg + 2)))
+        r = ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8);
+    if (r) {
+        for (u32 H : added) {
+            release_geometry(h, h->spares[H]);
+            h->spares.erase(H);
+        }
+        return r;
+    }
+    h->keep_spares = true;
+    return LPMA_OK;
+}
+
+int lpma_reserve_capacity(lpma_t *h, uint64_t max_edges, void *stream) {
+    if (!h) return LPMA_ERR_INVALID_ARGUMENT;
+    if (max_edges > (1ull << 40)) return LPMA_ERR_TOO_LARGE;
+    DevGuard guard(h->device);
+    h->cur_stream = (hipStream_t)stream;
+    return reserve_growth(h, max_edges + h->V, (hipStream_t)stream);
 }
 
 int lpma_get_profile(lpma_t *h, lpma_profile_t *out) {
