@@ -254,13 +254,15 @@ int lpma_set_graph(lpma_t *h, int on);
  * later rounds of that size capture nothing. */
 int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream);
 
-/* Makes growth up to max_edges live edges free of device allocation: the
- * arrays of every height that resizes up to that size pass through are
- * allocated now and held by the handle, and the rebuild scratch is sized
- * for the largest.  From then on a resize also keeps the arrays it leaves,
- * so shrinking back to a height held before allocates nothing either.  The
- * memory comes from the device pool and counts against its budget;
- * LPMA_ERR_OUT_OF_MEMORY leaves the structure unchanged.  No counterpart in the reference, whose resize
+/* Makes root rebuilds and resizes up to max_edges live edges free of device
+ * allocation.  A rebuild writes new arrays and keeps the old ones, so the
+ * handle now allocates and holds one spare set of arrays of its current
+ * height and two of every height above it up to the one max_edges needs
+ * (about three times the arrays at that height); a shrink back to a height
+ * held before allocates nothing either.  The scratch of large windows is
+ * sized for max_edges.  The memory comes from the device pool and counts
+ * against its budget; LPMA_ERR_OUT_OF_MEMORY leaves the structure
+ * unchanged.  No counterpart in the reference, whose resize
  * allocates its new levels when it happens (rpma.cuh:1396-1415); for
  * callers whose latency bound covers resize batches. */
 int lpma_reserve_capacity(lpma_t *h, uint64_t max_edges, void *stream);

@@ -66,8 +66,10 @@ u32 pyramid_blocks(u32 H);
 // global (multi-workgroup) window path, in place, and the full rebuild
 hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st);
 // ubeg: nseg + 1 words of scratch
-hipError_t rebuild_push(const DevState &src, u32 pid, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m,
-                        u32 *gpre, u32 *ubeg, u32 *scan_scratch, hipStream_t st);
+// root rebuild / resize of PMA pid: its keys after the batch (updates [u0, u1)
+// of the compacted batch) from src's arrays straight into dst's, spread evenly
+hipError_t rebuild_scatter(const DevState &src, const DevState &dst, u32 pid, u32 u0, u32 u1, const BatchBufs &b,
+                           u64 m, hipStream_t st);
 hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);

@@ -237,10 +237,10 @@ struct lpma {
     DBuf<u32> cnt, pyr, row_off;
     DBuf<u64> owner;
     DBuf<u64> sep;
-    // arrays of other heights held for resizes (lpma_reserve_capacity): a
-    // resize takes its new arrays from here and, while keep_spares is set,
+    // arrays held for rebuilds, by height (lpma_reserve_capacity): a
+    // rebuild takes its new arrays from here and, while keep_spares is set,
     // leaves its old ones here
-    std::map<u32, Geometry> spares;
+    std::map<u32, std::vector<Geometry>> spares;
     bool keep_spares = false;
 
     Pipeline pl;  // plain updates, views and analytics scratch
@@ -491,9 +491,9 @@ int alloc_geometry(lpma *h, u32 H, Geometry &g, hipStream_t st) {
     u64 cap = nseg << h->log2S;
     int r;
     auto spare = h->spares.find(H);
-    if (spare != h->spares.end()) {
-        g = spare->second;
-        h->spares.erase(spare);
+    if (spare != h->spares.end() && !spare->second.empty()) {
+        g = spare->second.back();
+        spare->second.pop_back();
     } else if ((r = ensure(h, g.keys, cap)) || (r = ensure(h, g.cnt, nseg)) || (r = ensure(h, g.pyr, nseg)) ||
                (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
         release(h, g.keys); release(h, g.cnt); release(h, g.pyr); release(h, g.sep); release(h, g.owner);
@@ -516,9 +516,10 @@ void release_geometry(lpma *h, Geometry &g) {
 
 // installs g as h's arrays (g is left empty)
 void install_geometry(lpma *h, u32 H, Geometry &g) {
-    if (h->keep_spares && h->keys.p && !h->spares.count(h->st.H)) {
-        Geometry &old = h->spares[h->st.H];
+    if (h->keep_spares && h->keys.p) {
+        Geometry old;
         old.keys = h->keys; old.cnt = h->cnt; old.pyr = h->pyr; old.sep = h->sep; old.owner = h->owner;
+        h->spares[h->st.H].push_back(old);
     } else {
         release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->sep); release(h, h->owner);
     }
@@ -614,18 +615,27 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
 
 // Spread m sorted keys from `merged` over the fresh arrays g of height H and
 // install them (create and resize); g is released on failure.
-int spread_and_install(lpma *h, u32 H, Geometry &g, const u64 *merged, u64 m, hipStream_t st) {
-    DevState d2 = state_with(h, H, g);
+// the count pyramid of new arrays g (keys and counts written), then g
+// replaces h's arrays; g is released on failure
+int pyramid_and_install(lpma *h, u32 H, Geometry &g, hipStream_t st) {
     // old arrays may still be read by queued kernels: wait before they
     // return to the pool
-    if (rebuild_spread(d2, merged, m, st) != hipSuccess || rebuild_pyramid(d2, true, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+    if (rebuild_pyramid(state_with(h, H, g), true, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
         (void)hipGetLastError();
         release_geometry(h, g);
         return LPMA_ERR_DEVICE;
     }
     install_geometry(h, H, g);
     return LPMA_OK;
+}
+
+int spread_and_install(lpma *h, u32 H, Geometry &g, const u64 *merged, u64 m, hipStream_t st) {
+    if (rebuild_spread(state_with(h, H, g), merged, m, st) != hipSuccess) {
+        (void)hipGetLastError();
+        release_geometry(h, g);
+        return LPMA_ERR_DEVICE;
+    }
+    return pyramid_and_install(h, H, g, st);
 }
 
 // One PMA's share of a pipeline run.
@@ -719,7 +729,7 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
     // before any kernel changes a PMA
     std::vector<u64> Mnew(npma);
     std::vector<u32> newH(npma);
-    u64 m_sum = 0, nseg_rebuild = 0;
+    u64 m_sum = 0;
     bool any_rebuild = false;
     for (u32 p = 0; p < npma; ++p) {
         lpma *h = mem[p].h;
@@ -730,7 +740,6 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
             any_rebuild = true;
             if (ps[p].flags & FLAG_RESIZE) newH[p] = fit_height(h, Mnew[p]);
             if (!height_ok(h, newH[p])) return LPMA_ERR_TOO_LARGE;
-            nseg_rebuild = std::max<u64>(nseg_rebuild, 1ull << h->st.H);
         }
     }
     const bool reelect = windows_done && any_rebuild && g_in.defer_rebuild;
@@ -739,8 +748,8 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
     g.defer_rebuild = 0;
 
     // allocations: scratch of the large windows (bounded by the runs when the
-    // election is still to come), of the rebuilds, and the new arrays of
-    // every resizing member
+    // election is still to come), and the new arrays of every member that
+    // is rebuilt (a resize, or a root rebalance at the same height)
     std::vector<Geometry> geo(npma);
     auto undo = [&](int r) {
         for (u32 p = 0; p < npma; ++p) release_geometry(mem[p].h, geo[p]);
@@ -749,17 +758,13 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
     {
         int r = LPMA_OK;
         const u64 nlarge = reelect ? n_eff : (windows_done ? 0 : s.nwork[CLASS_LARGE]);
-        if (nlarge || any_rebuild) r = ensure(pl, pl.merged, m_sum + 1);
-        if (!r && nlarge) {
-            if (!(r = ensure(pl, pl.lchunk, nlarge + 1)) && !(r = ensure(pl, pl.lmbase, nlarge)))
+        if (nlarge) {
+            if (!(r = ensure(pl, pl.merged, m_sum + 1)) && !(r = ensure(pl, pl.lchunk, nlarge + 1)) &&
+                !(r = ensure(pl, pl.lmbase, nlarge)))
                 r = ensure(pl, pl.lhdr, 4);
         }
-        if (!r && any_rebuild) {
-            if (!(r = ensure(pl, pl.gpre, 2 * nseg_rebuild + 2)))
-                r = ensure(pl, pl.scan_scr, scan_scratch_words(nseg_rebuild) + 8);
-            for (u32 p = 0; !r && p < npma; ++p)
-                if (newH[p] != mem[p].h->st.H) r = alloc_geometry(mem[p].h, newH[p], geo[p], st);
-        }
+        for (u32 p = 0; !r && any_rebuild && p < npma; ++p)
+            if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) r = alloc_geometry(mem[p].h, newH[p], geo[p], st);
         if (r) return undo(r);
     }
     BatchBufs bb = batch_bufs(pl);
@@ -809,23 +814,15 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
 
     if (any_rebuild) {
         // root rebalance or resize: every live key and every update of the
-        // PMA is merged into scratch and spread again (rebuild_rpma,
-        // rpma.cuh:1417-1472)
+        // PMA go straight to their slots in the new arrays, which then
+        // replace the old ones (rebuild_rpma, rpma.cuh:1417-1472)
         PhaseTimer t(pl, st, PH_REBUILD);
         for (u32 p = 0; p < npma; ++p) {
             if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
             lpma *h = mem[p].h;
-            const u64 M = Mnew[p];
-            u64 nseg = 1ull << h->st.H;
-            HIPUNDO(rebuild_push(h->st, p, ps[p].u0, ps[p].u1, bb, pl.merged.p, M, pl.gpre.p, pl.gpre.p + nseg + 1,
-                                 pl.scan_scr.p, st));
-            if (newH[p] != h->st.H) {
-                int r = spread_and_install(h, newH[p], geo[p], pl.merged.p, M, st);
-                if (r) return undo(r);
-            } else {
-                HIPUNDO(rebuild_spread(h->st, pl.merged.p, M, st));
-                HIPUNDO(rebuild_pyramid(h->st, true, st));
-            }
+            HIPUNDO(rebuild_scatter(h->st, state_with(h, newH[p], geo[p]), p, ps[p].u0, ps[p].u1, bb, Mnew[p], st));
+            const int r = pyramid_and_install(h, newH[p], geo[p], st);
+            if (r) return undo(r);
             h->rebuilds++;
             h->rebuild_slots += (u64)1 << (h->st.H + h->log2S);
             pl.rebuilds++;
@@ -886,11 +883,16 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     // A lone PMA's single-segment and wave windows are queued before the
     // host reads the status (below): the large windows that may follow
     // them get their scratch for the worst case now (their merged keys are
-    // at most the live keys after the batch), or the status is read first.
+    // at most the live keys plus the batch's inserts), or the status is
+    // read first.
     bool early = npma == 1;
-    if (early && (ensure(pl, pl.merged, mem[0].h->live + n + 1) || ensure(pl, pl.lchunk, n + 1) ||
-                  ensure(pl, pl.lmbase, n) || ensure(pl, pl.lhdr, 4)))
-        early = false;
+    if (early) {
+        u64 nins = 0;
+        for (u32 r = 0; r < mem[0].nreq; ++r) nins += mem[0].reqs[r].nins;
+        if (ensure(pl, pl.merged, mem[0].h->live + nins + 1) || ensure(pl, pl.lchunk, n + 1) ||
+            ensure(pl, pl.lmbase, n) || ensure(pl, pl.lhdr, 4))
+            early = false;
+    }
 
     u32 epoch, pyr_blocks;
     CHECKED(begin_epoch(mem, npma, epoch, st));
@@ -1398,7 +1400,8 @@ int lpma_destroy(lpma_t *h) {
     DevGuard guard(h->device);
     (void)hipDeviceSynchronize();
     pipeline_free(h->pl);
-    for (auto &kv : h->spares) release_geometry(h, kv.second);
+    for (auto &kv : h->spares)
+        for (Geometry &g : kv.second) release_geometry(h, g);
     release(h, h->keys); release(h, h->cnt); release(h, h->pyr); release(h, h->owner);
     release(h, h->row_off); release(h, h->sep);
     release(h, h->aq0); release(h, h->apoff); release(h, h->apcb);
@@ -1846,35 +1849,36 @@ int lpma_reserve(lpma_t *h, uint64_t max_batch, void *stream) {
     return reserve_pipeline(h->pl, one, 1, max_batch, (hipStream_t)stream);
 }
 
-// Growth up to m live keys without device allocation: the arrays of every
-// height a resize may pass through are allocated now and held by the handle
-// (resizes take them and leave their old arrays in their place), and the
-// pipeline's rebuild scratch is sized for the largest.
+// Rebuilds up to m live keys without device allocation.  A rebuild writes
+// new arrays and keeps the old ones as spares, so the handle holds a spare
+// of the current height (for root rebuilds) and two of every height above
+// it up to the one m needs (one to grow into, one for root rebuilds there);
+// the merged scratch of large windows is sized for m keys.
 int reserve_growth(lpma *h, u64 m, hipStream_t st) {
     const u32 Ht = fit_height(h, m);
     if (!height_ok(h, Ht)) return LPMA_ERR_TOO_LARGE;
     std::vector<u32> added;
     int r = LPMA_OK;
-    for (u32 H = h->st.H + 1; H <= Ht && r == LPMA_OK; ++H) {
-        if (h->spares.count(H)) continue;
-        const u64 nseg = 1ull << H;
-        Geometry g;
-        if ((r = ensure(h, g.keys, nseg << h->log2S)) || (r = ensure(h, g.cnt, nseg)) ||
-            (r = ensure(h, g.pyr, nseg)) || (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
-            release_geometry(h, g);
-            break;
+    for (u32 H = h->st.H; H <= std::max(Ht, h->st.H) && r == LPMA_OK; ++H) {
+        const size_t want = H == h->st.H ? 1 : 2;
+        while (r == LPMA_OK && h->spares[H].size() < want) {
+            const u64 nseg = 1ull << H;
+            Geometry g;
+            if ((r = ensure(h, g.keys, nseg << h->log2S)) || (r = ensure(h, g.cnt, nseg)) ||
+                (r = ensure(h, g.pyr, nseg)) || (r = ensure(h, g.sep, nseg)) || (r = ensure(h, g.owner, 2 * nseg))) {
+                release_geometry(h, g);
+                break;
+            }
+            h->spares[H].push_back(g);
+            added.push_back(H);
         }
-        h->spares[H] = g;
-        added.push_back(H);
     }
     h->pl.cur_stream = st;
-    const u64 nseg = 1ull << std::max(Ht, h->st.H);
-    if (!r && !(r = ensure(h->pl, h->pl.merged, m + 1)) && !(r = ensure(h->pl, h->pl.gpre, 2 * nseg + 2)))
-        r = ensure(h->pl, h->pl.scan_scr, scan_scratch_words(nseg) + 8);
+    if (!r) r = ensure(h->pl, h->pl.merged, m + 1);
     if (r) {
         for (u32 H : added) {
-            release_geometry(h, h->spares[H]);
-            h->spares.erase(H);
+            release_geometry(h, h->spares[H].back());
+            h->spares[H].pop_back();
         }
         return r;
     }

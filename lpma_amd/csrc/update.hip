@@ -1164,55 +1164,9 @@ hipError_t rebalance_fixed(const Group &g, const BatchBufs &b, u32 runs_cap, u64
 
 // ---------------------------------------------------- global window path
 //
-// Windows too large for one workgroup's LDS, the root window, and resizes.
-// Push: every live key of the source window and every insert computes its
-// merged rank and writes it to a scratch array.  Pull: every destination
-// slot reads the key the even spread assigns to it.  Source and destination
-// are the same window (in place) or the whole old and new arrays (resize).
-
-// first update of every segment's run (or of the next run): one search per
-// segment, so that k_push_old searches each key's segment run only
-__global__ void k_run_starts(const u64 *__restrict__ cseg, u32 u0, u32 u1, u64 pb, u64 nseg, u32 *__restrict__ ubeg) {
-    for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x <= nseg; x += (u64)gridDim.x * blockDim.x)
-        ubeg[x] = x == nseg ? u1 : (u32)lower_bound_dev<u64>(cseg, u0, u1, pb | x);
-}
-
-__global__ void k_push_old(DevState s, u64 a, u32 L, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
-                           const i32 *__restrict__ pd, u32 u0, const u32 *__restrict__ ubeg,
-                           u64 *__restrict__ merged, u64 m) {
-    const u64 W = (1ull << L) << s.log2S;
-    const u64 base = a << s.log2S;
-    const u32 Smask = (1u << s.log2S) - 1;
-    const i32 pbase = pd[u0];
-    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < W; t += (u64)gridDim.x * blockDim.x) {
-        u64 seg = t >> s.log2S;
-        u32 lane = (u32)t & Smask;
-        if (lane >= s.cnt[a + seg]) continue;
-        u64 x = s.keys[base + t];
-        u64 r = (u64)gpre[seg] + lane;
-        // the updates before x: all runs of earlier segments, then those of
-        // its own segment's run below x
-        const u32 r1 = ubeg[seg + 1];
-        u64 u = lower_bound_dev<u64>(ckey, ubeg[seg], r1, x);
-        if (u < r1 && ckey[u] == x) continue;
-        u64 o = r + (i64)(pd[u] - pbase);
-        if (o < m) merged[o] = x;
-    }
-}
-
-__global__ void k_push_ins(DevState s, u64 a, const u32 *__restrict__ gpre, const u64 *__restrict__ ckey,
-                           const u64 *__restrict__ cseg, const i32 *__restrict__ pd, u32 u0, u32 u1,
-                           u64 *__restrict__ merged, u64 m) {
-    const i32 pbase = pd[u0];
-    for (u64 q = (u64)u0 + (u64)blockIdx.x * blockDim.x + threadIdx.x; q < u1; q += (u64)gridDim.x * blockDim.x) {
-        if (pd[q + 1] - pd[q] != 1) continue;
-        u64 y = ckey[q];
-        u64 seg = (u32)cseg[q];
-        const u64 *sk = s.keys + (seg << s.log2S);
-        u64 o = (u64)gpre[seg - a] + lower_bound_dev<u64>(sk, 0, s.cnt[seg], y) + (i64)(pd[q] - pbase);
-        if (o < m) merged[o] = y;
-    }
-}
+// Spread of merged keys (a bulk load's sorted keys, or a window's merged
+// scratch): every destination slot reads the key the even spread assigns to
+// it (k_pull), and every segment its count and separator (k_meta).
 
 __global__ void k_pull(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ merged, u64 *stats, int cls_slot) {
     const u64 W = (1ull << L) << t.log2S;
@@ -1264,7 +1218,7 @@ __global__ void k_meta(DevState t, u64 a, u32 L, u64 m, const u64 *__restrict__ 
 // window gets a private slice of the merged scratch (windows are disjoint,
 // so the slices add up to at most the live key count).
 
-__device__ __forceinline__ u32 chunk_level(u32 log2S, u32 L) {
+__host__ __device__ __forceinline__ u32 chunk_level(u32 log2S, u32 L) {
     u32 cl = log2S >= LARGE_CHUNK_LOG2_SLOTS ? 0 : LARGE_CHUNK_LOG2_SLOTS - log2S;
     return L < cl ? L : cl;
 }
@@ -1376,108 +1330,211 @@ __device__ __forceinline__ u32 wave_lower_bound(const u64 *__restrict__ a, u32 l
     return lo;
 }
 
-// Push of one chunk (2^cl segments, at most 4096 slots) into the window's
-// merged slice.  The chunk's slots are staged in LDS; each update of the
-// chunk finds its rank among the chunk's live keys there and records what it
-// does to the keys after it (+1 at its rank for an insert, -1 after the key
-// it deletes, which is flagged).  One scan over the ranks then gives every
-// live key its shift, so each key costs a coalesced load and store and no
-// search.  Inserted keys are stored by their updates.
+// Push of one chunk (2^cl segments, at most 4096 slots) of window (a, L):
+// every key that survives the batch and every insert of the chunk gets its
+// rank among the window's keys after the batch, and out(rank, key) places
+// it.  The chunk's slots are staged in LDS; each update of the chunk finds
+// its rank among the chunk's live keys there and records what it does to
+// the keys after it (+1 at its rank for an insert, -1 after the key it
+// deletes, which is flagged).  One scan over the ranks then gives every
+// live key its shift, so each key costs a coalesced load and no search.
+struct ChunkLds {
+    u64 ks[LG_SLOTS];           // the chunk's slots
+    u32 pre[LG_SEG_MAX + 1];    // live keys per segment, then their exclusive scan
+    u32 sh[LG_SLOTS + 2];       // per rank: net updates that first move it, then their scan
+    u32 del[LG_SLOTS / 32 + 1]; // ranks whose key is deleted (rank <= LG_SLOTS)
+    u32 stmp[LGP_TPB / 64 + 1];
+    u32 qr[2];
+    u64 ob;
+};
+
+// After the chunk's keys are placed, after(r0, r1) runs with the chunk's
+// rank range [r0, r1) in the window after the batch (every thread).
+template <class Out, class After>
+__device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 wu0, u32 wu1, u64 pb, u64 s0, u32 cl,
+                                           const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
+                                           const i32 *__restrict__ pd, ChunkLds &S, Out out, After after) {
+    const u32 log2S = d.log2S;
+    const u32 Smask = (1u << log2S) - 1;
+    const u32 wv = threadIdx.x >> 6;
+    const u32 nsc = 1u << cl;
+    const u32 W = nsc << log2S;  // <= LG_SLOTS (S <= 1024)
+    const u64 slot0 = s0 << log2S;
+    // the loads of this phase are independent of each other
+    u64 kv[LG_SLOTS / LGP_TPB];
+#pragma unroll
+    for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+        const u32 t = threadIdx.x + k * LGP_TPB;
+        kv[k] = t < W ? d.keys[slot0 + t] : KEY_NONE;
+    }
+    for (u32 x = threadIdx.x; x < nsc; x += LGP_TPB) S.pre[x] = d.cnt[s0 + x];
+    if (wv < 2) {
+        // the chunk's updates: [first >= its first segment, first >= the next chunk's)
+        const u32 q = wave_lower_bound(cseg, wu0, wu1, pb + s0 + (wv ? nsc : 0u));
+        if (lane_id() == 0) S.qr[wv] = q;
+    } else if (wv == 2) {
+        // rank of the chunk's first key in its window: the live counts of
+        // the aligned blocks left of it, one level per lane
+        const u64 rel = s0 - a;
+        const u32 l = cl + lane_id();
+        u64 v = 0;
+        if (l < L && ((rel >> l) & 1ull)) v = *count_ptr(d, l, (a + ((rel >> (l + 1)) << (l + 1))) >> l);
+        v = wave_sum(v);
+        if (lane_id() == 0) S.ob = v;
+    }
+    for (u32 t = threadIdx.x; t < W + 2; t += LGP_TPB) S.sh[t] = 0;
+    for (u32 t = threadIdx.x; t < LG_SLOTS / 32 + 1; t += LGP_TPB) S.del[t] = 0;
+#pragma unroll
+    for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+        const u32 t = threadIdx.x + k * LGP_TPB;
+        if (t < W) S.ks[t] = kv[k];
+    }
+    __syncthreads();
+    block_scan_runs<LGP_TPB, LGP_PER>(S.pre, nsc, S.stmp);  // pre[nsc] = live keys of the chunk
+    const u32 q0 = S.qr[0], q1 = S.qr[1];
+    const u64 obase = S.ob;
+    const i32 pbase = pd[wu0];
+    for (u32 q = q0 + threadIdx.x; q < q1; q += LGP_TPB) {
+        const u64 y = ckey[q];
+        const u32 ls = (u32)cseg[q] - (u32)s0;
+        const i32 dl = pd[q + 1] - pd[q];  // +1 insert (y absent), -1 delete (y present)
+        const u32 p0 = S.pre[ls];
+        const u32 r = p0 + (u32)lower_bound_dev<u64>(S.ks + (ls << log2S), 0, S.pre[ls + 1] - p0, y);
+        if (dl > 0) {
+            atomicAdd(&S.sh[r], 1u);
+            out(obase + r + (u64)(i64)(pd[q] - pbase), y);
+        } else {
+            atomicAdd(&S.sh[r + 1], 0xFFFFFFFFu);
+            atomicOr(&S.del[r >> 5], 1u << (r & 31));
+        }
+    }
+    __syncthreads();
+    block_scan_runs<LGP_TPB, LGP_PER>(S.sh, S.pre[nsc] + 1, S.stmp);  // sh[r + 1]: net updates below rank r's key
+    const i64 cbase = (i64)(pd[q0] - pbase);                         // the window's updates before the chunk
+#pragma unroll
+    for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+        const u32 t = threadIdx.x + k * LGP_TPB;
+        if (t >= W) break;
+        const u32 seg = t >> log2S, lane = t & Smask;
+        const u32 p0 = S.pre[seg];
+        if (lane >= S.pre[seg + 1] - p0) continue;
+        const u32 r = p0 + lane;
+        if ((S.del[r >> 5] >> (r & 31)) & 1u) continue;
+        out(obase + r + (u64)(cbase + (i64)(i32)S.sh[r + 1]), kv[k]);
+    }
+    const u64 r0 = obase + (u64)cbase;
+    after(r0, r0 + S.pre[nsc] + (u64)(i64)(pd[q1] - pd[q0]));
+    __syncthreads();  // the LDS arrays are reused by the next chunk
+}
+
+// Large windows: each chunk pushes into its window's slice of the merged
+// scratch (k_large_pull then spreads it).
 __global__ __launch_bounds__(LGP_TPB) void k_large_push(Group g, const Work *__restrict__ work,
                                                          const u32 *__restrict__ chunk_base,
                                                          const u64 *__restrict__ mbase, const u32 *__restrict__ hdr,
                                                          const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
                                                          const i32 *__restrict__ pd, u64 *__restrict__ merged) {
     if (g.rh) merged = g.rh->merged;
-    __shared__ u64 ks[LG_SLOTS];           // the chunk's slots
-    __shared__ u32 pre[LG_SEG_MAX + 1];    // live keys per segment, then their exclusive scan
-    __shared__ u32 sh[LG_SLOTS + 2];       // per rank: net updates that first move it, then their scan
-    __shared__ u32 del[LG_SLOTS / 32 + 1];  // ranks whose key is deleted (rank <= LG_SLOTS)
-    __shared__ u32 stmp[LGP_TPB / 64 + 1];
-    __shared__ u32 qr[2];
-    __shared__ u64 ob;
+    __shared__ ChunkLds S;
     const u32 nwin = hdr[0], nch = hdr[1];
-    const u32 log2S = g.log2S;
-    const u32 Smask = (1u << log2S) - 1;
-    const u32 wv = threadIdx.x >> 6;
     for (u32 c = blockIdx.x; c < nch; c += gridDim.x) {
         const u32 i = window_of_chunk(chunk_base, nwin, c);
         const Work wk = work[i];
         const DevState d = pma_of(g, wk.pid);
-        const u64 pb = (u64)wk.pid << 32;
-        const u32 cl = chunk_level(log2S, wk.L);
+        const u32 cl = chunk_level(g.log2S, wk.L);
         const u64 s0 = (u64)wk.a + ((u64)(c - chunk_base[i]) << cl);
-        const u32 nsc = 1u << cl;
-        const u32 W = nsc << log2S;  // <= LG_SLOTS (S <= 1024)
-        const u64 slot0 = s0 << log2S;
-        const u64 mb = mbase[i];
-        // the loads of this phase are independent of each other
-        u64 kv[LG_SLOTS / LGP_TPB];
-#pragma unroll
-        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
-            const u32 t = threadIdx.x + k * LGP_TPB;
-            kv[k] = t < W ? d.keys[slot0 + t] : KEY_NONE;
+        u64 *out = merged + mbase[i];
+        const u64 M = wk.M;
+        push_chunk(
+            d, wk.a, wk.L, wk.u0, wk.u1, (u64)wk.pid << 32, s0, cl, ckey, cseg, pd, S,
+            [=](u64 o, u64 key) {
+                if (o < M) out[o] = key;
+            },
+            [](u64, u64) {});
+    }
+}
+
+// segment of the even spread of M keys over 2^H segments that holds rank o
+// (o < M): the floating estimate of floor(o * 2^H / M) is within one of it
+__device__ __forceinline__ u64 spread_segment(u64 o, u64 M, u32 H, double seg_per_key, u64 &f, u64 &g) {
+    const u64 nseg = 1ull << H;
+    u64 x = (u64)((double)o * seg_per_key);
+    if (x >= nseg) x = nseg - 1;
+    f = spread_first(x, M, H);
+    while (f > o) f = spread_first(--x, M, H);
+    g = spread_first(x + 1, M, H);
+    while (g <= o) {
+        f = g;
+        g = spread_first(++x + 1, M, H);
+    }
+    return x;
+}
+
+// Root rebuild and resize in one pass: every chunk of the old arrays s
+// pushes its keys after the batch straight to their slots in the new arrays
+// t (2^t.H segments holding M keys, spread evenly): the key of rank o goes
+// to segment x = floor(o * 2^H / M) (the even spread's inverse), lane
+// o - spread_first(x).  The first key of a segment also writes its count
+// and separator, markers their row offsets, and the last key of a segment
+// the segments left empty after it (fewer keys than segments).  The chunk
+// then clears the empty tails of the segments whose last key it placed,
+// together, so no merged scratch, second pass or metadata pass is needed.
+__global__ __launch_bounds__(LGP_TPB) void k_rebuild_scatter(DevState s, DevState t, u64 pb, u32 u0, u32 u1, u64 M,
+                                                              double seg_per_key, const u64 *__restrict__ ckey,
+                                                              const u64 *__restrict__ cseg,
+                                                              const i32 *__restrict__ pd) {
+    __shared__ ChunkLds S;
+    const u32 cl = chunk_level(s.log2S, s.H);
+    const u64 nch = nseg_of(s) >> cl;
+    const u32 Ht = t.H, log2S = t.log2S;
+    const u64 nseg_t = nseg_of(t);
+    const u32 SL = 1u << log2S;
+    auto place = [=](u64 o, u64 key) {
+        if (o >= M) return;
+        u64 f, g;
+        const u64 x = spread_segment(o, M, Ht, seg_per_key, f, g);
+        const u32 lane = (u32)(o - f);
+        const u64 base = x << log2S;
+        t.keys[base + lane] = key;
+        if (is_marker(key) && (u32)(key >> 32) < t.V) t.row_off[(u32)(key >> 32) + 1] = (u32)(base + lane);
+        if (lane == 0) {
+            t.cnt[x] = (u32)(g - f);
+            t.sep[x] = x == 0 ? 0ull : key;
         }
-        for (u32 x = threadIdx.x; x < nsc; x += LGP_TPB) pre[x] = d.cnt[s0 + x];
-        if (wv < 2) {
-            // the chunk's updates: [first >= its first segment, first >= the next chunk's)
-            const u32 q = wave_lower_bound(cseg, wk.u0, wk.u1, pb + s0 + (wv ? nsc : 0u));
-            if (lane_id() == 0) qr[wv] = q;
-        } else if (wv == 2) {
-            // rank of the chunk's first key in its window: the live counts of
-            // the aligned blocks left of it, one level per lane
-            const u64 rel = s0 - wk.a;
-            const u32 l = cl + lane_id();
-            u64 v = 0;
-            if (l < wk.L && ((rel >> l) & 1ull))
-                v = *count_ptr(d, l, (wk.a + ((rel >> (l + 1)) << (l + 1))) >> l);
-            v = wave_sum(v);
-            if (lane_id() == 0) ob = v;
-        }
-        for (u32 t = threadIdx.x; t < W + 2; t += LGP_TPB) sh[t] = 0;
-        for (u32 t = threadIdx.x; t < LG_SLOTS / 32 + 1; t += LGP_TPB) del[t] = 0;
-#pragma unroll
-        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
-            const u32 t = threadIdx.x + k * LGP_TPB;
-            if (t < W) ks[t] = kv[k];
-        }
-        __syncthreads();
-        block_scan_runs<LGP_TPB, LGP_PER>(pre, nsc, stmp);  // pre[nsc] = live keys of the chunk
-        const u32 q0 = qr[0], q1 = qr[1];
-        const u64 obase = ob;
-        const i32 pbase = pd[wk.u0];
-        for (u32 q = q0 + threadIdx.x; q < q1; q += LGP_TPB) {
-            const u64 y = ckey[q];
-            const u32 ls = (u32)cseg[q] - (u32)s0;
-            const i32 dl = pd[q + 1] - pd[q];  // +1 insert (y absent), -1 delete (y present)
-            const u32 p0 = pre[ls];
-            const u32 r = p0 + (u32)lower_bound_dev<u64>(ks + (ls << log2S), 0, pre[ls + 1] - p0, y);
-            if (dl > 0) {
-                atomicAdd(&sh[r], 1u);
-                const u64 o = obase + r + (u64)(i64)(pd[q] - pbase);
-                if (o < wk.M) merged[mb + o] = y;
-            } else {
-                atomicAdd(&sh[r + 1], 0xFFFFFFFFu);
-                atomicOr(&del[r >> 5], 1u << (r & 31));
+        if (o + 1 == g) {
+            for (u64 y = x + 1; y < nseg_t && spread_first(y + 1, M, Ht) == g; ++y) {
+                t.cnt[y] = 0;
+                t.sep[y] = key + 1;
             }
         }
-        __syncthreads();
-        block_scan_runs<LGP_TPB, LGP_PER>(sh, pre[nsc] + 1, stmp);  // sh[r + 1]: net updates below rank r's key
-        const i64 cbase = (i64)(pd[q0] - pbase);                   // the window's updates before the chunk
-#pragma unroll
-        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
-            const u32 t = threadIdx.x + k * LGP_TPB;
-            if (t >= W) break;
-            const u32 seg = t >> log2S, lane = t & Smask;
-            const u32 p0 = pre[seg];
-            if (lane >= pre[seg + 1] - p0) continue;
-            const u32 r = p0 + lane;
-            if ((del[r >> 5] >> (r & 31)) & 1u) continue;
-            const u64 o = obase + r + (u64)(cbase + (i64)(i32)sh[r + 1]);
-            if (o < wk.M) merged[mb + o] = kv[k];
+    };
+    // the segments from the one holding rank r0 up to the one before the
+    // segment of rank r1 (the next chunk's first key), each cleared past
+    // its count (empty ones whole)
+    auto tails = [=](u64 r0, u64 r1) {
+        if (r0 >= M || r1 <= r0) return;
+        u64 f, g;
+        const u64 xa = spread_segment(r0, M, Ht, seg_per_key, f, g);
+        const u64 xb = r1 < M ? spread_segment(r1, M, Ht, seg_per_key, f, g) : nseg_t;
+        if (xb <= xa) return;
+        const u64 nslots = (xb - xa) << log2S;
+        for (u64 i = threadIdx.x; i < nslots; i += LGP_TPB) {
+            const u64 x = xa + (i >> log2S);
+            const u32 lane = (u32)i & (SL - 1);
+            if (lane >= spread_first(x + 1, M, Ht) - spread_first(x, M, Ht)) t.keys[(xa << log2S) + i] = KEY_NONE;
         }
-        __syncthreads();  // the LDS arrays are reused by the next chunk
-    }
+    };
+    for (u64 c = blockIdx.x; c < nch; c += gridDim.x)
+        push_chunk(s, 0, s.H, u0, u1, pb, c << cl, cl, ckey, cseg, pd, S, place, tails);
+}
+
+hipError_t rebuild_scatter(const DevState &src, const DevState &dst, u32 pid, u32 u0, u32 u1, const BatchBufs &b,
+                           u64 m, hipStream_t st) {
+    const u64 nch = nseg_of(src) >> chunk_level(src.log2S, src.H);
+    const u32 grid = (u32)std::min<u64>(nch, 512);  // ~58 KB of LDS per block: two per CU
+    k_rebuild_scatter<<<grid, LGP_TPB, 0, st>>>(src, dst, (u64)pid << 32, u0, u1, m,
+                                                (double)nseg_of(dst) / (double)m, b.ckey, b.cseg, b.pd);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__restrict__ work,
@@ -1569,21 +1626,6 @@ hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 
 
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);
-
-// full push of all live keys of one PMA and its effective updates, the
-// compacted range [u0, u1) (root rebalance / resize)
-hipError_t rebuild_push(const DevState &src, u32 pid, u32 u0, u32 u1, const BatchBufs &b, u64 *merged, u64 m,
-                        u32 *gpre, u32 *ubeg, u32 *scan_scratch, hipStream_t st) {
-    u64 nseg = nseg_of(src);
-    u64 W = nseg << src.log2S;
-    (void)device_excl_scan(src.cnt, nseg, gpre, scan_scratch, st);
-    k_run_starts<<<grid_for(nseg + 1, 256, 16384), 256, 0, st>>>(b.cseg, u0, u1, (u64)pid << 32, nseg, ubeg);
-    k_push_old<<<grid_for(W, 256, 16384), 256, 0, st>>>(src, 0, src.H, gpre, b.ckey, b.pd, u0, ubeg, merged, m);
-    if (u1 > u0)
-        k_push_ins<<<grid_for(u1 - u0, 256, 16384), 256, 0, st>>>(src, 0, gpre, b.ckey, b.cseg, b.pd, u0, u1, merged,
-                                                                  m);
-    return hipGetLastError();
-}
 
 // spread m merged keys over the window (a, L) of t
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,

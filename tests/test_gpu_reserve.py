@@ -4,8 +4,9 @@ A PMA bulk-loaded small and grown through several heights by insert
 batches: after reserve_capacity for its peak and reserve for its batch size,
 the updates (resizes included) take every array from what the handle holds,
 so the bytes the device pool holds from hipMalloc stay the same, and the
-result equals the oracle's set (rpma.cuh:1396-1415 is the reference's
-resize, which allocates its levels when it happens).
+result equals the oracle's set; deleting back down to the bulk-loaded set
+(shrinks, root rebuilds) allocates nothing either (rpma.cuh:1396-1415 is the
+reference's resize, which allocates its levels when it happens).
 """
 import numpy as np
 import pytest
@@ -39,8 +40,27 @@ def test_reserved_growth_allocates_nothing(seed):
     torch.cuda.synchronize()
     assert int(p.view().tree_height) >= h0 + 2, "the stream must resize through two heights"
     assert L.pool_reserved_bytes() == held, "a reserved resize allocated device memory"
+    check(p, ref, V)
+    torch.cuda.synchronize()
+    held = L.pool_reserved_bytes()  # the export and check above take scratch of their own
+    # back down to the bulk-loaded set: shrinks and root rebuilds at heights
+    # held before allocate nothing either
+    rng = np.random.default_rng(seed)
+    dels = rng.permutation(ins)
+    trace = []
+    for i in range(0, dels.size, batch):
+        p.update(dels[i:i + batch], 0)
+        ref = oracle.apply(ref, (), dels[i:i + batch])
+        torch.cuda.synchronize()
+        trace.append((int(p.view().tree_height), L.pool_reserved_bytes() - held))
+    assert int(p.view().tree_height) < h0 + 2, "the deletes must shrink the PMA"
+    assert L.pool_reserved_bytes() == held, f"a reserved shrink allocated device memory: h0 {h0}, {trace}"
+    check(p, ref, V)
+    p.close()
+
+
+def check(p, ref, V):
     ek, ro = p.export()
     assert np.array_equal(L.to_host_u64(ek), ref)
     assert np.array_equal(ro.cpu().numpy().view(np.uint32), oracle.row_offsets(ref, V))
     assert p.check() == 0
-    p.close()
