@@ -400,8 +400,10 @@ __global__ __launch_bounds__(OS_TPB) void k_os_pass(const u64 *__restrict__ in, 
 // then it packs its tile of SS_TILE keys, assigns each to its bucket, and
 // writes the tile back grouped by bucket (tile-major: no global offsets are
 // needed), with its per-bucket counts.  k_ss_bucket: one workgroup per
-// bucket gathers the bucket's keys from every tile into registers, sorts
-// them with a bitonic network (shuffles inside a wave, LDS across waves) and
+// bucket gathers the bucket's keys from every tile into registers, finds
+// the bits on which they differ (AND / OR over the bucket), sorts them by
+// LSD radix passes of 8 bits over those bits only, ranked in LDS (wave64
+// "match any" ballots, per-wave digit counts joined by a DPP scan), and
 // writes them at the bucket's offset.  A sample-chosen split keeps buckets
 // even under R-MAT skew, where a split on the top key bits puts a hub's keys
 // in one bucket.  A bucket beyond the workgroup's SB_CAP keys (many equal
