@@ -68,8 +68,11 @@ hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 
 // ubeg: nseg + 1 words of scratch
 // root rebuild / resize of PMA pid: its keys after the batch (updates [u0, u1)
 // of the compacted batch) from src's arrays straight into dst's, spread evenly
+// (index: rebuild_index_words(src) words of scratch for the per-chunk
+// update starts and ranks)
 hipError_t rebuild_scatter(const DevState &src, const DevState &dst, u32 pid, u32 u0, u32 u1, const BatchBufs &b,
-                           u64 m, hipStream_t st);
+                           u64 m, u32 *index, hipStream_t st);
+u64 rebuild_index_words(const DevState &src);
 hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStream_t st);
 hipError_t window_spread(const DevState &t, u64 a, u32 L, const u64 *merged, u64 m, u64 *stats, int cls_slot,
                          hipStream_t st);

@@ -764,7 +764,10 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
                 r = ensure(pl, pl.lhdr, 4);
         }
         for (u32 p = 0; !r && any_rebuild && p < npma; ++p)
-            if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) r = alloc_geometry(mem[p].h, newH[p], geo[p], st);
+            if (ps[p].flags & (FLAG_ROOT | FLAG_RESIZE)) {
+                r = ensure(pl, pl.gpre, rebuild_index_words(mem[p].h->st));
+                if (!r) r = alloc_geometry(mem[p].h, newH[p], geo[p], st);
+            }
         if (r) return undo(r);
     }
     BatchBufs bb = batch_bufs(pl);
@@ -820,7 +823,8 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
         for (u32 p = 0; p < npma; ++p) {
             if (!(ps[p].flags & (FLAG_ROOT | FLAG_RESIZE))) continue;
             lpma *h = mem[p].h;
-            HIPUNDO(rebuild_scatter(h->st, state_with(h, newH[p], geo[p]), p, ps[p].u0, ps[p].u1, bb, Mnew[p], st));
+            HIPUNDO(rebuild_scatter(h->st, state_with(h, newH[p], geo[p]), p, ps[p].u0, ps[p].u1, bb, Mnew[p],
+                                    pl.gpre.p, st));
             const int r = pyramid_and_install(h, newH[p], geo[p], st);
             if (r) return undo(r);
             h->rebuilds++;
@@ -1875,6 +1879,11 @@ int reserve_growth(lpma *h, u64 m, hipStream_t st) {
     }
     h->pl.cur_stream = st;
     if (!r) r = ensure(h->pl, h->pl.merged, m + 1);
+    if (!r) {
+        DevState top = h->st;
+        top.H = std::max(Ht, h->st.H);
+        r = ensure(h->pl, h->pl.gpre, rebuild_index_words(top));  // a rebuild's chunk index
+    }
     if (r) {
         for (u32 H : added) {
             release_geometry(h, h->spares[H].back());

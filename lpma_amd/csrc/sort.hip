@@ -530,13 +530,19 @@ __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict_
     if (tid < nb) bc[(u64)tid * ntiles + t] = (base[tid] << 16) | cnt[tid];
 }
 
+// Keys of a workgroup are spread over every wave: with R = ceil(m / SB_TPB)
+// rounds, wave w holds keys w*64R + r*64 + lane (r < R), so a bucket of up
+// to SB_TPB keys is ranked in one round by all waves at once.
+__device__ __forceinline__ u32 sb_rounds(u32 m) { return (m + SB_TPB - 1) / SB_TPB; }
+__device__ __forceinline__ u32 sb_index(u32 w, u32 r, u32 lane, u32 R) { return w * (64 * R) + r * 64 + lane; }
+
 // One LSD pass over the (up to SB_CAP) keys x[] of the workgroup, digit
-// (x >> shift) & 255: returns each key's rank inside the chunk (stable:
-// wave w holds keys w*256 + r*64 + lane) in rk[] and leaves in dofs[d] the
-// chunk's exclusive count of smaller digits, in whist[w][d] the keys of digit
-// d in waves before w.  Every thread of the workgroup calls it.
-__device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 shift, u32 (*whist)[256], u32 *dofs,
-                                        u32 *stmp, u32 (&rk)[SB_ITEMS]) {
+// (x >> shift) & 255: returns each key's rank inside the chunk (stable, in
+// the order of sb_index) in rk[] and leaves in dofs[d] the chunk's exclusive
+// count of smaller digits, in whist[w][d] the keys of digit d in waves
+// before w.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 R, u32 shift, u32 (*whist)[256],
+                                        u32 *dofs, u32 *stmp, u32 (&rk)[SB_ITEMS]) {
     const u32 lane = lane_id();
     const u32 w = threadIdx.x >> 6;
     const u64 lt = (1ull << lane) - 1ull;
@@ -544,8 +550,8 @@ __device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 shi
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SB_ITEMS; ++r) {
-        if (w * (64 * SB_ITEMS) + r * 64 >= m) break;  // the rest of this wave's rounds are empty
-        const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+        if (r >= (int)R || sb_index(w, r, 0, R) >= m) break;  // the rest of this wave's rounds are empty
+        const u32 j = sb_index(w, r, lane, R);
         const bool valid = j < m;
         const u32 dg = (u32)(x[r] >> shift) & 255u;
         u64 peers = __ballot(valid);
@@ -651,15 +657,16 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
     };
 
     if (m <= SB_CAP) {
-        // gather: element j = w * 256 + r * 64 + lane into x[r] (addresses
+        // gather: element j = sb_index(w, r, lane, R) into x[r] (addresses
         // first, then the loads), and the bits on which the keys differ
+        const u32 R = sb_rounds(m);
         u64 x[SB_ITEMS];
         u32 rk[SB_ITEMS];
         {
             u64 ad[SB_ITEMS];
 #pragma unroll
             for (int r = 0; r < SB_ITEMS; ++r) {
-                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                const u32 j = r < (int)R ? sb_index(w, r, lane, R) : m;
                 ad[r] = j < m ? addr_of(j) : ~0ull;
             }
 #pragma unroll
@@ -668,7 +675,7 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
         u64 andv = ~0ull, orv = 0;
 #pragma unroll
         for (int r = 0; r < SB_ITEMS; ++r) {
-            const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+            const u32 j = r < (int)R ? sb_index(w, r, lane, R) : m;
             if (j < m) {
                 andv &= x[r];
                 orv |= x[r];
@@ -687,10 +694,10 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
         const u32 hi_bit = diff ? 64u - (u32)__builtin_clzll(diff) : 0u;
         u64 *kbuf = kb[0];
         for (u32 shift = lo_bit; shift < hi_bit; shift += 8) {
-            sb_rank(x, m, shift, whist, dofs, stmp, rk);
+            sb_rank(x, m, R, shift, whist, dofs, stmp, rk);
 #pragma unroll
             for (int r = 0; r < SB_ITEMS; ++r) {
-                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                const u32 j = r < (int)R ? sb_index(w, r, lane, R) : m;
                 if (j < m) {
                     const u32 dg = (u32)(x[r] >> shift) & 255u;
                     kbuf[dofs[dg] + whist[w][dg] + rk[r]] = x[r];
@@ -699,14 +706,14 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < SB_ITEMS; ++r) {
-                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                const u32 j = r < (int)R ? sb_index(w, r, lane, R) : m;
                 if (j < m) x[r] = kbuf[j];
             }
             // (the next pass's first barrier keeps kbuf until every thread has read it)
         }
 #pragma unroll
         for (int r = 0; r < SB_ITEMS; ++r) {
-            const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+            const u32 j = r < (int)R ? sb_index(w, r, lane, R) : m;
             if (j < m) out[obase + j] = x[r];
         }
         return;
@@ -753,17 +760,18 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
         __syncthreads();
         for (u32 c0 = 0; c0 < m; c0 += SB_CAP) {
             const u32 mc = m - c0 < SB_CAP ? m - c0 : SB_CAP;
+            const u32 R = sb_rounds(mc);
             u64 x[SB_ITEMS];
             u32 rk[SB_ITEMS];
 #pragma unroll
             for (int r = 0; r < SB_ITEMS; ++r) {
-                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                const u32 j = r < (int)R ? sb_index(w, r, lane, R) : mc;
                 x[r] = j < mc ? src[c0 + j] : 0ull;
             }
-            sb_rank(x, mc, shift, whist, dofs, stmp, rk);
+            sb_rank(x, mc, R, shift, whist, dofs, stmp, rk);
 #pragma unroll
             for (int r = 0; r < SB_ITEMS; ++r) {
-                const u32 j = w * (64 * SB_ITEMS) + r * 64 + lane;
+                const u32 j = r < (int)R ? sb_index(w, r, lane, R) : mc;
                 if (j < mc) {
                     const u32 dg = (u32)(x[r] >> shift) & 255u;
                     dst[run[dg] + whist[w][dg] + rk[r]] = x[r];

@@ -1348,27 +1348,50 @@ struct ChunkLds {
     u64 ob;
 };
 
-// After the chunk's keys are placed, after(r0, r1) runs with the chunk's
-// rank range [r0, r1) in the window after the batch (every thread).
-template <class Out, class After>
-__device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 wu0, u32 wu1, u64 pb, u64 s0, u32 cl,
-                                           const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
-                                           const i32 *__restrict__ pd, ChunkLds &S, Out out, After after) {
+// A chunk's update range and window rank when computed beforehand
+// (k_chunk_index): q[c], q[c + 1] and ob[c]; otherwise (q null) the chunk
+// finds them itself.
+struct ChunkPre {
+    const u32 *q;
+    const u64 *ob;
+    u64 c;
+};
+
+// What ranking a chunk leaves: its live keys' exclusive prefix per segment
+// (S.pre), their shifts (S.sh) and deletions (S.del) in LDS, and these.
+struct ChunkRanks {
+    u64 obase;  // the chunk's first live key's rank among the window's old keys
+    i64 cbase;  // net updates of the window before the chunk
+    u64 r0, r1; // the chunk's rank range in the window after the batch
+    u32 nsc, W; // segments and slots of the chunk
+};
+
+// Phases of a chunk push up to the ranks: loads (the chunk's slots into kv
+// and LDS, its counts, its update range and window rank), the counts' scan,
+// the updates (on_insert(rank, key) for each insert), and the shifts' scan.
+template <class OnInsert>
+__device__ __forceinline__ ChunkRanks chunk_rank(const DevState &d, u64 a, u32 L, u32 wu0, u32 wu1, u64 pb, u64 s0,
+                                                 u32 cl, const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
+                                                 const i32 *__restrict__ pd, ChunkLds &S, ChunkPre pre,
+                                                 u64 (&kv)[LG_SLOTS / LGP_TPB], OnInsert on_insert) {
     const u32 log2S = d.log2S;
-    const u32 Smask = (1u << log2S) - 1;
     const u32 wv = threadIdx.x >> 6;
-    const u32 nsc = 1u << cl;
-    const u32 W = nsc << log2S;  // <= LG_SLOTS (S <= 1024)
+    ChunkRanks cr;
+    cr.nsc = 1u << cl;
+    cr.W = cr.nsc << log2S;  // <= LG_SLOTS (S <= 1024)
+    const u32 nsc = cr.nsc, W = cr.W;
     const u64 slot0 = s0 << log2S;
     // the loads of this phase are independent of each other
-    u64 kv[LG_SLOTS / LGP_TPB];
 #pragma unroll
     for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
         const u32 t = threadIdx.x + k * LGP_TPB;
         kv[k] = t < W ? d.keys[slot0 + t] : KEY_NONE;
     }
     for (u32 x = threadIdx.x; x < nsc; x += LGP_TPB) S.pre[x] = d.cnt[s0 + x];
-    if (wv < 2) {
+    if (pre.q) {
+        if (threadIdx.x < 2) S.qr[threadIdx.x] = pre.q[pre.c + threadIdx.x];
+        if (threadIdx.x == 2) S.ob = pre.ob[pre.c];
+    } else if (wv < 2) {
         // the chunk's updates: [first >= its first segment, first >= the next chunk's)
         const u32 q = wave_lower_bound(cseg, wu0, wu1, pb + s0 + (wv ? nsc : 0u));
         if (lane_id() == 0) S.qr[wv] = q;
@@ -1392,7 +1415,7 @@ __device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 
     __syncthreads();
     block_scan_runs<LGP_TPB, LGP_PER>(S.pre, nsc, S.stmp);  // pre[nsc] = live keys of the chunk
     const u32 q0 = S.qr[0], q1 = S.qr[1];
-    const u64 obase = S.ob;
+    cr.obase = S.ob;
     const i32 pbase = pd[wu0];
     for (u32 q = q0 + threadIdx.x; q < q1; q += LGP_TPB) {
         const u64 y = ckey[q];
@@ -1402,7 +1425,7 @@ __device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 
         const u32 r = p0 + (u32)lower_bound_dev<u64>(S.ks + (ls << log2S), 0, S.pre[ls + 1] - p0, y);
         if (dl > 0) {
             atomicAdd(&S.sh[r], 1u);
-            out(obase + r + (u64)(i64)(pd[q] - pbase), y);
+            on_insert(cr.obase + r + (u64)(i64)(pd[q] - pbase), y);
         } else {
             atomicAdd(&S.sh[r + 1], 0xFFFFFFFFu);
             atomicOr(&S.del[r >> 5], 1u << (r & 31));
@@ -1410,20 +1433,44 @@ __device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 
     }
     __syncthreads();
     block_scan_runs<LGP_TPB, LGP_PER>(S.sh, S.pre[nsc] + 1, S.stmp);  // sh[r + 1]: net updates below rank r's key
-    const i64 cbase = (i64)(pd[q0] - pbase);                         // the window's updates before the chunk
+    cr.cbase = (i64)(pd[q0] - pbase);
+    cr.r0 = cr.obase + (u64)cr.cbase;
+    cr.r1 = cr.r0 + S.pre[nsc] + (u64)(i64)(pd[q1] - pd[q0]);
+    return cr;
+}
+
+// rank after the batch of the old key in chunk slot t, or ~0 (no live key
+// there, or deleted)
+__device__ __forceinline__ u64 chunk_key_rank(const ChunkLds &S, const ChunkRanks &cr, u32 t, u32 log2S) {
+    const u32 seg = t >> log2S, lane = t & ((1u << log2S) - 1);
+    const u32 p0 = S.pre[seg];
+    if (lane >= S.pre[seg + 1] - p0) return ~0ull;
+    const u32 r = p0 + lane;
+    if ((S.del[r >> 5] >> (r & 31)) & 1u) return ~0ull;
+    return cr.obase + r + (u64)(cr.cbase + (i64)(i32)S.sh[r + 1]);
+}
+
+// Push of one chunk (2^cl segments, at most 4096 slots) of window (a, L):
+// every key that survives the batch and every insert of the chunk gets its
+// rank among the window's keys after the batch, and out(rank, key) places
+// it.  The chunk's slots are staged in LDS; each update of the chunk finds
+// its rank among the chunk's live keys there and records what it does to
+// the keys after it (+1 at its rank for an insert, -1 after the key it
+// deletes, which is flagged).  One scan over the ranks then gives every
+// live key its shift, so each key costs a coalesced load and no search.
+template <class Out>
+__device__ __forceinline__ void push_chunk(const DevState &d, u64 a, u32 L, u32 wu0, u32 wu1, u64 pb, u64 s0, u32 cl,
+                                           const u64 *__restrict__ ckey, const u64 *__restrict__ cseg,
+                                           const i32 *__restrict__ pd, ChunkLds &S, ChunkPre pre, Out out) {
+    u64 kv[LG_SLOTS / LGP_TPB];
+    const ChunkRanks cr = chunk_rank(d, a, L, wu0, wu1, pb, s0, cl, ckey, cseg, pd, S, pre, kv, out);
 #pragma unroll
     for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
         const u32 t = threadIdx.x + k * LGP_TPB;
-        if (t >= W) break;
-        const u32 seg = t >> log2S, lane = t & Smask;
-        const u32 p0 = S.pre[seg];
-        if (lane >= S.pre[seg + 1] - p0) continue;
-        const u32 r = p0 + lane;
-        if ((S.del[r >> 5] >> (r & 31)) & 1u) continue;
-        out(obase + r + (u64)(cbase + (i64)(i32)S.sh[r + 1]), kv[k]);
+        if (t >= cr.W) break;
+        const u64 o = chunk_key_rank(S, cr, t, d.log2S);
+        if (o != ~0ull) out(o, kv[k]);
     }
-    const u64 r0 = obase + (u64)cbase;
-    after(r0, r0 + S.pre[nsc] + (u64)(i64)(pd[q1] - pd[q0]));
     __syncthreads();  // the LDS arrays are reused by the next chunk
 }
 
@@ -1445,14 +1492,32 @@ __global__ __launch_bounds__(LGP_TPB) void k_large_push(Group g, const Work *__r
         const u64 s0 = (u64)wk.a + ((u64)(c - chunk_base[i]) << cl);
         u64 *out = merged + mbase[i];
         const u64 M = wk.M;
-        push_chunk(
-            d, wk.a, wk.L, wk.u0, wk.u1, (u64)wk.pid << 32, s0, cl, ckey, cseg, pd, S,
-            [=](u64 o, u64 key) {
-                if (o < M) out[o] = key;
-            },
-            [](u64, u64) {});
+        push_chunk(d, wk.a, wk.L, wk.u0, wk.u1, (u64)wk.pid << 32, s0, cl, ckey, cseg, pd, S,
+                   ChunkPre{nullptr, nullptr, 0}, [=](u64 o, u64 key) {
+                       if (o < M) out[o] = key;
+                   });
     }
 }
+
+// Root rebuild and resize in one pass: every chunk of the old arrays s
+// ranks its keys after the batch (chunk_rank) and writes them to new arrays
+// t (2^t.H segments holding M keys, spread evenly).  The chunk's ranks
+// [r0, r1) are contiguous, so its merged keys are assembled in LDS in rank
+// order (its surviving keys from registers, its inserts from a stash), and
+// then the destination segments that hold its ranks are written slot by
+// slot, coalesced: a slot of segment x, lane j holds rank
+// spread_first(x) + j if j is below the segment's count.  A chunk writes
+// the slots of its own ranks, the count and separator of the segments whose
+// first rank is its, and the empty tail of the segments whose last rank is
+// its (with the segments left empty after them: fewer keys than segments);
+// markers also write their row offsets.  A chunk whose keys outgrow the
+// LDS (many inserts) places each key by the even spread's inverse instead.
+static constexpr u32 RB_STASH = 256;  // inserts a chunk keeps in LDS
+struct RebuildLds {
+    u64 iy[RB_STASH];
+    u64 io[RB_STASH];
+    u32 n;
+};
 
 // segment of the even spread of M keys over 2^H segments that holds rank o
 // (o < M): the floating estimate of floor(o * 2^H / M) is within one of it
@@ -1470,25 +1535,21 @@ __device__ __forceinline__ u64 spread_segment(u64 o, u64 M, u32 H, double seg_pe
     return x;
 }
 
-// Root rebuild and resize in one pass: every chunk of the old arrays s
-// pushes its keys after the batch straight to their slots in the new arrays
-// t (2^t.H segments holding M keys, spread evenly): the key of rank o goes
-// to segment x = floor(o * 2^H / M) (the even spread's inverse), lane
-// o - spread_first(x).  The first key of a segment also writes its count
-// and separator, markers their row offsets, and the last key of a segment
-// the segments left empty after it (fewer keys than segments).  The chunk
-// then clears the empty tails of the segments whose last key it placed,
-// together, so no merged scratch, second pass or metadata pass is needed.
 __global__ __launch_bounds__(LGP_TPB) void k_rebuild_scatter(DevState s, DevState t, u64 pb, u32 u0, u32 u1, u64 M,
                                                               double seg_per_key, const u64 *__restrict__ ckey,
                                                               const u64 *__restrict__ cseg,
-                                                              const i32 *__restrict__ pd) {
+                                                              const i32 *__restrict__ pd, const u32 *__restrict__ cq,
+                                                              const u64 *__restrict__ cob) {
     __shared__ ChunkLds S;
+    __shared__ RebuildLds R;
     const u32 cl = chunk_level(s.log2S, s.H);
     const u64 nch = nseg_of(s) >> cl;
     const u32 Ht = t.H, log2S = t.log2S;
     const u64 nseg_t = nseg_of(t);
     const u32 SL = 1u << log2S;
+    const u64 Nm1 = nseg_t - 1;
+    // one key by the even spread's inverse, with the segments left empty
+    // after a segment's last key
     auto place = [=](u64 o, u64 key) {
         if (o >= M) return;
         u64 f, g;
@@ -1508,32 +1569,141 @@ __global__ __launch_bounds__(LGP_TPB) void k_rebuild_scatter(DevState s, DevStat
             }
         }
     };
-    // the segments from the one holding rank r0 up to the one before the
-    // segment of rank r1 (the next chunk's first key), each cleared past
-    // its count (empty ones whole)
-    auto tails = [=](u64 r0, u64 r1) {
-        if (r0 >= M || r1 <= r0) return;
-        u64 f, g;
-        const u64 xa = spread_segment(r0, M, Ht, seg_per_key, f, g);
-        const u64 xb = r1 < M ? spread_segment(r1, M, Ht, seg_per_key, f, g) : nseg_t;
-        if (xb <= xa) return;
-        const u64 nslots = (xb - xa) << log2S;
-        for (u64 i = threadIdx.x; i < nslots; i += LGP_TPB) {
-            const u64 x = xa + (i >> log2S);
-            const u32 lane = (u32)i & (SL - 1);
-            if (lane >= spread_first(x + 1, M, Ht) - spread_first(x, M, Ht)) t.keys[(xa << log2S) + i] = KEY_NONE;
+    for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
+        if (threadIdx.x == 0) R.n = 0;
+        __syncthreads();
+        u64 kv[LG_SLOTS / LGP_TPB];
+        const ChunkRanks cr = chunk_rank(s, 0, s.H, u0, u1, pb, c << cl, cl, ckey, cseg, pd, S,
+                                         ChunkPre{cq, cob, c}, kv, [&](u64 o, u64 y) {
+                                             const u32 i = atomicAdd(&R.n, 1u);
+                                             if (i < RB_STASH) {
+                                                 R.io[i] = o;
+                                                 R.iy[i] = y;
+                                             } else {
+                                                 place(o, y);
+                                             }
+                                         });
+        const u64 r0 = cr.r0, r1 = cr.r1;
+        const bool in_lds = r1 - r0 <= LG_SLOTS && R.n <= RB_STASH;
+        if (!in_lds) {
+            // keys by the inverse, then the empty tails of the segments whose
+            // last key the chunk placed
+            for (u32 i = threadIdx.x; i < min(R.n, RB_STASH); i += LGP_TPB) place(R.io[i], R.iy[i]);
+#pragma unroll
+            for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+                const u32 ts = threadIdx.x + k * LGP_TPB;
+                if (ts >= cr.W) break;
+                const u64 o = chunk_key_rank(S, cr, ts, s.log2S);
+                if (o != ~0ull) place(o, kv[k]);
+            }
+            if (r0 < M && r1 > r0) {
+                u64 f, g;
+                const u64 xa = spread_segment(r0, M, Ht, seg_per_key, f, g);
+                const u64 xb = r1 < M ? spread_segment(r1, M, Ht, seg_per_key, f, g) : nseg_t;
+                const u64 nslots = xb > xa ? (xb - xa) << log2S : 0;
+                for (u64 i = threadIdx.x; i < nslots; i += LGP_TPB) {
+                    const u64 x = xa + (i >> log2S);
+                    const u32 lane = (u32)i & (SL - 1);
+                    const u64 tt = x * M + Nm1;
+                    if (lane >= ((tt + M) >> Ht) - (tt >> Ht)) t.keys[(xa << log2S) + i] = KEY_NONE;
+                }
+            }
+            __syncthreads();
+            continue;
         }
-    };
-    for (u64 c = blockIdx.x; c < nch; c += gridDim.x)
-        push_chunk(s, 0, s.H, u0, u1, pb, c << cl, cl, ckey, cseg, pd, S, place, tails);
+        // merged keys of the chunk in rank order, into the LDS of its slots
+        // (every read of those by the ranking is behind the scans' barriers)
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
+            const u32 ts = threadIdx.x + k * LGP_TPB;
+            if (ts >= cr.W) break;
+            const u64 o = chunk_key_rank(S, cr, ts, s.log2S);
+            if (o != ~0ull) S.ks[o - r0] = kv[k];
+        }
+        for (u32 i = threadIdx.x; i < R.n; i += LGP_TPB) S.ks[R.io[i] - r0] = R.iy[i];
+        __syncthreads();
+        if (r1 > r0) {
+            u64 f, g;
+            const u64 x0 = spread_segment(r0, M, Ht, seg_per_key, f, g);
+            const u64 xk = spread_segment(r1 - 1, M, Ht, seg_per_key, f, g);
+            const u64 xe = r1 < M ? spread_segment(r1, M, Ht, seg_per_key, f, g) : nseg_t;
+            const u64 x1 = xe > xk + 1 ? xe : xk + 1;
+            const u64 nslots = (x1 - x0) << log2S;
+            for (u64 i = threadIdx.x; i < nslots; i += LGP_TPB) {
+                const u64 x = x0 + (i >> log2S);
+                const u32 lane = (u32)i & (SL - 1);
+                const u64 tt = x * M + Nm1;
+                const u64 fx = tt >> Ht, gx = (tt + M) >> Ht;  // spread_first(x), spread_first(x + 1)
+                const u64 slot = (x0 << log2S) + i;
+                if (lane < gx - fx) {
+                    const u64 o = fx + lane;
+                    if (o >= r0 && o < r1) {
+                        const u64 key = S.ks[o - r0];
+                        t.keys[slot] = key;
+                        if (is_marker(key) && (u32)(key >> 32) < t.V) t.row_off[(u32)(key >> 32) + 1] = (u32)slot;
+                        if (lane == 0) {
+                            t.cnt[x] = (u32)(gx - fx);
+                            t.sep[x] = x == 0 ? 0ull : key;
+                        }
+                    }
+                } else {
+                    // the tail belongs to the chunk of the segment's last
+                    // rank, an empty segment's to the chunk of the rank before
+                    const u64 last = gx - 1;  // gx >= 1: fx == gx == 0 is impossible past segment 0
+                    if (last >= r0 && last < r1) {
+                        t.keys[slot] = KEY_NONE;
+                        if (lane == 0 && gx == fx) {
+                            t.cnt[x] = 0;
+                            t.sep[x] = S.ks[last - r0] + 1;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the LDS arrays are reused by the next chunk
+    }
+}
+
+// For every chunk of a whole-PMA rebuild, one wave: the chunk's first
+// update (a 64-pivot search) and its rank in the PMA (one pyramid count per
+// level, one level per lane); q[nch] = u1.
+__global__ __launch_bounds__(256) void k_chunk_index(DevState s, u64 pb, u32 u0, u32 u1, const u64 *__restrict__ cseg,
+                                                     u32 *__restrict__ q, u64 *__restrict__ ob) {
+    const u32 cl = chunk_level(s.log2S, s.H);
+    const u64 nch = nseg_of(s) >> cl;
+    const u64 c = ((u64)blockIdx.x * 256 + threadIdx.x) >> 6;
+    if (c > nch) return;  // whole waves
+    if (c == nch) {
+        if (lane_id() == 0) q[nch] = u1;
+        return;
+    }
+    const u64 s0 = c << cl;
+    const u32 qc = wave_lower_bound(cseg, u0, u1, pb + s0);
+    const u32 l = cl + lane_id();
+    u64 v = 0;
+    if (l < s.H && ((s0 >> l) & 1ull)) v = *count_ptr(s, l, ((s0 >> (l + 1)) << (l + 1)) >> l);
+    v = wave_sum(v);
+    if (lane_id() == 0) {
+        q[c] = qc;
+        ob[c] = v;
+    }
+}
+
+u64 rebuild_index_words(const DevState &src) {
+    const u64 nch = nseg_of(src) >> chunk_level(src.log2S, src.H);
+    return 3 * (nch + 1) + 2;
 }
 
 hipError_t rebuild_scatter(const DevState &src, const DevState &dst, u32 pid, u32 u0, u32 u1, const BatchBufs &b,
-                           u64 m, hipStream_t st) {
+                           u64 m, u32 *index, hipStream_t st) {
     const u64 nch = nseg_of(src) >> chunk_level(src.log2S, src.H);
+    u32 *cq = index;                                                 // nch + 1 words
+    u64 *cob = (u64 *)(((uintptr_t)(index + nch + 1) + 7) & ~(uintptr_t)7);  // nch u64, 8-byte aligned
+    const u64 pb = (u64)pid << 32;
+    k_chunk_index<<<(u32)((nch + 1 + 3) / 4), 256, 0, st>>>(src, pb, u0, u1, b.cseg, cq, cob);
     const u32 grid = (u32)std::min<u64>(nch, 512);  // ~58 KB of LDS per block: two per CU
-    k_rebuild_scatter<<<grid, LGP_TPB, 0, st>>>(src, dst, (u64)pid << 32, u0, u1, m,
-                                                (double)nseg_of(dst) / (double)m, b.ckey, b.cseg, b.pd);
+    k_rebuild_scatter<<<grid, LGP_TPB, 0, st>>>(src, dst, pb, u0, u1, m, (double)nseg_of(dst) / (double)m, b.ckey,
+                                                b.cseg, b.pd, cq, cob);
     return hipGetLastError();
 }
 
