@@ -1868,8 +1868,36 @@ __global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
     }
 }
 
+// Levels above the chunk levels, by one workgroup.  Up to PYR_TOP_LDS
+// chunk-level counts (H <= 23) are loaded once into LDS and every level
+// above is summed there, laid out as in the pyramid (each level right after
+// the one below), so the levels are written out together at the end: a
+// barrier waits for the workgroup's outstanding global stores, so none is
+// issued between the levels.  Taller pyramids go level by level through
+// global memory.
+static constexpr u32 PYR_TOP_LDS = 4096;
+
 __device__ void pyr_top_levels(const DevState &d) {
+    __shared__ u32 v[2 * PYR_TOP_LDS];
     const u64 nseg = nseg_of(d);
+    if (d.H <= PYR_CHL) return;
+    const u64 nc = nseg >> PYR_CHL;
+    if (nc <= PYR_TOP_LDS) {
+        const u32 *in = d.pyr + pyr_off(nseg, PYR_CHL);
+        for (u32 x = threadIdx.x; x < (u32)nc; x += blockDim.x) v[x] = in[x];
+        __syncthreads();
+        u32 lo = 0, n = (u32)nc;  // the level below: v[lo, lo + n)
+        for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
+            for (u32 x = threadIdx.x; x < n / 2; x += blockDim.x) v[lo + n + x] = v[lo + 2 * x] + v[lo + 2 * x + 1];
+            lo += n;
+            n /= 2;
+            __syncthreads();
+        }
+        // levels PYR_CHL + 1 .. H: v[nc, 2 nc - 1), contiguous in the pyramid too
+        u32 *out = d.pyr + pyr_off(nseg, PYR_CHL + 1);
+        for (u32 x = threadIdx.x; x < (u32)nc - 1; x += blockDim.x) out[x] = v[nc + x];
+        return;
+    }
     for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
         u64 nw = nseg >> l;
         const u32 *in = d.pyr + pyr_off(nseg, l - 1);
