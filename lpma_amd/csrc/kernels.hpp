@@ -22,6 +22,7 @@ struct BatchBufs {
     u64 *cseg;     // n: their run keys
     i32 *pd;       // n+1: exclusive prefix of +1 (insert) / -1 (delete)
     u32 *rlevel;   // n: chosen level per run start
+    uint4 *rwin;   // n: per run start, its chosen window's update range [u0, u1) and counts C, M
     i32 *part;     // 2 * nblocks + 2: block partials (count, delta)
     Work *work[NUM_CLASSES];  // n each
     u32 *lchunk;   // n+1: chunk prefix over large windows

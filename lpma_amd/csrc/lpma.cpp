@@ -156,6 +156,7 @@ struct Pipeline {
 
     DBuf<u64> pk0, pk1, ukey, useg, ckey, cseg, merged, lmbase;
     DBuf<u32> sort_scr, rlevel, gpre, scan_scr, lchunk, lhdr, lvmask, wfirst;
+    DBuf<uint4> rwin;
     DBuf<i32> pd, part;
     DBuf<Work> work[NUM_CLASSES];
     DBuf<PmaEntry> tab;
@@ -390,7 +391,7 @@ void pipeline_free(Pipeline &pl) {
     int d = pl.device;
     release_on(d, pl.pk0); release_on(d, pl.pk1); release_on(d, pl.ukey); release_on(d, pl.useg);
     release_on(d, pl.ckey); release_on(d, pl.cseg); release_on(d, pl.merged); release_on(d, pl.lmbase);
-    release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
+    release_on(d, pl.sort_scr); release_on(d, pl.rlevel); release_on(d, pl.rwin); release_on(d, pl.gpre); release_on(d, pl.scan_scr);
     release_on(d, pl.lchunk); release_on(d, pl.lhdr); release_on(d, pl.pd); release_on(d, pl.part);
     release_on(d, pl.lvmask); release_on(d, pl.wfirst);
     for (int c = 0; c < NUM_CLASSES; ++c) release_on(d, pl.work[c]);
@@ -552,6 +553,7 @@ BatchBufs batch_bufs(Pipeline &pl) {
     b.cseg = pl.cseg.p;
     b.pd = pl.pd.p;
     b.rlevel = pl.rlevel.p;
+    b.rwin = pl.rwin.p;
     b.part = pl.part.p;
     for (int c = 0; c < NUM_CLASSES; ++c) b.work[c] = pl.work[c].p;
     b.lchunk = pl.lchunk.p;
@@ -569,7 +571,7 @@ int ensure_batch(Pipeline &pl, u64 n, u32 npma, u32 nsegs, u64 runs) {
     if ((r = ensure(pl, pl.pk0, n)) || (r = ensure(pl, pl.pk1, n)) ||
         (r = ensure(pl, pl.sort_scr, sort_scratch_words(n))) || (r = ensure(pl, pl.ukey, n)) ||
         (r = ensure(pl, pl.useg, n)) || (r = ensure(pl, pl.ckey, n)) || (r = ensure(pl, pl.cseg, n)) ||
-        (r = ensure(pl, pl.pd, n + 1)) || (r = ensure(pl, pl.rlevel, n)) || (r = ensure(pl, pl.part, 2 * nb + 2)) ||
+        (r = ensure(pl, pl.pd, n + 1)) || (r = ensure(pl, pl.rlevel, n)) || (r = ensure(pl, pl.rwin, n)) || (r = ensure(pl, pl.part, 2 * nb + 2)) ||
         (r = ensure(pl, pl.tab, npma)) || (r = ensure(pl, pl.segs, std::max<u32>(nsegs, INLINE_SEGS))) ||
         (r = ensure(pl, pl.lvmask, (u64)npma * LV_REPL)) || (r = ensure(pl, pl.wfirst, WFIRST_N + 1)) ||
         (r = ensure(pl, pl.out, out_words)))
@@ -1072,10 +1074,10 @@ int graph_for(Pipeline &pl, u64 cap, u32 npma, u32 pshift, u32 hi_bit, u32 log2S
     const u32 nsegs = 2 * npma;
     std::vector<u64> key = {cap, npma, pshift, hi_bit, log2S, (u64)(i64)pl.seg_slots_log2,
                             (u64)pl.pk0.p, (u64)pl.pk1.p, (u64)pl.sort_scr.p, (u64)pl.ukey.p, (u64)pl.useg.p,
-                            (u64)pl.ckey.p, (u64)pl.cseg.p, (u64)pl.pd.p, (u64)pl.rlevel.p, (u64)pl.part.p,
+                            (u64)pl.ckey.p, (u64)pl.cseg.p, (u64)pl.pd.p, (u64)pl.rlevel.p, (u64)pl.rwin.p, (u64)pl.part.p,
                             (u64)pl.lchunk.p, (u64)pl.lmbase.p, (u64)pl.lhdr.p, (u64)pl.out.p, (u64)pl.tab.p,
                             (u64)pl.segs.p, (u64)pl.rhdr.p, (u64)pl.d_stats, (u64)pl.h_out, (u64)pl.h_segs,
-                            (u64)pl.h_rhdr, (u64)pl.lvmask.p};
+                            (u64)pl.h_rhdr, (u64)pl.lvmask.p, (u64)pl.wfirst.p};
     for (int c = 0; c < NUM_CLASSES; ++c) key.push_back((u64)pl.work[c].p);
     auto it = pl.graphs.find(cap);
     if (it != pl.graphs.end() && it->second.key != key) {

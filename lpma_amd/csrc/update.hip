@@ -319,7 +319,7 @@ __device__ __forceinline__ void fill_window_firsts(const u32 *__restrict__ wfirs
 template <bool ONE>
 __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
                                                  const u32 *__restrict__ wfirst, const Status *__restrict__ status_in,
-                                                 u32 *__restrict__ rlevel, Status *status) {
+                                                 u32 *__restrict__ rlevel, uint4 *__restrict__ rwin, Status *status) {
     __shared__ u32 F[WFIRST_N + 1];
     __shared__ u32 wtot[256 / 64];
     const u32 n_eff = status_in->n_eff;
@@ -329,6 +329,7 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
     u32 pid = 0xFFFFFFFFu, lvbit = 0;
     bool leader = false, stopped = false, fit = false;
     u32 l = 0, u0 = 0, u1 = 0, s = 0;
+    u32 fC = 0, fM = 0;  // the fitting window's count before and after the batch
     u64 pb = 0;
     if (j < n_eff) {
         const u64 rk = cseg[j];
@@ -391,6 +392,8 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
                         atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, w)],
                                   (unsigned long long)(tag | (u64)(~j)));
                     fit = true;
+                    fC = (u32)C;
+                    fM = (u32)m;
                     break;
                 }
             }
@@ -427,6 +430,7 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
             while (!fit && l <= d.H) {
                 u64 Cl[DENSITY_LOCKSTEP];
                 i32 p0[DENSITY_LOCKSTEP], p1[DENSITY_LOCKSTEP];
+                u32 a0[DENSITY_LOCKSTEP], a1[DENSITY_LOCKSTEP];
                 const u32 ng = min(DENSITY_LOCKSTEP, d.H + 1 - l);
 #pragma unroll
                 for (u32 q = 0; q < DENSITY_LOCKSTEP; ++q) {
@@ -434,16 +438,23 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
                     if (q < ng) {
                         const u64 w = (u64)s >> lq;
                         Cl[q] = *count_ptr(d, lq, w);
-                        p0[q] = pd[F[(u32)(w << (lq - lb))]];
-                        p1[q] = pd[F[(u32)((w + 1) << (lq - lb))]];
+                        a0[q] = F[(u32)(w << (lq - lb))];
+                        a1[q] = F[(u32)((w + 1) << (lq - lb))];
+                        p0[q] = pd[a0[q]];
+                        p1[q] = pd[a1[q]];
                     }
                 }
                 for (u32 q = 0; q < ng; ++q, ++l) {
-                    if (window_fits(dn, l, d.H, (u64)((i64)Cl[q] + (i64)(p1[q] - p0[q])))) {
+                    const i64 m = (i64)Cl[q] + (i64)(p1[q] - p0[q]);
+                    if (window_fits(dn, l, d.H, (u64)m)) {
                         if (l > lb && l < d.H)
                             atomicMax((unsigned long long *)&d.owner[owner_idx(nseg, l, (u64)s >> l)],
                                       (unsigned long long)(tag | (u64)(~j)));
                         fit = true;
+                        u0 = a0[q];
+                        u1 = a1[q];
+                        fC = (u32)Cl[q];
+                        fM = (u32)m;
                         break;
                     }
                 }
@@ -515,6 +526,8 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
                 const i64 m = (i64)Cl[q] + (i64)(p1[q] - p0[q]);
                 if (window_fits(dn, l, d.H, (u64)m)) {
                     fit = true;
+                    fC = (u32)Cl[q];
+                    fM = (u32)m;
                     break;
                 }
             }
@@ -527,6 +540,11 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
             if (l >= H) local_flags = (l == H) ? FLAG_ROOT : FLAG_RESIZE;
             else if (l > 0) lvbit = 1u << l;
         }
+        // the window the election may choose: its updates and counts.  A run
+        // stopped by an earlier arrival can still own the window (the claim
+        // keeps the smallest run index) without having evaluated it: its
+        // record is marked absent and the election finds the range itself.
+        rwin[j] = (!stopped && fit && l < H) ? make_uint4(u0, u1, fC, fM) : make_uint4(~0u, 0u, 0u, 0u);
     }
     // one atomic per wave for the flags (a batch that resizes raises them in
     // every run: atomics on one word from each run would serialise); per PMA
@@ -553,10 +571,10 @@ hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t 
     // one thread per update (no grid-stride loop): the grid covers n
     if (g.inline_one)
         k_density<true><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.wfirst, b.status, b.rlevel,
-                                                                    b.status);
+                                                                    b.rwin, b.status);
     else
         k_density<false><<<grid_for(n, 256, 1u << 30), 256, 0, st>>>(g, b.cseg, b.pd, b.wfirst, b.status, b.rlevel,
-                                                                     b.status);
+                                                                     b.rwin, b.status);
     return hipGetLastError();
 }
 
@@ -564,8 +582,8 @@ hipError_t density_check(const Group &g, u64 n, const BatchBufs &b, hipStream_t 
 
 template <bool ONE>
 __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict__ cseg, const i32 *__restrict__ pd,
-                          const u32 *__restrict__ rlevel, Work *w0, Work *w1, Work *w2, Work *w3,
-                          Status *status) {
+                          const u32 *__restrict__ rlevel, const uint4 *__restrict__ rwin, Work *w0, Work *w1,
+                          Work *w2, Work *w3, Status *status) {
     const u32 stop = FLAG_INVALID_KEY | FLAG_SORT_STALL | (g.defer_rebuild ? FLAG_ROOT | FLAG_RESIZE : 0u);
     if (status->flags & stop) return;
     const u32 n_eff = status->n_eff;
@@ -616,19 +634,25 @@ __global__ __launch_bounds__(1024) void k_leaders(Group g, const u64 *__restrict
             if (dom) leader = false;
         }
         if (leader) {
-            const u64 pb = (u64)pid << 32;
-            u64 a = w << L;
-            u32 u0 = gallop_back(cseg, j, pb + a, 1);
-            u32 u1 = gallop_fwd(cseg, j + 1, n_eff, pb + a + (1ull << L), 1);
-            u32 C = *count_ptr(d, L, w);
+            // the window's updates and counts, as the density check found
+            // them, or found here for a run that stopped on arrival
+            uint4 rw = rwin[j];
+            const u64 a = w << L;
+            if (rw.x == ~0u) {
+                const u64 pb = (u64)pid << 32;
+                rw.x = gallop_back(cseg, j, pb + a, 1);
+                rw.y = gallop_fwd(cseg, j + 1, n_eff, pb + a + (1ull << L), 1);
+                rw.z = *count_ptr(d, L, w);
+                rw.w = (u32)((i32)rw.z + pd[rw.y] - pd[rw.x]);
+            }
             wk.pid = pid;
             wk.a = (u32)a;
             wk.L = L;
-            wk.u0 = u0;
-            wk.u1 = u1;
-            wk.C = C;
-            wk.M = (u32)((i32)C + pd[u1] - pd[u0]);
-            cls = work_class((u64)C + (u1 - u0), L, d.log2S);
+            wk.u0 = rw.x;
+            wk.u1 = rw.y;
+            wk.C = rw.z;
+            wk.M = rw.w;
+            cls = work_class((u64)rw.z + (rw.y - rw.x), L, d.log2S);
         }
     }
     // list appends: LDS counters per class, then one global atomic per
@@ -661,10 +685,10 @@ hipError_t reset_work_counts(Status *status, hipStream_t st) {
 // count from the status
 hipError_t elect_windows(const Group &g, u64 n, const BatchBufs &b, hipStream_t st) {
     if (g.inline_one)
-        k_leaders<true><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1],
+        k_leaders<true><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.rwin, b.work[0], b.work[1],
                                                                       b.work[2], b.work[3], b.status);
     else
-        k_leaders<false><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.work[0], b.work[1],
+        k_leaders<false><<<grid_for(n, 1024, 1u << 30), 1024, 0, st>>>(g, b.cseg, b.pd, b.rlevel, b.rwin, b.work[0], b.work[1],
                                                                        b.work[2], b.work[3], b.status);
     return hipGetLastError();
 }
