@@ -1894,11 +1894,10 @@ __global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
 
 // Levels above the chunk levels, by one workgroup.  Up to PYR_TOP_LDS
 // chunk-level counts (H <= 23) are loaded once into LDS and every level
-// above is summed there, laid out as in the pyramid (each level right after
-// the one below), so the levels are written out together at the end: a
-// barrier waits for the workgroup's outstanding global stores, so none is
-// issued between the levels.  Taller pyramids go level by level through
-// global memory.
+// above is summed there (no global round trip per level), laid out as in
+// the pyramid (each level right after the one below), so the levels are
+// written out together at the end.  Taller pyramids go level by level
+// through global memory.
 static constexpr u32 PYR_TOP_LDS = 4096;
 
 __device__ void pyr_top_levels(const DevState &d) {
