@@ -63,7 +63,7 @@ static constexpr u32 WAVE_SEG_CAP = 64;
 static constexpr u32 BLOCK_E_CAP = 2048;
 static constexpr u32 BLOCK_SEG_CAP = 1024;
 // the multi-workgroup path cuts windows into chunks of this many slots
-static constexpr u32 LARGE_CHUNK_LOG2_SLOTS = 12;
+static constexpr u32 LARGE_CHUNK_LOG2_SLOTS = 11;
 // Update index of a lone PMA, rebuilt every batch: for each window of the
 // index level (at most WFIRST_N windows), the first compacted update inside
 // it.  It turns a window's update range above that level into two lookups.

@@ -1305,7 +1305,7 @@ __device__ __forceinline__ u32 window_of_chunk(const u32 *chunk_base, u32 n, u32
 }
 
 static constexpr int LG_TPB = 256;     // pull: threads per block
-static constexpr int LGP_TPB = 512;    // push: threads per block
+static constexpr int LGP_TPB = 256;    // push: threads per block
 static constexpr u32 LG_SLOTS = 1u << LARGE_CHUNK_LOG2_SLOTS;
 static constexpr u32 LG_SEG_MAX = LG_SLOTS / 2;  // segments per chunk at S = 2
 static constexpr int LGP_PER = (int)((LG_SLOTS + 2 + LGP_TPB - 1) / LGP_TPB);  // odd: 9
@@ -1559,7 +1559,7 @@ __device__ __forceinline__ u64 spread_segment(u64 o, u64 M, u32 H, double seg_pe
     return x;
 }
 
-__global__ __launch_bounds__(LGP_TPB) void k_rebuild_scatter(DevState s, DevState t, u64 pb, u32 u0, u32 u1, u64 M,
+__global__ __launch_bounds__(LGP_TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_rebuild_scatter(DevState s, DevState t, u64 pb, u32 u0, u32 u1, u64 M,
                                                               double seg_per_key, const u64 *__restrict__ ckey,
                                                               const u64 *__restrict__ cseg,
                                                               const i32 *__restrict__ pd, const u32 *__restrict__ cq,
@@ -1725,7 +1725,7 @@ hipError_t rebuild_scatter(const DevState &src, const DevState &dst, u32 pid, u3
     u64 *cob = (u64 *)(((uintptr_t)(index + nch + 1) + 7) & ~(uintptr_t)7);  // nch u64, 8-byte aligned
     const u64 pb = (u64)pid << 32;
     k_chunk_index<<<(u32)((nch + 1 + 3) / 4), 256, 0, st>>>(src, pb, u0, u1, b.cseg, cq, cob);
-    const u32 grid = (u32)std::min<u64>(nch, 512);  // ~58 KB of LDS per block: two per CU
+    const u32 grid = (u32)std::min<u64>(nch, 1024);  // ~33 KB of LDS per block: four per CU
     k_rebuild_scatter<<<grid, LGP_TPB, 0, st>>>(src, dst, pb, u0, u1, m, (double)nseg_of(dst) / (double)m, b.ckey,
                                                 b.cseg, b.pd, cq, cob);
     return hipGetLastError();
@@ -1811,8 +1811,8 @@ __global__ __launch_bounds__(LG_TPB) void k_large_pull(Group g, const Work *__re
 
 hipError_t rebalance_large(const Group &g, const BatchBufs &b, u64 *merged, u32 grid, hipStream_t st) {
     k_large_prep<<<1, 1024, 0, st>>>(g, b.work[CLASS_LARGE], b.status, b.lchunk, b.lmbase, b.lhdr);
-    // push: ~58 KB of LDS per block, two blocks per CU
-    k_large_push<<<min(grid, 512u), LGP_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
+    // push: ~29 KB of LDS per block, four blocks per CU
+    k_large_push<<<min(grid, 1024u), LGP_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, b.ckey, b.cseg, b.pd,
                                           merged);
     k_large_pull<<<grid, LG_TPB, 0, st>>>(g, b.work[CLASS_LARGE], b.lchunk, b.lmbase, b.lhdr, merged);
     return hipGetLastError();
