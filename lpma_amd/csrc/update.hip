@@ -1527,7 +1527,8 @@ __global__ __launch_bounds__(LGP_TPB) void k_large_push(Group g, const Work *__r
 // ranks its keys after the batch (chunk_rank) and writes them to new arrays
 // t (2^t.H segments holding M keys, spread evenly).  The chunk's ranks
 // [r0, r1) are contiguous, so its merged keys are assembled in LDS in rank
-// order (its surviving keys from registers, its inserts from a stash), and
+// order (its surviving keys from registers, its inserts from a stash; up to
+// RB_MERGED of them, more than a full chunk holds), and
 // then the destination segments that hold its ranks are written slot by
 // slot, coalesced: a slot of segment x, lane j holds rank
 // spread_first(x) + j if j is below the segment's count.  A chunk writes
@@ -1537,6 +1538,10 @@ __global__ __launch_bounds__(LGP_TPB) void k_large_push(Group g, const Work *__r
 // markers also write their row offsets.  A chunk whose keys outgrow the
 // LDS (many inserts) places each key by the even spread's inverse instead.
 static constexpr u32 RB_STASH = 256;  // inserts a chunk keeps in LDS
+// merged keys a chunk assembles in LDS: the room of its slots, counts and
+// shifts (ChunkLds up to del), read by then
+static constexpr u32 RB_MERGED = (u32)(offsetof(ChunkLds, del) / sizeof(u64));
+static_assert(offsetof(ChunkLds, ks) == 0, "the merged keys start at the chunk's slots");
 struct RebuildLds {
     u64 iy[RB_STASH];
     u64 io[RB_STASH];
@@ -1608,7 +1613,7 @@ __global__ __launch_bounds__(LGP_TPB) __attribute__((amdgpu_waves_per_eu(4))) vo
                                              }
                                          });
         const u64 r0 = cr.r0, r1 = cr.r1;
-        const bool in_lds = r1 - r0 <= LG_SLOTS && R.n <= RB_STASH;
+        const bool in_lds = r1 - r0 <= RB_MERGED && R.n <= RB_STASH;
         if (!in_lds) {
             // keys by the inverse, then the empty tails of the segments whose
             // last key the chunk placed
@@ -1635,16 +1640,22 @@ __global__ __launch_bounds__(LGP_TPB) __attribute__((amdgpu_waves_per_eu(4))) vo
             __syncthreads();
             continue;
         }
-        // merged keys of the chunk in rank order, into the LDS of its slots
-        // (every read of those by the ranking is behind the scans' barriers)
+        // merged keys of the chunk in rank order, into the chunk's LDS (its
+        // slots, counts and shifts: every thread first takes its keys' ranks,
+        // then the barrier, then the stores)
+        u32 ov[LG_SLOTS / LGP_TPB];
 #pragma unroll
         for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k) {
             const u32 ts = threadIdx.x + k * LGP_TPB;
-            if (ts >= cr.W) break;
-            const u64 o = chunk_key_rank(S, cr, ts, s.log2S);
-            if (o != ~0ull) S.ks[o - r0] = kv[k];
+            const u64 o = ts < cr.W ? chunk_key_rank(S, cr, ts, s.log2S) : ~0ull;
+            ov[k] = o != ~0ull ? (u32)(o - r0) : ~0u;
         }
-        for (u32 i = threadIdx.x; i < R.n; i += LGP_TPB) S.ks[R.io[i] - r0] = R.iy[i];
+        __syncthreads();
+        u64 *mk = reinterpret_cast<u64 *>(&S);
+#pragma unroll
+        for (u32 k = 0; k < LG_SLOTS / LGP_TPB; ++k)
+            if (ov[k] != ~0u) mk[ov[k]] = kv[k];
+        for (u32 i = threadIdx.x; i < R.n; i += LGP_TPB) mk[R.io[i] - r0] = R.iy[i];
         __syncthreads();
         if (r1 > r0) {
             u64 f, g;
@@ -1662,7 +1673,7 @@ __global__ __launch_bounds__(LGP_TPB) __attribute__((amdgpu_waves_per_eu(4))) vo
                 if (lane < gx - fx) {
                     const u64 o = fx + lane;
                     if (o >= r0 && o < r1) {
-                        const u64 key = S.ks[o - r0];
+                        const u64 key = mk[o - r0];
                         t.keys[slot] = key;
                         if (is_marker(key) && (u32)(key >> 32) < t.V) t.row_off[(u32)(key >> 32) + 1] = (u32)slot;
                         if (lane == 0) {
@@ -1678,7 +1689,7 @@ __global__ __launch_bounds__(LGP_TPB) __attribute__((amdgpu_waves_per_eu(4))) vo
                         t.keys[slot] = KEY_NONE;
                         if (lane == 0 && gx == fx) {
                             t.cnt[x] = 0;
-                            t.sep[x] = S.ks[last - r0] + 1;
+                            t.sep[x] = mk[last - r0] + 1;
                         }
                     }
                 }
