@@ -207,10 +207,11 @@ struct Pipeline {
 static constexpr u64 STATUS_BYTES = (sizeof(Status) + 15) / 16 * 16;
 // workgroups of the wave-class rebalance queued before its window count is
 // known (each loops over windows): the last round's count with a quarter
-// more, at least 256, at most one per batch key and 16384
+// more (a round that rebuilt a PMA leaves the count of the round before),
+// at least 1024, at most one per batch key and 16384
 static u32 early_wave_grid(const Pipeline &pl, u64 n) {
     u64 g = (u64)pl.last_wave_windows + pl.last_wave_windows / 4 + 64;
-    g = std::max<u64>(g, 256);
+    g = std::max<u64>(g, 1024);
     g = std::min<u64>(g, 16384);
     return (u32)std::max<u64>(1, std::min<u64>(g, n));
 }
@@ -718,7 +719,8 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
                  hipStream_t st, bool early = false) {
     Status s = *(const Status *)pl.h_out;
     const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
-    pl.last_wave_windows = s.nwork[CLASS_WAVE];
+    // (a rebuild elects no windows for its PMA: not a guide to the next round)
+    if (!(s.flags & (FLAG_ROOT | FLAG_RESIZE))) pl.last_wave_windows = s.nwork[CLASS_WAVE];
     if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
     if (s.flags & FLAG_SORT_STALL) return LPMA_ERR_DEVICE;
     const u64 n_eff = s.n_eff;
