@@ -331,6 +331,7 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
     u32 l = 0, u0 = 0, u1 = 0, s = 0;
     u32 fC = 0, fM = 0;  // the fitting window's count before and after the batch
     u64 pb = 0;
+    bool grows = false;  // a lone PMA over its slots: resized, no climb
     if (j < n_eff) {
         const u64 rk = cseg[j];
         pid = (u32)(rk >> 32);
@@ -361,7 +362,10 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
                 // overflows its capacity (a full window has a child over its
                 // own), so that run reaches the root and the PMA grows
                 const i64 mroot = (i64)*count_ptr(d, d.H, 0) + (i64)status_in->delta;
-                if (mroot > (i64)(nseg << d.log2S)) l = d.H + 1;
+                if (mroot > (i64)(nseg << d.log2S)) {
+                    l = d.H + 1;
+                    grows = true;
+                }
             }
             const u32 lg = lone ? index_level(d.H) : DENSITY_GALLOP_LEVELS + 1;
             for (; l <= d.H && l < lg; ++l) {
@@ -536,7 +540,8 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
     if (leader) {
         const u32 H = geom<ONE>(g, pid).H;
         rlevel[j] = l;
-        if (!stopped) {
+        // (every run of a growing PMA finds the same: the first raises it)
+        if (!stopped && (!grows || j == 0)) {
             if (l >= H) local_flags = (l == H) ? FLAG_ROOT : FLAG_RESIZE;
             else if (l > 0) lvbit = 1u << l;
         }
@@ -546,13 +551,27 @@ __global__ __launch_bounds__(256) void k_density(Group g, const u64 *__restrict_
         // record is marked absent and the election finds the range itself.
         rwin[j] = (!stopped && fit && l < H) ? make_uint4(u0, u1, fC, fM) : make_uint4(~0u, 0u, 0u, 0u);
     }
-    // one atomic per wave for the flags (a batch that resizes raises them in
-    // every run: atomics on one word from each run would serialise); per PMA
-    // the flags and the claimed levels: lanes of the first PMA in the wave
-    // combine theirs, the others (rare) go alone (the levels into the
+    // the flags: one atomic per workgroup for a lone PMA, per wave for a
+    // group (a batch that resizes raises them in many runs, and atomics on
+    // one word serialise: thousands of them cost tens of microseconds); per
+    // PMA the flags and the claimed levels: lanes of the first PMA in the
+    // wave combine theirs, the others (rare) go alone (the levels into the
     // workgroup's replica of the PMA's mask)
     const u32 f = dpp_reduce32<false>(local_flags);
-    if (lane_id() == 0 && f) atomicOr(&status->flags, f);
+    if (lone) {
+        __shared__ u32 s_flags;
+        if (threadIdx.x == 0) s_flags = 0;
+        __syncthreads();
+        if (lane_id() == 0 && f) atomicOr(&s_flags, f);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_flags) {
+            atomicOr(&status->flags, s_flags);
+            atomicOr(&g.pstat[0].flags, s_flags);
+        }
+        local_flags = 0;  // raised
+    } else if (lane_id() == 0 && f) {
+        atomicOr(&status->flags, f);
+    }
     const u32 pid0 = __shfl(pid, 0, 64);
     const u32 rep = blockIdx.x & (LV_REPL - 1);
     const u32 lv = dpp_reduce32<false>(pid == pid0 ? lvbit : 0u);
