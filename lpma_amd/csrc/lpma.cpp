@@ -964,9 +964,14 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     u64 *pk;
     {
         PhaseTimer t(pl, st, PH_SORT);
-        HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), bb.wfirst, st));
-        pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
-                           pl.ukey.p, pl.sort_scr.p, &bb.status->flags, st);
+        if (npma == 1 && nsegs <= INLINE_SEGS && sort_carries_first(n)) {
+            pk = pack_and_sort_first(g, bb.status, bb.wfirst, in, n, pshift, sbits, hi_bit, pl.pk0.p, pl.pk1.p,
+                                     pl.ukey.p, pl.sort_scr.p, st);
+        } else {
+            HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), bb.wfirst, st));
+            pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
+                               pl.ukey.p, pl.sort_scr.p, &bb.status->flags, st);
+        }
         HIPCHECK(hipGetLastError());
     }
     {

@@ -15,6 +15,7 @@
 //   larger         LSD radix, 8-bit digits, histogram + scan + scatter.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "begin.hpp"
 
 namespace pma {
 
@@ -422,6 +423,9 @@ static constexpr int SB_TPB = 1024;                 // bucket sort workgroup
 static constexpr int SB_ITEMS = 4;
 static constexpr u32 SB_CAP = SB_TPB * SB_ITEMS;    // keys sorted in LDS
 static constexpr u32 SB_NW = SB_TPB / 64;
+// a tile's count entry: offset << 16 | count, bit 31 on bucket 0's entry
+// when the tile met an invalid key
+static constexpr u32 SS_BAD = 1u << 31;
 
 // buckets of a batch of n keys: about a thousand keys each, a power of two
 static u32 ss_buckets(u64 n) {
@@ -441,19 +445,35 @@ __device__ __forceinline__ u64 pack_element(const BatchSeg &sg, u64 i, u32 pshif
 // keys, x >> psh with psh >= the bits below the key (request order,
 // operation): equal keys project equally, so they share a bucket, and any
 // non-decreasing splitters give a correct split.
+// FIRST: a lone PMA's batch of inline sources, whose first work (status,
+// update index, pyramid top) one more workgroup, past the tiles, does here
+// instead of a kernel of its own; the sources come from the kernel
+// arguments.  A tile that met an invalid key marks its count of bucket 0
+// (bit 31): the bucket kernel raises the flag, after the status is cleared.
+template <bool FIRST>
 __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
                                                       const u64 *n_dev, u32 pshift, u32 sbits, u32 psh, u32 nb,
-                                                      u64 *__restrict__ tmp, u32 *__restrict__ flags,
-                                                      u32 *__restrict__ bc) {
+                                                      u64 *__restrict__ tmp, u32 *__restrict__ bc, InlineSegs in,
+                                                      Group g, Status *status, u32 *wfirst) {
     __shared__ u64 smp[SS_SAMPLE];  // projection << 32 | sample index: distinct
     __shared__ u32 spl[SS_MAX_BUCKETS];
     __shared__ u32 cnt[SS_MAX_BUCKETS];
     __shared__ u32 base[SS_MAX_BUCKETS];
+    __shared__ BatchSeg lsegs[INLINE_SEGS];
     if (n_dev) n = *n_dev;
     const u32 t = blockIdx.x;
     const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
-    if (t >= ntiles) return;  // whole workgroup: grids sized for a capacity
     const u32 tid = threadIdx.x;
+    if (FIRST) {
+        if (t == ntiles) {
+            batch_first_work(g, status, 0, nullptr, 0, wfirst);
+            return;
+        }
+        if (tid < in.n) lsegs[tid] = in.s[tid];
+        __syncthreads();
+        segs = lsegs;
+    }
+    if (t >= ntiles) return;  // whole workgroup: grids sized for a capacity
 
     // the tile's keys and the sample key are loaded first, all in flight
     // while the sample is ranked
@@ -514,8 +534,7 @@ __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict_
         bk[r] = lo;
         loc[r] = valid[r] ? atomicAdd(&cnt[lo], 1u) : 0u;
     }
-    if (__any(bad) && lane_id() == 0) atomicOr(flags, FLAG_INVALID_KEY);
-    __syncthreads();
+    const bool tbad = __syncthreads_or(bad);
     if (tid < 64) {
         // exclusive scan of the bucket counts (nb <= 128: two per lane)
         const u32 c0 = 2 * tid < nb ? cnt[2 * tid] : 0u, c1 = 2 * tid + 1 < nb ? cnt[2 * tid + 1] : 0u;
@@ -527,7 +546,7 @@ __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict_
 #pragma unroll
     for (int r = 0; r < SS_ITEMS; ++r)
         if (valid[r]) tmp[tbase + base[bk[r]] + loc[r]] = key[r];
-    if (tid < nb) bc[(u64)tid * ntiles + t] = (base[tid] << 16) | cnt[tid];
+    if (tid < nb) bc[(u64)tid * ntiles + t] = (base[tid] << 16) | cnt[tid] | (tid == 0 && tbad ? SS_BAD : 0u);
 }
 
 // Keys of a workgroup are spread over every wave: with R = ceil(m / SB_TPB)
@@ -595,7 +614,7 @@ __device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 R, 
 
 __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tmp, u64 *__restrict__ out,
                                                        u64 *__restrict__ aux, u64 n, const u64 *n_dev, u32 nb,
-                                                       const u32 *__restrict__ bc) {
+                                                       const u32 *__restrict__ bc, u32 *__restrict__ flags) {
     __shared__ u64 kb[1][SB_CAP];
     __shared__ u32 whist[SB_NW][256];
     __shared__ u32 dofs[256];
@@ -634,11 +653,12 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
     u32 m;
     {
         const u32 e = tid < ntiles ? bc[(u64)b * ntiles + tid] : 0u;
+        if (b == 0 && __syncthreads_or((e & SS_BAD) != 0u) && tid == 0) atomicOr(flags, FLAG_INVALID_KEY);
         u32 tot;
         const u32 ex = block_excl_scan<SB_TPB, u32>(e & 0xFFFFu, tot, stmp);
         if (tid < ntiles) {
             toff[tid] = ex;
-            tb[tid] = e >> 16;
+            tb[tid] = (e & ~SS_BAD) >> 16;
         }
         if (tid == 0) toff[ntiles] = tot;
         m = tot;
@@ -927,8 +947,9 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
         u32 *bc = scratch + OS_GH_WORDS;
         const u32 ks = 1 + sbits;
         const u32 psh = hi_bit > 32 + ks ? hi_bit - 32 : ks;
-        k_ss_split<<<ntiles, SS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, psh, nb, k1, flags, bc);
-        k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, n_dev, nb, bc);
+        k_ss_split<false><<<ntiles, SS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, psh, nb, k1, bc,
+                                                      InlineSegs{}, Group{}, nullptr, nullptr);
+        k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, n_dev, nb, bc, flags);
         return k0;
     }
     if (n <= OS_MAX_N) {
@@ -960,6 +981,25 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
         u64 *t = src; src = dst; dst = t;
     }
     return src;
+}
+
+bool sort_carries_first(u64 n) { return n > 0 && n <= SS_MAX_N; }
+
+// The sample-split sort of a lone PMA's batch of at most SS_MAX_N keys from
+// inline sources, with the batch's first work (batch_begin without its
+// sort-scratch clearing, which the sample-split sort does not need) in one
+// more workgroup of the split kernel.
+u64 *pack_and_sort_first(const Group &g, Status *status, u32 *wfirst, const InlineSegs &in, u64 n, u32 pshift,
+                         u32 sbits, u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, hipStream_t st) {
+    const u32 nb = ss_buckets(n);
+    const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
+    u32 *bc = scratch + OS_GH_WORDS;
+    const u32 ks = 1 + sbits;
+    const u32 psh = hi_bit > 32 + ks ? hi_bit - 32 : ks;
+    k_ss_split<true><<<ntiles + 1, SS_TPB, 0, st>>>(nullptr, in.n, n, nullptr, pshift, sbits, psh, nb, k1, bc, in, g,
+                                                     status, wfirst);
+    k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, nullptr, nb, bc, &status->flags);
+    return k0;
 }
 
 }  // namespace pma

@@ -26,6 +26,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "begin.hpp"
 
 namespace pma {
 
@@ -1872,7 +1873,6 @@ hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStre
 // segments through LDS (levels 1..11), a single workgroup finishes the
 // levels above.  O(nseg) streaming work, no atomics.
 
-static constexpr u32 PYR_CHL = 11;
 static constexpr u32 PYR_CH = 1u << PYR_CHL;
 
 __device__ void pyr_chunk(const DevState &d, u64 chunk) {
@@ -1922,68 +1922,12 @@ __global__ __launch_bounds__(256) void k_pyr_chunk_group(Group g) {
     }
 }
 
-// Levels above the chunk levels, by one workgroup.  Up to PYR_TOP_LDS
-// chunk-level counts (H <= 23) are loaded once into LDS and every level
-// above is summed there (no global round trip per level), laid out as in
-// the pyramid (each level right after the one below), so the levels are
-// written out together at the end.  Taller pyramids go level by level
-// through global memory.
-static constexpr u32 PYR_TOP_LDS = 4096;
-
-__device__ void pyr_top_levels(const DevState &d) {
-    __shared__ u32 v[2 * PYR_TOP_LDS];
-    const u64 nseg = nseg_of(d);
-    if (d.H <= PYR_CHL) return;
-    const u64 nc = nseg >> PYR_CHL;
-    if (nc <= PYR_TOP_LDS) {
-        const u32 *in = d.pyr + pyr_off(nseg, PYR_CHL);
-        for (u32 x = threadIdx.x; x < (u32)nc; x += blockDim.x) v[x] = in[x];
-        __syncthreads();
-        u32 lo = 0, n = (u32)nc;  // the level below: v[lo, lo + n)
-        for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
-            for (u32 x = threadIdx.x; x < n / 2; x += blockDim.x) v[lo + n + x] = v[lo + 2 * x] + v[lo + 2 * x + 1];
-            lo += n;
-            n /= 2;
-            __syncthreads();
-        }
-        // levels PYR_CHL + 1 .. H: v[nc, 2 nc - 1), contiguous in the pyramid too
-        u32 *out = d.pyr + pyr_off(nseg, PYR_CHL + 1);
-        for (u32 x = threadIdx.x; x < (u32)nc - 1; x += blockDim.x) out[x] = v[nc + x];
-        return;
-    }
-    for (u32 l = PYR_CHL + 1; l <= d.H; ++l) {
-        u64 nw = nseg >> l;
-        const u32 *in = d.pyr + pyr_off(nseg, l - 1);
-        u32 *out = d.pyr + pyr_off(nseg, l);
-        for (u64 x = threadIdx.x; x < nw; x += blockDim.x) out[x] = in[2 * x] + in[2 * x + 1];
-        __syncthreads();
-    }
-}
-
 __global__ __launch_bounds__(1024) void k_pyr_top(DevState d) { pyr_top_levels(d); }
 
 __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, InlineSegs in, BatchSeg *segs, u32 *zero,
                                                       u32 nzero, u32 *wfirst) {
-    if (blockIdx.x == 0) {
-        for (u32 i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
-        if (g.npma == 1)
-            for (u32 i = threadIdx.x; i <= WFIRST_N; i += blockDim.x) wfirst[i] = 0xFFFFFFFFu;
-        if (threadIdx.x == 0) {
-            s->flags = 0;
-            s->n_eff = 0;
-            s->delta = 0;
-            for (int c = 0; c < NUM_CLASSES; ++c) s->nwork[c] = 0;
-        }
-        if (threadIdx.x < in.n) segs[threadIdx.x] = in.s[threadIdx.x];
-    }
-    if (threadIdx.x == 0) {
-        PmaStat z;
-        z.flags = 0; z.u0 = 0; z.u1 = 0; z.pd0 = 0; z.pd1 = 0;
-        g.pstat[blockIdx.x] = z;
-    }
-    if (threadIdx.x < LV_REPL) g.lvmask[(u64)blockIdx.x * LV_REPL + threadIdx.x] = 0;
-    const DevState d = pma_of(g, blockIdx.x);
-    pyr_top_levels(d);
+    if (blockIdx.x == 0 && threadIdx.x < in.n) segs[threadIdx.x] = in.s[threadIdx.x];
+    batch_first_work(g, s, blockIdx.x, zero, nzero, wfirst);
 }
 
 // chunk levels of the PMAs of a run (their top levels follow in the next

@@ -13,8 +13,10 @@
         writes); a device copy of known size in the same program checks the
         method.
 
-An update batch is the run of dispatches from one k_batch_begin to the next.
-A batch whose sort took the large-batch radix kernels is a bulk load (the
+An update batch is the run of dispatches from its first kernel to the next
+batch's: k_batch_begin, or the sample-split sort's k_ss_split where that
+carries the batch's first work (a lone PMA's batch of up to 2^17 keys).  A
+batch whose sort took the large-batch radix kernels is a bulk load (the
 update batches sort with the sample-split kernels) and is left out.
 """
 import argparse
@@ -42,7 +44,8 @@ def batches(dispatches):
     (update batches, bulk batches, dispatches before the first batch)."""
     segs, cur, pre = [], [], []
     for d in dispatches:
-        if d[0] == "k_batch_begin":
+        first = d[0] == "k_batch_begin" or (d[0] == "k_ss_split" and not (cur and cur[-1][0] == "k_batch_begin"))
+        if first:
             if cur:
                 segs.append(cur)
             cur = [d]
@@ -71,8 +74,9 @@ def cmd_kernels(a):
             per[k].append((e - s) / 1e3)
     tot = sum(sum(v) for v in per.values())
     lines = [f"# {a.title}", "", "Command (repo root, one MI355X):", "", "```", a.command, "```", "",
-             f"Launch durations from the kernel trace over {len(upd)} update batches (each from one "
-             f"`k_batch_begin` to the next); the {len(bulk)} bulk loads' batches are left out.  "
+             f"Launch durations from the kernel trace over {len(upd)} update batches (each from its first "
+             f"kernel, `k_batch_begin` or the sort's `k_ss_split`, to the next batch's); the {len(bulk)} bulk "
+             f"loads' batches are left out.  "
              "The trace slows the launch chain (gaps between kernels), not the kernels.", "",
              "| kernel | launches | avg us | median us | p99 us | max us | total ms | share |",
              "|---|---:|---:|---:|---:|---:|---:|---:|"]

@@ -103,12 +103,14 @@ def test_invalid_keys_leave_structure_unchanged():
     before = L.to_host_u64(t.pma.copy_view()[0])
     for bad in (keys_of([100], [0]), keys_of([0], [100]), keys_of([5], [0xFFFFFFFF]),
                 np.array([L.KEY_NONE], dtype=np.uint64)):
-        k = np.concatenate([rand_keys(rng, 100, 20), bad])
-        with pytest.raises(L.LpmaError) as e:
-            t.pma.insert(k)
-        assert e.value.code == 2
-        with pytest.raises(L.LpmaError):
-            t.pma.delete(k)
+        # one sort tile, then the invalid key in a later tile of a batch of several
+        for head, tail in ((20, 0), (3000, 2000)):
+            k = np.concatenate([rand_keys(rng, 100, head), bad, rand_keys(rng, 100, tail)])
+            with pytest.raises(L.LpmaError) as e:
+                t.pma.insert(k)
+            assert e.value.code == 2
+            with pytest.raises(L.LpmaError):
+                t.pma.delete(k)
     after = L.to_host_u64(t.pma.copy_view()[0])
     assert np.array_equal(before, after)
     t.compare()
