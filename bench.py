@@ -731,9 +731,13 @@ def headline(args):
     lat = r["lat_ms"]
     st, pf = rp["stats"], rp["prof"]
     # dominant rebalance kernel: the one that moves the most algorithmic bytes
+    # (a lone PMA's one-wave windows run inside the single-segment launch,
+    # which then has no wave-class launch of its own)
+    wave_in_seg = pf.launches_wave == 0
     kern = {
-        "k_rebalance_seg": (pf.rebalance_seg_ms, st.alg_bytes_seg, pf.launches_seg),
-        "k_rebalance_wave": (pf.rebalance_wave_ms, st.alg_bytes_wave, pf.launches_wave),
+        "k_rebalance_seg": (pf.rebalance_seg_ms, st.alg_bytes_seg + (st.alg_bytes_wave if wave_in_seg else 0),
+                            pf.launches_seg),
+        "k_rebalance_wave": (pf.rebalance_wave_ms, 0 if wave_in_seg else st.alg_bytes_wave, pf.launches_wave),
         "k_rebalance_block": (pf.rebalance_block_ms, st.alg_bytes_block, pf.launches_block),
     }
     name = max(kern, key=lambda k: kern[k][1])

@@ -67,6 +67,10 @@ hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStr
 // the device (launches queued before the host knows it)
 hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0 = nullptr,
                                hipEvent_t e1 = nullptr);
+// the same launch followed, in its own workgroups, by the one-wave windows
+// (a lone PMA's early rebalance: one launch for both classes)
+hipError_t rebalance_seg_wave_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st,
+                                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 u32 seg_lanes_log2(u32 log2S, int slots_log2);
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st);
 u32 pyramid_blocks(u32 H);

@@ -201,20 +201,9 @@ struct Pipeline {
     u64 launches_wave = 0, launches_block = 0, launches_seg = 0;
 
     hipEvent_t ev_status = nullptr;  // the round's status has reached the host
-    u32 last_wave_windows = 0;       // wave-class windows of the last round
 };
 
 static constexpr u64 STATUS_BYTES = (sizeof(Status) + 15) / 16 * 16;
-// workgroups of the wave-class rebalance queued before its window count is
-// known (each loops over windows): the last round's count with a quarter
-// more (a round that rebuilt a PMA leaves the count of the round before),
-// at least 1024, at most one per batch key and 16384
-static u32 early_wave_grid(const Pipeline &pl, u64 n) {
-    u64 g = (u64)pl.last_wave_windows + pl.last_wave_windows / 4 + 64;
-    g = std::max<u64>(g, 1024);
-    g = std::min<u64>(g, 16384);
-    return (u32)std::max<u64>(1, std::min<u64>(g, n));
-}
 
 // ------------------------------------------------------------------ handle
 
@@ -719,8 +708,6 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
                  hipStream_t st, bool early = false) {
     Status s = *(const Status *)pl.h_out;
     const PmaStat *ps = (const PmaStat *)(pl.h_out + STATUS_BYTES);
-    // (a rebuild elects no windows for its PMA: not a guide to the next round)
-    if (!(s.flags & (FLAG_ROOT | FLAG_RESIZE))) pl.last_wave_windows = s.nwork[CLASS_WAVE];
     if (s.flags & FLAG_INVALID_KEY) return LPMA_ERR_INVALID_KEY;
     if (s.flags & FLAG_SORT_STALL) return LPMA_ERR_DEVICE;
     const u64 n_eff = s.n_eff;
@@ -998,13 +985,8 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
         HIPCHECK(hipEventRecord(pl.ev_status, st));
         {
             LaunchTimer t(pl, PH_SEG);
-            HIPCHECK(rebalance_seg_bound(g, bb, n, st, t.a, t.b));
+            HIPCHECK(rebalance_seg_wave_bound(g, bb, n, st, t.a, t.b));
             pl.launches_seg++;
-        }
-        {
-            LaunchTimer t(pl, PH_WAVE);
-            HIPCHECK(rebalance_lds(g, bb, CLASS_WAVE, early_wave_grid(pl, n), st, t.a, t.b));
-            pl.launches_wave++;
         }
         HIPCHECK(hipEventSynchronize(pl.ev_status));
     } else {

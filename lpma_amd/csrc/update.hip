@@ -857,6 +857,14 @@ __device__ void rebalance_loop(const Group &g, const Work *__restrict__ work, co
 // front-packed, gaps hold KEY_NONE), so neither the segment counts nor a scan
 // sit on the dependency chain: the slot loads and the update loads are
 // issued together, then one LDS merge, then the coalesced store.
+// The lanes of one wave see each other's LDS stores: the barrier of code
+// that each wave of a workgroup runs on its own (a one-wave window).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u64 *__restrict__ ckey,
                                       const i32 *__restrict__ pd, u64 *A, u64 *B, i32 *P, BlockStats &bs) {
     const u32 lane = lane_id();
@@ -890,7 +898,7 @@ __device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u
             off += __popcll(m);
         }
     }
-    __syncthreads();
+    wave_sync();
 
     const u32 M = (u32)((i32)C + P[k]);
     const u64 *U = A + C;
@@ -914,7 +922,7 @@ __device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u
         }
         B[lo + P[q]] = y;
     }
-    __syncthreads();
+    wave_sync();
 
     u32 markers = 0;
     for (u32 t = lane; t < W; t += 64) {
@@ -938,11 +946,11 @@ __device__ void rebalance_window_wave(const DevState &d, const Work &wk, const u
     markers = wave_sum(markers);
     if (lane == 0) {
         bs.markers += markers;
-        bs.windows[L] += 1;
+        atomicAdd(&bs.windows[L], 1u);  // the waves of a workgroup share it
         bs.updates += k;
         bs.bytes += 16ull * W + 8ull * k;
     }
-    __syncthreads();
+    wave_sync();
 }
 
 __global__ __launch_bounds__(64) void k_rebalance_wave(Group g, const Work *__restrict__ work,
@@ -992,17 +1000,63 @@ static constexpr int SEG_TPB = 256;
 // over windows (fewer, longer-lived workgroups keep more of the device busy
 // than one short workgroup per 32 windows)
 static constexpr u32 SEG_GRID_MAX = 1024;
+// workgroups of four waves for a lone PMA's one-wave windows in the same
+// launch (resident beside the single-segment ones: 1536 workgroups of at
+// most 21 KB of LDS)
+static constexpr u32 EARLY_WAVE_GRID = 512;
 
 // u32 words of a group's LDS slice: the updates (k <= 2S, u64) and their
 // prefixes (k + 1 <= 2S + 1, i32), padded so that the slices of a wave's
 // groups start on different banks
 __host__ __device__ constexpr u32 seg_slice_words(u32 S) { return 4 * S + (2 * S + 2) + 16; }
 
+// With wwork, a lone PMA's one-wave windows run in the same launch, each
+// wave on its own (the LDS then holds one window's arrays per wave): the
+// workgroups past the first sgrid take the first of them while the first
+// sgrid take the single-segment windows, and these take the rest after
+// theirs (wave w of the launch's one-wave order: the extra workgroups'
+// waves, then the others').
+static constexpr u32 WAVE_LDS_BYTES = 2 * WAVE_E_CAP * 8 + (WAVE_E_CAP + 1) * 4 + 4;  // A, B, P (8-aligned)
+
+// one-wave windows i = first + wave, first + wave + stride, ... of a
+// workgroup, each of its waves on its own
+__device__ void rebalance_waves_of(const Group &g, const Work *__restrict__ wwork, const Status *__restrict__ status,
+                                   const u64 *__restrict__ ckey, const i32 *__restrict__ pd, unsigned char *lds,
+                                   u32 first, u32 stride) {
+    __shared__ u32 wl[32];
+    if (threadIdx.x < 32) wl[threadIdx.x] = 0;
+    __syncthreads();
+    BlockStats bs;
+    bs.updates = bs.markers = bs.bytes = 0;
+    bs.windows = wl;
+    const u32 wv = threadIdx.x >> 6;
+    u64 *A = (u64 *)(lds + (u64)wv * WAVE_LDS_BYTES);
+    u64 *B = A + WAVE_E_CAP;
+    i32 *P = (i32 *)(B + WAVE_E_CAP);
+    const u32 nww = status->nwork[CLASS_WAVE];
+    for (u32 i = first + wv; i < nww; i += stride) {
+        const Work wk = wwork[i];
+        const DevState d = pma_of(g, wk.pid);
+        rebalance_window_wave(d, wk, ckey, pd, A, B, P, bs);
+    }
+    // per wave (lane 0 holds its wave's sums), the window counts per workgroup
+    u64 *row = g.stats + (u64)(blockIdx.x % ST_ROWS) * ST_NUM;
+    if (lane_id() == 0 && bs.bytes) {
+        atomicAdd((unsigned long long *)&row[ST_UPDATES], (unsigned long long)bs.updates);
+        atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)bs.markers);
+        atomicAdd((unsigned long long *)&row[ST_ALG_BYTES_WAVE], (unsigned long long)(bs.bytes + 4ull * bs.markers));
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 && wl[threadIdx.x])
+        atomicAdd((unsigned long long *)&row[ST_WINDOWS + threadIdx.x], (unsigned long long)wl[threadIdx.x]);
+}
+
 template <int SPL>
 __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *__restrict__ work,
                                                            const Status *__restrict__ status,
                                                            const u64 *__restrict__ ckey, const i32 *__restrict__ pd,
-                                                           u32 log2G, u32 nw_host) {
+                                                           u32 log2G, u32 nw_host, const Work *__restrict__ wwork,
+                                                           u32 sgrid) {
     extern __shared__ u64 lds[];
     __shared__ u64 red[4];
     const u32 G = 1u << log2G;
@@ -1015,11 +1069,18 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
     // the group's lanes within the wave
     const u32 gshift = (threadIdx.x & 63) & ~(G - 1);
     const u64 gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gshift;
+    constexpr u32 WPB = SEG_TPB / 64;
+    if (wwork && blockIdx.x >= sgrid) {
+        rebalance_waves_of(g, wwork, status, ckey, pd, (unsigned char *)lds, (blockIdx.x - sgrid) * WPB,
+                           gridDim.x * WPB);
+        return;
+    }
     if (threadIdx.x < 4) red[threadIdx.x] = 0;
     // the host passes the window count when it knows it
     const u32 nw = nw_host != ~0u ? nw_host : status->nwork[CLASS_SEG];
     u64 bytes = 0, upd = 0, markers = 0, wins = 0;
-    for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += gridDim.x * groups_per_block) {
+    const u32 nblk = wwork ? sgrid : gridDim.x;
+    for (u32 w = blockIdx.x * groups_per_block + grp; w < nw; w += nblk * groups_per_block) {
         const Work wk = work[w];
         const DevState d = pma_of(g, wk.pid);
         const u32 C = wk.C;
@@ -1114,6 +1175,11 @@ __global__ __launch_bounds__(SEG_TPB) void k_rebalance_seg(Group g, const Work *
         atomicAdd((unsigned long long *)&row[ST_MARKERS], (unsigned long long)red[2]);
         atomicAdd((unsigned long long *)&row[ST_ALG_BYTES_SEG], (unsigned long long)red[3]);
     }
+    if (wwork) {
+        __syncthreads();  // every group is done with its LDS slice
+        rebalance_waves_of(g, wwork, status, ckey, pd, (unsigned char *)lds, (gridDim.x - sgrid + blockIdx.x) * WPB,
+                           gridDim.x * WPB);
+    }
 }
 
 // lanes per window (log2) of the single-segment rebalance: `slots_log2`
@@ -1132,33 +1198,39 @@ u32 seg_lanes_log2(u32 log2S, int slots_log2) {
 // one single-segment launch; with events, the dispatch itself records its
 // start and end in them (the kernel's execution time, no separate packets)
 template <int SPL>
-static void seg_dispatch(u32 grid, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1, const Group &g,
-                         const BatchBufs &b, u32 log2G, u32 nw) {
+static void seg_dispatch(u32 grid, u32 wgrid, size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                         const Group &g, const BatchBufs &b, u32 log2G, u32 nw, const Work *wwork) {
     if (e0)
-        hipExtLaunchKernelGGL(k_rebalance_seg<SPL>, dim3(grid), dim3(SEG_TPB), (u32)lds, st, e0, e1, 0u, g,
+        hipExtLaunchKernelGGL(k_rebalance_seg<SPL>, dim3(grid + wgrid), dim3(SEG_TPB), (u32)lds, st, e0, e1, 0u, g,
                               (const Work *)b.work[CLASS_SEG], (const Status *)b.status, (const u64 *)b.ckey,
-                              (const i32 *)b.pd, log2G, nw);
+                              (const i32 *)b.pd, log2G, nw, wwork, grid);
     else
-        k_rebalance_seg<SPL><<<grid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G, nw);
+        k_rebalance_seg<SPL><<<grid + wgrid, SEG_TPB, lds, st>>>(g, b.work[CLASS_SEG], b.status, b.ckey, b.pd, log2G,
+                                                                 nw, wwork, grid);
 }
 
 // `windows`: the exact window count (eager rounds) or a bound that sizes the
 // grid, the kernel then reading the count from the status (graph replays)
 static hipError_t rebalance_seg_launch(const Group &g, const BatchBufs &b, u64 windows, bool exact, hipStream_t st,
-                                       hipEvent_t e0, hipEvent_t e1) {
+                                       hipEvent_t e0, hipEvent_t e1, bool with_wave = false) {
     u32 S = 1u << g.log2S;
     u32 log2G = g.seg_log2G;
     u32 groups_per_block = SEG_TPB >> log2G;
     size_t lds = (size_t)groups_per_block * seg_slice_words(S) * sizeof(u32);
+    if (with_wave) lds = std::max<size_t>(lds, (size_t)(SEG_TPB / 64) * WAVE_LDS_BYTES);
+    const Work *wwork = with_wave ? b.work[CLASS_WAVE] : nullptr;
+    // one-wave windows: up to EARLY_WAVE_GRID workgroups of their own (their
+    // count is on the device), at most a wave per 64 batch keys
+    const u32 wgrid = with_wave ? (u32)std::max<u64>(1, std::min<u64>(EARLY_WAVE_GRID, (windows + 255) / 256)) : 0u;
     u64 blocks = (windows + groups_per_block - 1) / groups_per_block;
     u32 grid = (u32)(blocks < 1 ? 1 : (blocks > SEG_GRID_MAX ? SEG_GRID_MAX : blocks));
     u32 nw = exact ? (u32)windows : ~0u;
     switch (S >> log2G) {
-        case 1: seg_dispatch<1>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
-        case 2: seg_dispatch<2>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
-        case 4: seg_dispatch<4>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
-        case 8: seg_dispatch<8>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
-        case 16: seg_dispatch<16>(grid, lds, st, e0, e1, g, b, log2G, nw); break;
+        case 1: seg_dispatch<1>(grid, wgrid, lds, st, e0, e1, g, b, log2G, nw, wwork); break;
+        case 2: seg_dispatch<2>(grid, wgrid, lds, st, e0, e1, g, b, log2G, nw, wwork); break;
+        case 4: seg_dispatch<4>(grid, wgrid, lds, st, e0, e1, g, b, log2G, nw, wwork); break;
+        case 8: seg_dispatch<8>(grid, wgrid, lds, st, e0, e1, g, b, log2G, nw, wwork); break;
+        case 16: seg_dispatch<16>(grid, wgrid, lds, st, e0, e1, g, b, log2G, nw, wwork); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1172,6 +1244,11 @@ hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStr
 hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0,
                                hipEvent_t e1) {
     return rebalance_seg_launch(g, b, bound, false, st, e0, e1);
+}
+
+hipError_t rebalance_seg_wave_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0,
+                                    hipEvent_t e1) {
+    return rebalance_seg_launch(g, b, bound, false, st, e0, e1, true);
 }
 
 hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, hipStream_t st, hipEvent_t e0,
