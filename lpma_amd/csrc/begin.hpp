@@ -11,6 +11,33 @@ namespace pma {
 // batch's start
 static constexpr u32 PYR_CHL = 11;
 
+static constexpr u32 PYR_CH = 1u << PYR_CHL;
+
+// Levels 1..PYR_CHL of one chunk of 2^PYR_CHL segments (all levels of a
+// smaller PMA), by one workgroup of 256 threads, through LDS.
+__device__ inline void pyr_chunk(const DevState &d, u64 chunk) {
+    __shared__ u32 v0[PYR_CH];
+    __shared__ u32 v1[PYR_CH / 2];
+    const u64 nseg = nseg_of(d);
+    const u32 levels = d.H < PYR_CHL ? d.H : PYR_CHL;
+    const u32 n = (u32)(nseg < PYR_CH ? nseg : PYR_CH);
+    const u64 c0 = chunk * n;
+    for (u32 i = threadIdx.x; i < n; i += 256) v0[i] = d.cnt[c0 + i];
+    __syncthreads();
+    u32 *in = v0, *tmp = v1;
+    for (u32 l = 1; l <= levels; ++l) {
+        u32 nw = n >> l;
+        u32 *out = d.pyr + pyr_off(nseg, l) + (c0 >> l);
+        for (u32 x = threadIdx.x; x < nw; x += 256) {
+            u32 s = in[2 * x] + in[2 * x + 1];
+            tmp[x] = s;
+            out[x] = s;
+        }
+        __syncthreads();
+        u32 *sw = in; in = tmp; tmp = sw;
+    }
+}
+
 // Levels above the chunk levels, by one workgroup.  Up to PYR_TOP_LDS
 // chunk-level counts (H <= 23) are loaded once into LDS and every level
 // above is summed there (no global round trip per level), laid out as in
@@ -50,10 +77,12 @@ __device__ inline void pyr_top_levels(const DevState &d) {
 }
 
 // One workgroup's share of a batch's first work for PMA pid: its per-batch
-// record and claimed-level masks cleared, its pyramid's top levels summed;
-// the workgroup of PMA 0 also clears the status, zero[0, nzero) and, for a
-// lone PMA, resets the update index wfirst.
-__device__ inline void batch_first_work(const Group &g, Status *s, u32 pid, u32 *zero, u32 nzero, u32 *wfirst) {
+// record and claimed-level masks cleared, its pyramid's top levels summed
+// (unless top is false: then a later kernel sums them); the workgroup of
+// PMA 0 also clears the status, zero[0, nzero) and, for a lone PMA, resets
+// the update index wfirst.
+__device__ inline void batch_first_work(const Group &g, Status *s, u32 pid, u32 *zero, u32 nzero, u32 *wfirst,
+                                        bool top = true) {
     if (pid == 0) {
         for (u32 i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
         if (g.npma == 1)
@@ -71,8 +100,7 @@ __device__ inline void batch_first_work(const Group &g, Status *s, u32 pid, u32 
         g.pstat[pid] = z;
     }
     if (threadIdx.x < LV_REPL) g.lvmask[(u64)pid * LV_REPL + threadIdx.x] = 0;
-    const DevState d = pma_of(g, pid);
-    pyr_top_levels(d);
+    if (top) pyr_top_levels(pma_of(g, pid));
 }
 
 }  // namespace pma

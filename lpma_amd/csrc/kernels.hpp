@@ -41,10 +41,12 @@ struct InlineSegs {
 
 // sort.hip: a lone PMA's batch of at most 2^17 keys from inline sources is
 // sorted with the batch's first work (batch_begin's, without clearing the
-// sort scratch) carried by the split kernel: no batch_begin launch
+// sort scratch) carried by the sort kernels: no batch_begin launch, and the
+// pyramid's pyr_chunks chunk levels owed by the last batch summed there too
 bool sort_carries_first(u64 n);
 u64 *pack_and_sort_first(const Group &g, Status *status, u32 *wfirst, const InlineSegs &in, u64 n, u32 pshift,
-                         u32 sbits, u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, hipStream_t st);
+                         u32 sbits, u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, u32 pyr_chunks,
+                         hipStream_t st);
 
 u32 locate_blocks(u64 n);
 // also clears zero[0, nzero) (the sort's digit totals)

@@ -220,6 +220,10 @@ struct lpma {
     u32 V = 0, vbits = 1, log2S = 5;
     u64 live = 0;
     u64 version = 0;
+    // the count pyramid's chunk levels are owed: the last batch changed
+    // segment counts and left them to the next batch's first kernels (a
+    // lone PMA's early path); any other path sums them before its batch
+    bool pyr_owed = false;
     u32 last_epoch = 0;  // epoch of the last run that updated this PMA
     DevState st;
     Density dn;
@@ -507,6 +511,7 @@ void release_geometry(lpma *h, Geometry &g) {
 
 // installs g as h's arrays (g is left empty)
 void install_geometry(lpma *h, u32 H, Geometry &g) {
+    h->pyr_owed = false;  // new arrays come with their pyramid
     if (h->keep_spares && h->keys.p) {
         Geometry old;
         old.keys = h->keys; old.cnt = h->cnt; old.pyr = h->pyr; old.sep = h->sep; old.owner = h->owner;
@@ -671,6 +676,19 @@ int begin_epoch(const Member *mem, u32 npma, u32 &epoch, hipStream_t st) {
 
 // Device table of the members (uploaded only when it changed); returns the
 // workgroups of the group pyramid launch.
+// the chunk levels a member's last batch left owing (the batch-begin kernel
+// then sums the levels above them)
+hipError_t pay_owed_pyramids(const Member *mem, u32 npma, hipStream_t st) {
+    for (u32 p = 0; p < npma; ++p) {
+        lpma *h = mem[p].h;
+        if (!h->pyr_owed) continue;
+        const hipError_t e = rebuild_pyramid(h->st, false, st);
+        if (e != hipSuccess) return e;
+        h->pyr_owed = false;
+    }
+    return hipSuccess;
+}
+
 int upload_table(Pipeline &pl, const Member *mem, u32 npma, u32 &pyr_blocks, hipStream_t st) {
     pyr_blocks = 0;
     for (u32 p = 0; p < npma; ++p) {
@@ -800,7 +818,11 @@ int finish_round(Pipeline &pl, const Member *mem, u32 npma, const Group &g_in, u
             PhaseTimer t(pl, st, PH_LARGE);
             HIPUNDO(rebalance_large(g, bb, pl.merged.p, 1024, st));
         }
-        {
+        if (early) {
+            // a lone PMA's next batch sums the chunk levels in its sort
+            // kernels (or pays them first on another path)
+            mem[0].h->pyr_owed = true;
+        } else {
             PhaseTimer t(pl, st, PH_PYR);
             HIPUNDO(group_pyramid(g, pyr_blocks, st));
         }
@@ -952,9 +974,13 @@ int run_pipeline(Pipeline &pl, const Member *mem, u32 npma, hipStream_t st) {
     {
         PhaseTimer t(pl, st, PH_SORT);
         if (npma == 1 && nsegs <= INLINE_SEGS && sort_carries_first(n)) {
+            lpma *h0 = mem[0].h;
+            const u32 chunks = h0->pyr_owed ? pyramid_blocks(h0->st.H) : 0u;
             pk = pack_and_sort_first(g, bb.status, bb.wfirst, in, n, pshift, sbits, hi_bit, pl.pk0.p, pl.pk1.p,
-                                     pl.ukey.p, pl.sort_scr.p, st);
+                                     pl.ukey.p, pl.sort_scr.p, chunks, st);
+            h0->pyr_owed = false;
         } else {
+            HIPCHECK(pay_owed_pyramids(mem, npma, st));
             HIPCHECK(batch_begin(g, bb.status, in, pl.segs.p, pl.sort_scr.p, sort_zero_words(), bb.wfirst, st));
             pk = pack_and_sort(pl.segs.p, nsegs, n, nullptr, pshift, sbits, lo_bit, hi_bit, pl.pk0.p, pl.pk1.p,
                                pl.ukey.p, pl.sort_scr.p, &bb.status->flags, st);
@@ -1123,6 +1149,7 @@ int run_pipeline_graph(Pipeline &pl, const Member *mem, u32 npma, u64 n, u32 psh
     u64 live_sum = 0;
     for (u32 p = 0; p < npma; ++p) live_sum += mem[p].h->live;
     CHECKED(prepare_graph_buffers(pl, cap, npma, live_sum + n));
+    HIPCHECK(pay_owed_pyramids(mem, npma, st));
 
     u32 epoch, pyr_blocks;
     CHECKED(begin_epoch(mem, npma, epoch, st));

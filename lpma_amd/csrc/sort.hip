@@ -446,10 +446,12 @@ __device__ __forceinline__ u64 pack_element(const BatchSeg &sg, u64 i, u32 pshif
 // operation): equal keys project equally, so they share a bucket, and any
 // non-decreasing splitters give a correct split.
 // FIRST: a lone PMA's batch of inline sources, whose first work (status,
-// update index, pyramid top) one more workgroup, past the tiles, does here
-// instead of a kernel of its own; the sources come from the kernel
-// arguments.  A tile that met an invalid key marks its count of bucket 0
-// (bit 31): the bucket kernel raises the flag, after the status is cleared.
+// update index) one more workgroup, past the tiles, does here instead of a
+// kernel of its own, and the next pyr_chunks workgroups the count pyramid's
+// chunk levels the last batch left owing (its top levels follow in the
+// bucket kernel); the sources come from the kernel arguments.  A tile that
+// met an invalid key marks its count of bucket 0 (bit 31): the bucket
+// kernel raises the flag, after the status is cleared.
 template <bool FIRST>
 __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict__ segs, u32 nsegs, u64 n,
                                                       const u64 *n_dev, u32 pshift, u32 sbits, u32 psh, u32 nb,
@@ -465,8 +467,11 @@ __global__ __launch_bounds__(SS_TPB) void k_ss_split(const BatchSeg *__restrict_
     const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
     const u32 tid = threadIdx.x;
     if (FIRST) {
-        if (t == ntiles) {
-            batch_first_work(g, status, 0, nullptr, 0, wfirst);
+        if (t >= ntiles) {
+            if (t == ntiles)
+                batch_first_work(g, status, 0, nullptr, 0, wfirst, false);
+            else
+                pyr_chunk(pma_of(g, 0), t - ntiles - 1);
             return;
         }
         if (tid < in.n) lsegs[tid] = in.s[tid];
@@ -612,9 +617,12 @@ __device__ __forceinline__ void sb_rank(const u64 (&x)[SB_ITEMS], u32 m, u32 R, 
     __syncthreads();
 }
 
+// top: one more workgroup, past the buckets, sums the lone PMA's pyramid
+// top levels (after the split kernel's chunk levels)
 __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tmp, u64 *__restrict__ out,
                                                        u64 *__restrict__ aux, u64 n, const u64 *n_dev, u32 nb,
-                                                       const u32 *__restrict__ bc, u32 *__restrict__ flags) {
+                                                       const u32 *__restrict__ bc, u32 *__restrict__ flags,
+                                                       bool top, DevState topd) {
     __shared__ u64 kb[1][SB_CAP];
     __shared__ u32 whist[SB_NW][256];
     __shared__ u32 dofs[256];
@@ -627,6 +635,10 @@ __global__ __launch_bounds__(SB_TPB) void k_ss_bucket(const u64 *__restrict__ tm
     const u32 b = blockIdx.x;
     const u32 tid = threadIdx.x;
     const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
+    if (top && b == nb) {
+        pyr_top_levels(topd);
+        return;
+    }
     if (b >= nb || ntiles == 0) return;
     // the bucket's offset: the keys of the buckets before it, whose
     // per-tile counts are the first b rows of bc
@@ -949,7 +961,7 @@ u64 *pack_and_sort(const BatchSeg *segs, u32 nsegs, u64 n, const u64 *n_dev, u32
         const u32 psh = hi_bit > 32 + ks ? hi_bit - 32 : ks;
         k_ss_split<false><<<ntiles, SS_TPB, 0, st>>>(segs, nsegs, n, n_dev, pshift, sbits, psh, nb, k1, bc,
                                                       InlineSegs{}, Group{}, nullptr, nullptr);
-        k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, n_dev, nb, bc, flags);
+        k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, n_dev, nb, bc, flags, false, DevState{});
         return k0;
     }
     if (n <= OS_MAX_N) {
@@ -988,17 +1000,20 @@ bool sort_carries_first(u64 n) { return n > 0 && n <= SS_MAX_N; }
 // The sample-split sort of a lone PMA's batch of at most SS_MAX_N keys from
 // inline sources, with the batch's first work (batch_begin without its
 // sort-scratch clearing, which the sample-split sort does not need) in one
-// more workgroup of the split kernel.
+// more workgroup of the split kernel, pyr_chunks more for the pyramid's
+// chunk levels the last batch left owing, and one more of the bucket kernel
+// for the pyramid's top levels.
 u64 *pack_and_sort_first(const Group &g, Status *status, u32 *wfirst, const InlineSegs &in, u64 n, u32 pshift,
-                         u32 sbits, u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, hipStream_t st) {
+                         u32 sbits, u32 hi_bit, u64 *k0, u64 *k1, u64 *k2, u32 *scratch, u32 pyr_chunks,
+                         hipStream_t st) {
     const u32 nb = ss_buckets(n);
     const u32 ntiles = (u32)((n + SS_TILE - 1) / SS_TILE);
     u32 *bc = scratch + OS_GH_WORDS;
     const u32 ks = 1 + sbits;
     const u32 psh = hi_bit > 32 + ks ? hi_bit - 32 : ks;
-    k_ss_split<true><<<ntiles + 1, SS_TPB, 0, st>>>(nullptr, in.n, n, nullptr, pshift, sbits, psh, nb, k1, bc, in, g,
-                                                     status, wfirst);
-    k_ss_bucket<<<nb, SB_TPB, 0, st>>>(k1, k0, k2, n, nullptr, nb, bc, &status->flags);
+    k_ss_split<true><<<ntiles + 1 + pyr_chunks, SS_TPB, 0, st>>>(nullptr, in.n, n, nullptr, pshift, sbits, psh, nb,
+                                                                  k1, bc, in, g, status, wfirst);
+    k_ss_bucket<<<nb + 1, SB_TPB, 0, st>>>(k1, k0, k2, n, nullptr, nb, bc, &status->flags, true, g.one);
     return k0;
 }
 

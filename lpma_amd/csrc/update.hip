@@ -1950,31 +1950,6 @@ hipError_t rebuild_spread(const DevState &dst, const u64 *merged, u64 m, hipStre
 // segments through LDS (levels 1..11), a single workgroup finishes the
 // levels above.  O(nseg) streaming work, no atomics.
 
-static constexpr u32 PYR_CH = 1u << PYR_CHL;
-
-__device__ void pyr_chunk(const DevState &d, u64 chunk) {
-    __shared__ u32 v0[PYR_CH];
-    __shared__ u32 v1[PYR_CH / 2];
-    const u64 nseg = nseg_of(d);
-    const u32 levels = d.H < PYR_CHL ? d.H : PYR_CHL;
-    const u32 n = (u32)(nseg < PYR_CH ? nseg : PYR_CH);
-    const u64 c0 = chunk * n;
-    for (u32 i = threadIdx.x; i < n; i += 256) v0[i] = d.cnt[c0 + i];
-    __syncthreads();
-    u32 *in = v0, *tmp = v1;
-    for (u32 l = 1; l <= levels; ++l) {
-        u32 nw = n >> l;
-        u32 *out = d.pyr + pyr_off(nseg, l) + (c0 >> l);
-        for (u32 x = threadIdx.x; x < nw; x += 256) {
-            u32 s = in[2 * x] + in[2 * x + 1];
-            tmp[x] = s;
-            out[x] = s;
-        }
-        __syncthreads();
-        u32 *sw = in; in = tmp; tmp = sw;
-    }
-}
-
 __global__ __launch_bounds__(256) void k_pyr_chunk(DevState d) { pyr_chunk(d, blockIdx.x); }
 
 // the chunks of every PMA of the run that took window rebalances
