@@ -715,7 +715,8 @@ int upload_table(Pipeline &pl, const Member *mem, u32 npma, u32 &pyr_blocks, hip
 // the pyramid have already been queued (graph replays); `early` that the
 // single-segment and wave classes were queued before the status was read
 // (a lone PMA: they only touch a PMA without a root rebuild or resize, whose
-// windows the election drops, and their scratch was reserved beforehand).
+// windows the election drops, and their scratch was reserved beforehand),
+// and that the pyramid's chunk levels are left owing to the next batch.
 //
 // Every fallible allocation of the round comes before its first change of a
 // PMA, so a round that fails with LPMA_ERR_OUT_OF_MEMORY changes nothing.  A

@@ -1983,7 +1983,7 @@ __global__ __launch_bounds__(1024) void k_batch_begin(Group g, Status *s, Inline
 }
 
 // chunk levels of the PMAs of a run (their top levels follow in the next
-// batch_begin); nblocks = the pyr_blk0 prefix total
+// batch's first kernels); nblocks = the pyr_blk0 prefix total
 hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
     k_pyr_chunk_group<<<nblocks < 2048 ? nblocks : 2048, 256, 0, st>>>(g);
@@ -1992,8 +1992,8 @@ hipError_t group_pyramid(const Group &g, u32 nblocks, hipStream_t st) {
 
 u32 pyramid_blocks(u32 H) { return H == 0 ? 0 : (H <= PYR_CHL ? 1 : 1u << (H - PYR_CHL)); }
 
-// full: also the levels above the chunks (otherwise the next batch_begin
-// computes them before anything reads them)
+// full: also the levels above the chunks (otherwise the next batch's first
+// kernels compute them before anything reads them)
 hipError_t rebuild_pyramid(const DevState &d, bool full, hipStream_t st) {
     if (d.H == 0) return hipSuccess;
     u64 nseg = nseg_of(d);
