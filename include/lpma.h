@@ -13,7 +13,10 @@
  * vectors in place, rpma.cuh:1557, :453).  The device view returned by
  * lpma_get_view is owned by the handle and stays valid until the next
  * mutating call on that handle.  One handle = one writer: calls on the same
- * handle must not overlap.  Errors are returned, never thrown across the ABI;
+ * handle must not overlap, and consecutive calls on it must be stream-ordered
+ * (the same stream, or the previous call's stream synchronised first: a call
+ * may return with its last kernels still queued).  Errors are returned, never
+ * thrown across the ABI;
  * a failed update leaves the structure unchanged (every allocation of an
  * update precedes its first change), except after LPMA_ERR_DEVICE: a HIP
  * runtime failure leaves the handle undefined, and it must be destroyed.
