@@ -1,8 +1,10 @@
 """Summaries of rocprofv3 output for the update path, written under profiles/.
 
-    python3 profile_summary.py kernels RESULTS_DB OUT_MD --command "..."
+    python3 profile_summary.py kernels RESULTS_DB OUT_MD --command "..." [--last N]
         per-kernel launch durations (a --kernel-trace sqlite database) over
-        the update batches, without the bulk loads' dispatches.
+        the update batches, without the bulk loads' dispatches; with --last,
+        over the last N batches whatever their sort (a workload whose timed
+        batches follow its bulk loads, such as the batched form's rounds).
 
     python3 profile_summary.py traffic OUT_JSON OUT_MD --config JSON --command "..." \
         FETCH_DIR WRITE_DIR RDREQ_DIR WRREQ_DIR
@@ -30,6 +32,8 @@ import statistics
 from collections import defaultdict
 
 BULK_KERNELS = {"k_pack_group", "k_rs_hist", "k_rs_scatter", "k_pack_count", "k_os_pass"}
+# the structural check a harness runs between batches: part of no batch
+NOT_BATCH = {"k_check"}
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
@@ -65,7 +69,12 @@ def batches(dispatches):
 def cmd_kernels(a):
     c = sqlite3.connect(a.db)
     rows = [(short(n), (s, e)) for n, s, e in c.execute("select name, start, end from kernels order by start")]
+    rows = [r for r in rows if r[0] not in NOT_BATCH]
     upd, bulk, _ = batches(rows)
+    if a.last:
+        # the last N batches in trace order, whatever their sort
+        allb = sorted(upd + bulk, key=lambda sg: sg[0][1][0])
+        upd, bulk = allb[-a.last:], allb[:-a.last]
     per = defaultdict(list)
     spans = []
     for sg in upd:
@@ -73,10 +82,14 @@ def cmd_kernels(a):
         for k, (s, e) in sg:
             per[k].append((e - s) / 1e3)
     tot = sum(sum(v) for v in per.values())
+    which = (f"the last {len(upd)} batches (each from its first kernel, `k_batch_begin` or the sort's "
+             f"`k_ss_split`, to the next batch's); the {len(bulk)} batches before them are left out"
+             if a.last else
+             f"{len(upd)} update batches (each from its first kernel, `k_batch_begin` or the sort's "
+             f"`k_ss_split`, to the next batch's); the {len(bulk)} bulk loads' batches are left out")
     lines = [f"# {a.title}", "", "Command (repo root, one MI355X):", "", "```", a.command, "```", "",
-             f"Launch durations from the kernel trace over {len(upd)} update batches (each from its first "
-             f"kernel, `k_batch_begin` or the sort's `k_ss_split`, to the next batch's); the {len(bulk)} bulk "
-             f"loads' batches are left out.  "
+             f"Launch durations from the kernel trace over {which}.  "
+             "`k_check` (a structural check run between batches) is left out.  "
              "The trace slows the launch chain (gaps between kernels), not the kernels.", "",
              "| kernel | launches | avg us | median us | p99 us | max us | total ms | share |",
              "|---|---:|---:|---:|---:|---:|---:|---:|"]
@@ -86,8 +99,9 @@ def cmd_kernels(a):
         lines.append(f"| `{k}` | {len(v)} | {sum(v) / len(v):.2f} | {statistics.median(v):.2f} | {p99:.2f} | "
                      f"{s[-1]:.2f} | {sum(v) / 1e3:.3f} | {sum(v) / tot:.3f} |")
     s = sorted(spans)
+    tail = "; the last batch's span also holds what ran after it" if a.last else ""
     lines += ["", f"Batch span (first kernel start to last kernel end, under the trace): median "
-                  f"{statistics.median(s):.1f} us, p99 {s[min(len(s) - 1, int(0.99 * len(s)))]:.1f} us.", ""]
+                  f"{statistics.median(s):.1f} us, p99 {s[min(len(s) - 1, int(0.99 * len(s)))]:.1f} us{tail}.", ""]
     with open(a.out_md, "w") as f:
         f.write("\n".join(lines))
 
@@ -191,6 +205,7 @@ def main():
     k.add_argument("out_md")
     k.add_argument("--title", default="Kernel profile")
     k.add_argument("--command", required=True)
+    k.add_argument("--last", type=int, default=0)
     t = sub.add_parser("traffic")
     t.add_argument("out_json")
     t.add_argument("out_md")
