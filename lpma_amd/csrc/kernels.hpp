@@ -65,12 +65,9 @@ hipError_t rebalance_lds(const Group &g, const BatchBufs &b, int cls, u32 grid, 
 // with events (timing): the launch records its own start and end in them
 hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStream_t st, hipEvent_t e0 = nullptr,
                          hipEvent_t e1 = nullptr);
-// grid sized for at most `bound` windows, the count read from the status on
-// the device (launches queued before the host knows it)
-hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0 = nullptr,
-                               hipEvent_t e1 = nullptr);
-// the same launch followed, in its own workgroups, by the one-wave windows
-// (a lone PMA's early rebalance: one launch for both classes)
+// single-segment and one-wave windows in one launch (a lone PMA's early
+// rebalance), its grid sized for at most `bound` windows, the counts read
+// from the status on the device (queued before the host knows them)
 hipError_t rebalance_seg_wave_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st,
                                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 u32 seg_lanes_log2(u32 log2S, int slots_log2);

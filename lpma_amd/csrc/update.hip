@@ -1241,11 +1241,6 @@ hipError_t rebalance_seg(const Group &g, const BatchBufs &b, u64 windows, hipStr
     return rebalance_seg_launch(g, b, windows, true, st, e0, e1);
 }
 
-hipError_t rebalance_seg_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0,
-                               hipEvent_t e1) {
-    return rebalance_seg_launch(g, b, bound, false, st, e0, e1);
-}
-
 hipError_t rebalance_seg_wave_bound(const Group &g, const BatchBufs &b, u64 bound, hipStream_t st, hipEvent_t e0,
                                     hipEvent_t e1) {
     return rebalance_seg_launch(g, b, bound, false, st, e0, e1, true);
