@@ -426,6 +426,7 @@ static constexpr u32 SB_NW = SB_TPB / 64;
 // a tile's count entry: offset << 16 | count, bit 31 on bucket 0's entry
 // when the tile met an invalid key
 static constexpr u32 SS_BAD = 1u << 31;
+static_assert(SS_TILE <= (1u << 15), "a tile's offsets and counts stay below the bad-key bit");
 
 // buckets of a batch of n keys: about a thousand keys each, a power of two
 static u32 ss_buckets(u64 n) {
